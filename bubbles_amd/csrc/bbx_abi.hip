@@ -1,0 +1,1082 @@
+// bbx_abi.hip -- C ABI of the MI355X SPH / PCISPH hot path: solver lifetime,
+// Setup scalars (FP64 on the host), id-ordered transfers, substep control.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "bbx_common.h"
+
+namespace bbx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+size_t scan_scratch_elems(size_t n);
+
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+int Profiler::name_id(const char *name) {
+    for (size_t i = 0; i < names.size(); i++)
+        if (names[i] == name) return (int)i;
+    names.emplace_back(name);
+    total_ms.push_back(0.0);
+    launches.push_back(0);
+    return (int)names.size() - 1;
+}
+
+static hipEvent_t pool_get(std::vector<hipEvent_t> &pool) {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void Profiler::begin(hipStream_t s, const char *, hipEvent_t *a) {
+    *a = pool_get(pool);
+    if (*a) (void)hipEventRecord(*a, s);
+}
+
+void Profiler::end(hipStream_t s, int id, hipEvent_t a) {
+    hipEvent_t b = pool_get(pool);
+    if (!a || !b) return;
+    (void)hipEventRecord(b, s);
+    pending.push_back({id, a, b});
+}
+
+int Profiler::resolve() {
+    for (auto &p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            total_ms[p.id] += ms;
+            launches[p.id] += 1;
+        }
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+    return BBX_OK;
+}
+
+void Profiler::reset() {
+    resolve();
+    for (auto &t : total_ms) t = 0.0;
+    for (auto &l : launches) l = 0;
+}
+
+void Profiler::destroy() {
+    resolve();
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Setup scalars in FP64 (host)
+// ---------------------------------------------------------------------------
+static inline double absd(double v) { return v > 0 ? v : -v; }
+static inline double maxd(double a, double b) { return a < b ? b : a; }
+static inline double mind(double a, double b) { return a < b ? a : b; }
+
+// Grid::Build after UtilBuildGridForDomain (src/core/util.cpp:269-287,
+// src/core/grid.h:626-666)
+int build_grid_geom(const bbx_params &p, GridGeom *g) {
+    double dmin[3], dmax[3];
+    for (int k = 0; k < 3; k++) {
+        double hx = p.tank_size[k] / 2.0;
+        dmin[k] = -hx + p.tank_center[k];
+        dmax[k] = hx + p.tank_center[k];
+    }
+    const double length = p.grid_spacing * p.grid_scale;
+    if (!(length > 0)) return fail(BBX_E_INVALID, "grid spacing * scale must be > 0");
+    const double hlen = 0.5 * length;
+    const double inv_len = 1.0f / length;
+    int m[3];
+    for (int k = 0; k < 3; k++) {
+        m[k] = (int)std::ceil(absd(dmax[k] - dmin[k]) * inv_len);
+        m[k] = (int)(m[k] + (m[k] % 2) * length);
+        if (m[k] <= 0) return fail(BBX_E_INVALID, "empty tank");
+    }
+    double pmin[3], pmax[3];
+    for (int k = 0; k < 3; k++) {
+        double center = (dmin[k] + dmax[k]) * 0.5;
+        pmin[k] = center - m[k] * hlen;
+        pmax[k] = center + m[k] * hlen;
+    }
+    // Build(resolution, pMin, pMax)
+    double lower[3], high[3];
+    double total = 1;
+    for (int k = 0; k < 3; k++) {
+        double dl = maxd(pmin[k], pmax[k]), du = mind(pmin[k], pmax[k]);
+        lower[k] = (double)3.402823466e+38f;
+        high[k] = -(double)3.402823466e+38f;
+        if (dl < lower[k]) lower[k] = dl;
+        if (dl > high[k]) high[k] = dl;
+        if (du > high[k]) high[k] = du;
+        if (du < lower[k]) lower[k] = du;
+    }
+    for (int k = 0; k < 3; k++) {
+        double s = high[k] - lower[k];
+        double len = s / (double)(unsigned)m[k];
+        g->len[k] = len;
+        g->n[k] = (int)std::ceil(s / len);
+        double maxp = lower[k] + (double)(unsigned)g->n[k] * len;
+        g->minp[k] = lower[k];
+        g->bmin[k] = mind(lower[k], maxp);
+        g->bmax[k] = maxd(lower[k], maxp);
+        total *= g->n[k];
+    }
+    if (total > 2.0e9) return fail(BBX_E_RANGE, "grid has more than 2^31 cells");
+    g->total = (unsigned)total;
+    return BBX_OK;
+}
+
+static double poly6_w64(double h, double distance) {
+    double h2 = h * h, h3 = h2 * h;
+    double d2 = distance * distance;
+    double of = d2 - h2;
+    if (absd(of) < 1e-8 || of > 0) return 0;
+    double x = 1.0 - d2 / h2;
+    return 315.0 / (64.0 * M_PI * h3) * x * x * x;
+}
+
+static double spiky_dw64(double h, double distance) {
+    double h4 = (h * h) * (h * h);
+    double of = distance - h;
+    if (absd(of) < 1e-8 || of > 0) return 0;
+    double x = 1.0 - distance / h;
+    return -45.0 / (M_PI * h4) * x * x;
+}
+
+// BCC lattice over [-1.5h, 1.5h]^3 (src/generator/bcclattice.cpp:5-76)
+static void bcc_points(double kr, double spacing, int maxn, std::vector<double> &pts) {
+    const double lo = -1.5 * kr, hi = 1.5 * kr;
+    const double half = spacing / 2;
+    const double ext = absd(hi - lo);
+    bool off = false, stop = false;
+    pts.clear();
+    for (int k = 0; k * half <= ext && !stop; k++) {
+        double z = k * half + lo;
+        double o = off ? half : 0;
+        for (int j = 0; j * spacing + o <= ext && !stop; j++) {
+            double y = j * spacing + o + lo;
+            for (int i = 0; i * spacing + o <= ext && !stop; i++) {
+                if (maxn >= 0 && (int)(pts.size() / 3) >= maxn) { stop = true; break; }
+                pts.push_back(i * spacing + o + lo);
+                pts.push_back(y);
+                pts.push_back(z);
+            }
+        }
+        off = !off;
+    }
+}
+
+// SphParticleSet3::ComputeMass (src/core/particle.h:557-582) and
+// PciSphSolver3::ComputeDeltaDenom (src/solvers/pcisph_solver3.cpp:148-180)
+int compute_mass_and_delta(bbx_solver *s) {
+    const double rho0 = s->prm.target_density;
+    const double spacing = s->prm.target_spacing;
+    s->radius = maxd(0, spacing);
+    s->kr = s->prm.relative_radius * spacing;
+    if (!(s->kr > 0) || !(spacing > 0)) return fail(BBX_E_INVALID, "kernel radius must be > 0");
+    std::vector<double> pts;
+    bcc_points(s->kr, spacing, 1024, pts);
+    const int c = (int)(pts.size() / 3);
+    double nden = 0;
+    for (int i = 0; i < c; i++) {
+        double sum = 0;
+        for (int j = 0; j < c; j++) {
+            double dx = pts[3 * i] - pts[3 * j], dy = pts[3 * i + 1] - pts[3 * j + 1],
+                   dz = pts[3 * i + 2] - pts[3 * j + 2];
+            sum += poly6_w64(s->kr, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        nden = maxd(nden, sum);
+    }
+    if (!(nden > 0)) return fail(BBX_E_INVALID, "zero number density");
+    s->mass = maxd(0, rho0 / nden);
+    double m = s->mass / rho0;
+    m *= m;
+    s->mass_over_rho2 = m;
+    bcc_points(s->kr, spacing, -1, pts);
+    const int np = (int)(pts.size() / 3);
+    const double kr2 = s->kr * s->kr;
+    double d1x = 0, d1y = 0, d1z = 0, denom2 = 0;
+    for (int i = 0; i < np; i++) {
+        double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+        double d2 = px * px + py * py + pz * pz;
+        if (d2 < kr2) {
+            double dist = std::sqrt(d2);
+            double dx = 0, dy = 0, dz = 0;
+            if (dist > 0) {
+                double inv = (double)1 / dist;
+                dx = px * inv; dy = py * inv; dz = pz * inv;
+            }
+            double w = -spiky_dw64(s->kr, dist);
+            double gx = dx * w, gy = dy * w, gz = dz * w;
+            d1x += gx; d1y += gy; d1z += gz;
+            denom2 += gx * gx + gy * gy + gz * gz;
+        }
+    }
+    double denom = 0;
+    denom += -(d1x * d1x + d1y * d1y + d1z * d1z) - denom2;
+    s->delta_denom = denom;
+    return BBX_OK;
+}
+
+// PciSphSolver3::ComputeDelta (src/solvers/pcisph_solver3.cpp:182-189)
+static double compute_delta(const bbx_solver *s, double dt) {
+    double tss = dt * dt;
+    double beta = 2.0 * s->mass_over_rho2 * tss;
+    return absd(s->delta_denom) > 0 ? -1 / (beta * s->delta_denom) : 0;
+}
+
+void fill_sim_const(bbx_solver *s) {
+    SimConst &c = s->sc;
+    c.n = s->n;
+    c.stride = s->stride;
+    c.g = s->grid;
+    const double h = s->kr;
+    c.h = (float)h;
+    c.h2 = (float)(h * h);
+    c.inv_h = (float)(1.0 / h);
+    c.inv_h2 = (float)(1.0 / (h * h));
+    c.poly6 = (float)(315.0 / (64.0 * M_PI * h * h * h));
+    c.spiky_grad = (float)(45.0 / (M_PI * h * h * h * h));
+    c.spiky_lap = (float)(90.0 / (M_PI * h * h * h * h * h));
+    c.h2d = h * h;
+    // FP32 prefilter band: outside [T(1-4e-6), T(1+4e-6)] with T = h^2 - 1e-8 the
+    // FP32 squared distance (relative error < 5 * 2^-24) decides; inside it the
+    // FP64 rule decides.
+    const double T = h * h - 1e-8;
+    if (T > 1e-6) {
+        c.lo2 = std::nextafter((float)(T * (1.0 - 4e-6)), 0.f);
+        c.hi2 = std::nextafter((float)(T * (1.0 + 4e-6)), 3.4e38f);
+    } else {
+        c.lo2 = -1.f;         // always take the FP64 path
+        c.hi2 = 3.4e38f;
+    }
+    c.mass = (float)s->mass;
+    c.mass2 = (float)(s->mass * s->mass);
+    c.rho0 = (float)s->prm.target_density;
+    c.visc = (float)s->prm.viscosity;
+    c.drag = (float)s->prm.drag_coefficient;
+    c.nps = (float)s->prm.negative_pressure_scale;
+    c.gx = (float)s->prm.gravity[0];
+    c.gy = (float)s->prm.gravity[1];
+    c.gz = (float)s->prm.gravity[2];
+    c.radius = (float)s->radius;
+    for (int k = 0; k < 3; k++) {
+        c.tank_c[k] = (float)s->prm.tank_center[k];
+        c.tank_h[k] = (float)(s->prm.tank_size[k] / 2.0);
+        c.bmin[k] = (float)s->grid.bmin[k];
+        c.bmax[k] = (float)s->grid.bmax[k];
+    }
+    c.min_len = (float)mind(s->grid.len[0], mind(s->grid.len[1], s->grid.len[2]));
+    c.eos_scale = (float)(s->prm.target_density * s->prm.sound_speed * s->prm.sound_speed);
+    c.eos_exp = (float)s->prm.eos_exponent;
+    c.pseudo_visc = (float)s->prm.pseudo_viscosity;
+}
+
+// host-side hash: Grid::GetLinearHashedPosition (src/core/grid.h:258-321)
+static long long host_hash(const GridGeom &g, const double p[3]) {
+    int u[3];
+    for (int i = 0; i < 3; i++) {
+        double pi = p[i];
+        double eps = 0;
+        if (absd(pi - g.bmin[i]) < 1e-8) eps = (double)0.0001f;
+        else if (absd(pi - g.bmax[i]) < 1e-8) eps = -(double)0.0001f;
+        pi += eps;
+        double f = std::floor((pi - g.minp[i]) / g.len[i]);
+        if (!(f >= 0) || f >= (double)g.n[i]) return -1;
+        u[i] = (int)f;
+    }
+    return (long long)u[0] + (long long)u[1] * g.n[0] + (long long)u[2] * g.n[0] * g.n[1];
+}
+
+}  // namespace bbx
+
+using namespace bbx;
+
+// ---------------------------------------------------------------------------
+// allocation helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(BBX_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return BBX_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static void free_all(bbx_solver *s) {
+    dfree(s->posA); dfree(s->posB); dfree(s->velA); dfree(s->velB);
+    dfree(s->cellA); dfree(s->cellB); dfree(s->home);
+    dfree(s->cell_count); dfree(s->cell_start); dfree(s->scan_tmp);
+    dfree(s->key); dfree(s->newc); dfree(s->rank_tmp); dfree(s->tmp_idx); dfree(s->hash_tmp);
+    dfree(s->nbr); dfree(s->nbr_cnt);
+    dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->q);
+    dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
+    dfree(s->max_err); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
+    if (s->h_max_err) (void)hipHostFree(s->h_max_err);
+    if (s->h_counters) (void)hipHostFree(s->h_counters);
+    if (s->h_max_force) (void)hipHostFree(s->h_max_force);
+    s->h_max_err = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    s->ev0 = s->ev1 = nullptr;
+    s->prof.destroy();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->stream = nullptr;
+}
+
+static int check_solver(bbx_solver *s) {
+    if (!s) return fail(BBX_E_INVALID, "null solver");
+    return BBX_OK;
+}
+
+static int check_setup(bbx_solver *s) {
+    BBX_CHECK(check_solver(s));
+    if (!s->setup_done) return fail(BBX_E_INVALID, "call bbx_solver_setup first");
+    return BBX_OK;
+}
+
+// download the slot -> id map (pos.w of the current storage)
+static int slot_ids(bbx_solver *s, std::vector<int> &ids, std::vector<float> *pos4 = nullptr) {
+    std::vector<float> tmp((size_t)s->n * 4);
+    BBX_HIP_CHECK(hipMemcpyAsync(tmp.data(), s->posA, sizeof(float4) * s->n, hipMemcpyDeviceToHost,
+                                 s->stream));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    ids.resize(s->n);
+    for (int i = 0; i < s->n; i++) {
+        int id;
+        std::memcpy(&id, &tmp[4 * (size_t)i + 3], sizeof(int));
+        if (id < 0 || id >= s->n) return fail(BBX_E_INVALID, "corrupt particle id table");
+        ids[i] = id;
+    }
+    if (pos4) pos4->swap(tmp);
+    return BBX_OK;
+}
+
+extern "C" {
+
+const char *bbx_last_error(void) { return g_last_error.c_str(); }
+
+int bbx_version(int *major, int *minor) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return BBX_OK;
+}
+
+int bbx_device_count(int *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    if (count) *count = n;
+    return BBX_OK;
+}
+
+int bbx_default_params(bbx_params *p) {
+    if (!p) return fail(BBX_E_INVALID, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->target_density = 1000.0;
+    p->target_spacing = 0.1;
+    p->relative_radius = 2.0;
+    p->tank_size[0] = p->tank_size[1] = p->tank_size[2] = 1.0;
+    p->grid_spacing = 0.1;
+    p->grid_scale = 2.0;
+    p->eos_exponent = 7.0;
+    p->negative_pressure_scale = 0.0;
+    p->viscosity = 0.04;
+    p->pseudo_viscosity = 10.0;
+    p->sound_speed = 100.0;
+    p->drag_coefficient = 0.0001;
+    p->gravity[1] = (double)(-9.8f);
+    return BBX_OK;
+}
+
+int bbx_solver_create(int kind, const bbx_params *params, const double *pos, const double *vel,
+                      int n, bbx_solver **out) {
+    if (!out || !params) return fail(BBX_E_INVALID, "null argument");
+    *out = nullptr;
+    if (kind != BBX_PCISPH && kind != BBX_SPH) return fail(BBX_E_INVALID, "unknown solver kind");
+    if (n < 0 || (n > 0 && !pos)) return fail(BBX_E_INVALID, "bad particle array");
+    if (n >= (1 << 28)) return fail(BBX_E_RANGE, "more than 2^28 particles");
+    int ndev = 0;
+    bbx_device_count(&ndev);
+    if (ndev <= 0) return fail(BBX_E_NODEVICE, "no HIP device visible");
+    bbx_solver *s = new bbx_solver();
+    s->kind = kind;
+    s->prm = *params;
+    s->n = n;
+    s->stride = ((n + 63) / 64) * 64;
+    if (s->stride == 0) s->stride = 64;
+    int rc = build_grid_geom(s->prm, &s->grid);
+    if (rc != BBX_OK) { delete s; return rc; }
+    hipError_t he = hipGetDevice(&s->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete s;
+        return fail(BBX_E_HIP, std::string("stream create: ") + hipGetErrorString(he));
+    }
+    const size_t N = (size_t)s->stride;
+    const size_t cells = (size_t)s->grid.total + 1;
+    s->scan_tmp_elems = scan_scratch_elems(cells);
+    rc = BBX_OK;
+    if (!rc) rc = dalloc(&s->posA, N);
+    if (!rc) rc = dalloc(&s->posB, N);
+    if (!rc) rc = dalloc(&s->velA, N);
+    if (!rc) rc = dalloc(&s->velB, N);
+    if (!rc) rc = dalloc(&s->cellA, N);
+    if (!rc) rc = dalloc(&s->cellB, N);
+    if (!rc) rc = dalloc(&s->home, N);
+    if (!rc) rc = dalloc(&s->cell_count, cells);
+    if (!rc) rc = dalloc(&s->cell_start, cells);
+    if (!rc) rc = dalloc(&s->scan_tmp, s->scan_tmp_elems);
+    if (!rc) rc = dalloc(&s->key, N);
+    if (!rc) rc = dalloc(&s->newc, N);
+    if (!rc) rc = dalloc(&s->rank_tmp, N);
+    if (!rc) rc = dalloc(&s->tmp_idx, N);
+    if (!rc) rc = dalloc(&s->hash_tmp, N);
+    if (!rc) rc = dalloc(&s->nbr, N * kBucket);
+    if (!rc) rc = dalloc(&s->nbr_cnt, N);
+    if (!rc) rc = dalloc(&s->rho, N);
+    if (!rc) rc = dalloc(&s->pres, N);
+    if (!rc) rc = dalloc(&s->rho_star, N);
+    if (!rc) rc = dalloc(&s->q, N);
+    if (!rc) rc = dalloc(&s->force, N);
+    if (!rc) rc = dalloc(&s->fp, N);
+    if (!rc) rc = dalloc(&s->xstar, N);
+    if (!rc) rc = dalloc(&s->feat, N * BBX_NFEAT);
+    if (!rc) rc = dalloc(&s->pwarm, N);
+    if (!rc) rc = dalloc(&s->max_err, 1);
+    if (!rc) rc = dalloc(&s->max_force, 1);
+    if (!rc) rc = dalloc(&s->flag, 1);
+    if (!rc) rc = dalloc(&s->counters, CNT_N);
+    if (!rc && hipHostMalloc((void **)&s->h_max_err, sizeof(unsigned long long)) != hipSuccess)
+        rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * CNT_N) != hipSuccess)
+        rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned)) != hipSuccess)
+        rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc && (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess))
+        rc = fail(BBX_E_HIP, "event create");
+    if (rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    // upload: FP64 -> FP32, id in pos.w, velocity w = 0 (no previous pressure)
+    std::vector<float4> hp(N), hv(N);
+    for (int i = 0; i < n; i++) {
+        float idf;
+        std::memcpy(&idf, &i, sizeof(int));
+        hp[i] = make_float4((float)pos[3 * i], (float)pos[3 * i + 1], (float)pos[3 * i + 2], idf);
+        hv[i] = vel ? make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], 0.f)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    hipError_t e = hipMemcpy(s->posA, hp.data(), sizeof(float4) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->velA, hv.data(), sizeof(float4) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->force, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(s->fp, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(s->xstar, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(s->rho, 0, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMemset(s->pres, 0, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMemset(s->rho_star, 0, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMemset(s->q, 0, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMemset(s->nbr_cnt, 0, sizeof(int) * N);
+    if (e == hipSuccess) e = hipMemset(s->feat, 0, sizeof(float) * N * BBX_NFEAT);
+    if (e == hipSuccess) e = hipMemset(s->max_force, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(s->max_err, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s->counters, 0, sizeof(int) * CNT_N);
+    if (e != hipSuccess) {
+        free_all(s);
+        delete s;
+        return fail(BBX_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    // every slot starts in no chain until Setup distributes
+    std::vector<int> none(N, (int)s->grid.total);
+    e = hipMemcpy(s->cellA, none.data(), sizeof(int) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->home, none.data(), sizeof(int) * N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_all(s);
+        delete s;
+        return fail(BBX_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    fill_sim_const(s);
+    *out = s;
+    return BBX_OK;
+}
+
+int bbx_solver_destroy(bbx_solver *s) {
+    if (!s) return BBX_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return BBX_OK;
+}
+
+int bbx_solver_setup(bbx_solver *s) {
+    BBX_CHECK(check_solver(s));
+    BBX_CHECK(compute_mass_and_delta(s));
+    fill_sim_const(s);
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    // full distribution into ascending-id chains; buckets come with the first substep
+    BBX_CHECK(distribute(s, true, false, false));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->setup_done = true;
+    return BBX_OK;
+}
+
+int bbx_solver_info(bbx_solver *s, bbx_info *out) {
+    BBX_CHECK(check_solver(s));
+    if (!out) return fail(BBX_E_INVALID, "null info");
+    out->count = s->n;
+    out->mass = s->mass;
+    out->kernel_radius = s->kr;
+    out->radius = s->radius;
+    out->delta_denom = s->delta_denom;
+    for (int k = 0; k < 3; k++) {
+        out->grid.usizes[k] = (unsigned)s->grid.n[k];
+        out->grid.cells_len[k] = s->grid.len[k];
+        out->grid.min_point[k] = s->grid.minp[k];
+        out->grid.bounds_min[k] = s->grid.bmin[k];
+        out->grid.bounds_max[k] = s->grid.bmax[k];
+    }
+    out->grid.total = s->grid.total;
+    return BBX_OK;
+}
+
+int bbx_solver_download(bbx_solver *s, int field, double *out) {
+    BBX_CHECK(check_solver(s));
+    if (!out) return fail(BBX_E_INVALID, "null output");
+    if (s->n == 0) return BBX_OK;
+    std::vector<int> ids;
+    BBX_CHECK(slot_ids(s, ids));
+    const size_t n = (size_t)s->n;
+    auto vec4 = [&](const float4 *src) -> int {
+        std::vector<float4> h(n);
+        BBX_HIP_CHECK(hipMemcpy(h.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            double *o = &out[3 * (size_t)ids[i]];
+            o[0] = h[i].x; o[1] = h[i].y; o[2] = h[i].z;
+        }
+        return BBX_OK;
+    };
+    auto scal = [&](const float *src) -> int {
+        std::vector<float> h(n);
+        BBX_HIP_CHECK(hipMemcpy(h.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) out[ids[i]] = h[i];
+        return BBX_OK;
+    };
+    switch (field) {
+        case BBX_FIELD_POSITION: return vec4(s->posA);
+        case BBX_FIELD_VELOCITY: return vec4(s->velA);
+        case BBX_FIELD_FORCE: return vec4(s->force);
+        case BBX_FIELD_PRESSURE_FORCE: return vec4(s->fp);
+        case BBX_FIELD_TEMP_POSITION: return vec4(s->xstar);
+        case BBX_FIELD_DENSITY: return scal(s->rho);
+        case BBX_FIELD_PRESSURE: return scal(s->pres);
+        case BBX_FIELD_DENSITY_PREDICTED: return scal(s->rho_star);
+        case BBX_FIELD_FEATURES: {
+            std::vector<float> h(n * BBX_NFEAT);
+            for (int f = 0; f < BBX_NFEAT; f++)
+                BBX_HIP_CHECK(hipMemcpy(&h[f * n], s->feat + (size_t)f * s->stride, sizeof(float) * n,
+                                        hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; i++)
+                for (int f = 0; f < BBX_NFEAT; f++) out[(size_t)ids[i] * BBX_NFEAT + f] = h[f * n + i];
+            return BBX_OK;
+        }
+        default: return fail(BBX_E_INVALID, "unknown field");
+    }
+}
+
+int bbx_solver_upload(bbx_solver *s, int field, const double *in) {
+    BBX_CHECK(check_solver(s));
+    if (!in) return fail(BBX_E_INVALID, "null input");
+    if (s->n == 0) return BBX_OK;
+    std::vector<int> ids;
+    const size_t n = (size_t)s->n;
+    if (field == BBX_FIELD_POSITION || field == BBX_FIELD_VELOCITY) {
+        float4 *dst = field == BBX_FIELD_POSITION ? s->posA : s->velA;
+        std::vector<float4> h(n);
+        BBX_CHECK(slot_ids(s, ids));
+        BBX_HIP_CHECK(hipMemcpy(h.data(), dst, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            const double *v = &in[3 * (size_t)ids[i]];
+            h[i].x = (float)v[0]; h[i].y = (float)v[1]; h[i].z = (float)v[2];
+        }
+        BBX_HIP_CHECK(hipMemcpy(dst, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        return BBX_OK;
+    }
+    if (field == BBX_FIELD_PRESSURE) {
+        BBX_CHECK(slot_ids(s, ids));
+        std::vector<float4> h(n);
+        std::vector<float> p(n);
+        BBX_HIP_CHECK(hipMemcpy(h.data(), s->velA, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            p[i] = (float)in[ids[i]];
+            h[i].w = p[i];
+        }
+        BBX_HIP_CHECK(hipMemcpy(s->velA, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        BBX_HIP_CHECK(hipMemcpy(s->pres, p.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        return BBX_OK;
+    }
+    return fail(BBX_E_INVALID, "field cannot be uploaded");
+}
+
+int bbx_solver_get_flag(bbx_solver *s, int *flag) {
+    BBX_CHECK(check_solver(s));
+    if (!flag) return fail(BBX_E_INVALID, "null flag");
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    *flag = s->h_counters[0];
+    return BBX_OK;
+}
+
+int bbx_solver_set_flag(bbx_solver *s, int flag) {
+    BBX_CHECK(check_solver(s));
+    BBX_CHECK(launch_fill_int(s, s->flag, flag ? 1 : 0, 1));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return BBX_OK;
+}
+
+static int read_stats(bbx_solver *s, bbx_stats *st) {
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->counters, sizeof(int) * CNT_N, hipMemcpyDeviceToHost,
+                                 s->stream));
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_err, s->max_err, sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, s->stream));
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                 s->stream));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (st) {
+        st->full_redistribution = s->h_counters[CNT_FULL];
+        st->bucket_overflows = s->h_counters[CNT_OVERFLOW];
+        st->lost_particles = s->h_counters[CNT_LOST];
+        double e;
+        std::memcpy(&e, s->h_max_err, sizeof(double));
+        st->max_density_error = e / s->prm.target_density;
+    }
+    return BBX_OK;
+}
+
+int bbx_grid_distribute(bbx_solver *s, bbx_stats *stats) {
+    BBX_CHECK(check_setup(s));
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(distribute(s, false, true, s->kind == BBX_SPH));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    BBX_CHECK(read_stats(s, stats));
+    if (stats) stats->max_density_error = 0;
+    if (s->prof.enabled) s->prof.resolve();
+    return BBX_OK;
+}
+
+int bbx_grid_get_chains(bbx_solver *s, int *cell_start, int *members) {
+    BBX_CHECK(check_solver(s));
+    if (!cell_start || !members) return fail(BBX_E_INVALID, "null output");
+    const size_t cells = (size_t)s->grid.total + 1;
+    std::vector<int> start(cells);
+    BBX_HIP_CHECK(hipMemcpy(start.data(), s->cell_start, sizeof(int) * cells, hipMemcpyDeviceToHost));
+    std::vector<int> ids;
+    if (s->n > 0) BBX_CHECK(slot_ids(s, ids));
+    std::memcpy(cell_start, start.data(), sizeof(int) * cells);
+    const int in_chains = start[cells - 1];
+    for (int i = 0; i < in_chains; i++) members[i] = ids[i];
+    return BBX_OK;
+}
+
+int bbx_grid_set_chains(bbx_solver *s, const int *cell_start, const int *members) {
+    BBX_CHECK(check_solver(s));
+    if (!cell_start || (!members && s->n > 0)) return fail(BBX_E_INVALID, "null input");
+    const unsigned total = s->grid.total;
+    const int in_chains = cell_start[total];
+    if (in_chains < 0 || in_chains > s->n) return fail(BBX_E_INVALID, "bad chain CSR");
+    const size_t n = (size_t)s->n;
+    std::vector<int> ids;
+    std::vector<float> pos4;
+    if (n) BBX_CHECK(slot_ids(s, ids, &pos4));
+    std::vector<float4> vel(n);
+    if (n) BBX_HIP_CHECK(hipMemcpy(vel.data(), s->velA, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    std::vector<int> slot_of(n, -1);
+    for (size_t i = 0; i < n; i++) slot_of[ids[i]] = (int)i;
+    std::vector<float4> np(n), nv(n);
+    std::vector<int> ncell(n, (int)total);
+    std::vector<char> used(n, 0);
+    size_t at = 0;
+    for (unsigned c = 0; c < total; c++) {
+        if (cell_start[c + 1] < cell_start[c]) return fail(BBX_E_INVALID, "bad chain CSR");
+        for (int k = cell_start[c]; k < cell_start[c + 1]; k++) {
+            int id = members[k];
+            if (id < 0 || id >= s->n || used[id]) return fail(BBX_E_INVALID, "bad chain member");
+            used[id] = 1;
+            int sl = slot_of[id];
+            std::memcpy(&np[at], &pos4[4 * (size_t)sl], sizeof(float4));
+            nv[at] = vel[sl];
+            ncell[at] = (int)c;
+            at++;
+        }
+    }
+    for (int id = 0; id < s->n; id++) {
+        if (used[id]) continue;
+        int sl = slot_of[id];
+        std::memcpy(&np[at], &pos4[4 * (size_t)sl], sizeof(float4));
+        nv[at] = vel[sl];
+        at++;
+    }
+    if (n) {
+        BBX_HIP_CHECK(hipMemcpy(s->posA, np.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        BBX_HIP_CHECK(hipMemcpy(s->velA, nv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        BBX_HIP_CHECK(hipMemcpy(s->cellA, ncell.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
+    // cell table for the new order (buckets are rebuilt by the next distribute)
+    std::vector<int> cnt(total + 1, 0), st(total + 1, 0);
+    for (unsigned c = 0; c < total; c++) cnt[c] = cell_start[c + 1] - cell_start[c];
+    cnt[total] = s->n - in_chains;
+    for (unsigned c = 0; c <= total; c++) st[c] = c < total ? cell_start[c] : in_chains;
+    BBX_HIP_CHECK(hipMemcpy(s->cell_count, cnt.data(), sizeof(int) * (total + 1), hipMemcpyHostToDevice));
+    BBX_HIP_CHECK(hipMemcpy(s->cell_start, st.data(), sizeof(int) * (total + 1), hipMemcpyHostToDevice));
+    s->setup_done = true;
+    return BBX_OK;
+}
+
+int bbx_grid_hash(bbx_solver *s, const double *points, int n, int *cell_ids) {
+    BBX_CHECK(check_solver(s));
+    if (n < 0 || (n > 0 && (!points || !cell_ids))) return fail(BBX_E_INVALID, "bad arrays");
+    for (int i = 0; i < n; i++) cell_ids[i] = (int)host_hash(s->grid, &points[3 * (size_t)i]);
+    return BBX_OK;
+}
+
+int bbx_grid_cell_neighbors(bbx_solver *s, int cell, int *out27, int *count) {
+    BBX_CHECK(check_solver(s));
+    if (!out27 || !count) return fail(BBX_E_INVALID, "null output");
+    if (cell < 0 || (unsigned)cell >= s->grid.total) return fail(BBX_E_INVALID, "cell out of range");
+    const int nx = s->grid.n[0], ny = s->grid.n[1], nz = s->grid.n[2];
+    int u2 = cell / (nx * ny), rem = cell - u2 * nx * ny, u1 = rem / nx, u0 = rem - u1 * nx;
+    int c = 0;
+    for (int j = u1 - 1; j <= u1 + 1; j++) {
+        if (j < 0 || j >= ny) continue;
+        for (int i = u0 - 1; i <= u0 + 1; i++) {
+            if (i < 0 || i >= nx) continue;
+            for (int k = u2 - 1; k <= u2 + 1; k++) {
+                if (k < 0 || k >= nz) continue;
+                out27[c++] = i + j * nx + k * nx * ny;
+            }
+        }
+    }
+    *count = c;
+    return BBX_OK;
+}
+
+int bbx_neighbors_get(bbx_solver *s, int *counts, int *out_ids) {
+    BBX_CHECK(check_solver(s));
+    if (!counts) return fail(BBX_E_INVALID, "null output");
+    const size_t n = (size_t)s->n;
+    if (!n) return BBX_OK;
+    std::vector<int> ids;
+    BBX_CHECK(slot_ids(s, ids));
+    std::vector<int> cnt(n);
+    BBX_HIP_CHECK(hipMemcpy(cnt.data(), s->nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) counts[ids[i]] = cnt[i];
+    if (out_ids) {
+        std::vector<int> row(n);
+        for (int k = 0; k < kBucket; k++) {
+            BBX_HIP_CHECK(hipMemcpy(row.data(), s->nbr + (size_t)k * s->stride, sizeof(int) * n,
+                                    hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; i++) {
+                int v = -1;
+                if (k < cnt[i]) {
+                    int j = row[i];
+                    v = (j >= 0 && (size_t)j < n) ? ids[j] : -2;
+                }
+                out_ids[(size_t)ids[i] * kBucket + k] = v;
+            }
+        }
+    }
+    return BBX_OK;
+}
+
+int bbx_stage_density(bbx_solver *s) {
+    BBX_CHECK(check_setup(s));
+    BBX_CHECK(run_density_only(s));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return BBX_OK;
+}
+
+int bbx_stage_nonpressure(bbx_solver *s) {
+    BBX_CHECK(check_setup(s));
+    StepConst st{};
+    BBX_CHECK(run_nonpressure(s, st, 0, 0));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return BBX_OK;
+}
+
+int bbx_pcisph_delta(bbx_solver *s, double dt, double *delta) {
+    BBX_CHECK(check_setup(s));
+    if (!delta) return fail(BBX_E_INVALID, "null output");
+    *delta = compute_delta(s, dt);
+    return BBX_OK;
+}
+
+static int max_force_host(bbx_solver *s, double *mf) {
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                 s->stream));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    float f;
+    std::memcpy(&f, s->h_max_force, sizeof(float));
+    *mf = (double)f;
+    return BBX_OK;
+}
+
+// SphParticleSet3::ComputeNumberOfTimeSteps (src/core/particle.h:584-608)
+static int time_steps(bbx_solver *s, double interval, double scale, unsigned *out) {
+    double maxf = 0;
+    BBX_CHECK(max_force_host(s, &maxf));
+    double ref = s->kr;
+    double limit = 0.40 * ref / s->prm.sound_speed;
+    if (!(absd(maxf) < 1e-8)) {
+        double by_force = 0.25 * std::sqrt(ref * s->mass / maxf);
+        limit = mind(by_force, limit);
+    }
+    double target = scale * limit;
+    *out = (unsigned)std::ceil(interval / target);
+    return BBX_OK;
+}
+
+int bbx_number_of_time_steps(bbx_solver *s, double interval, double scale, unsigned *n) {
+    BBX_CHECK(check_setup(s));
+    if (!n) return fail(BBX_E_INVALID, "null output");
+    return time_steps(s, interval, scale, n);
+}
+
+int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
+                       bbx_stats *stats) {
+    BBX_CHECK(check_setup(s));
+    if (s->kind != BBX_PCISPH) return fail(BBX_E_INVALID, "not a PCISPH solver");
+    if (max_it < 1) return fail(BBX_E_INVALID, "max_it must be >= 1");
+    if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
+    if (warm == BBX_WARM_MLP && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+    BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, 1));
+    // UpdateGridDistribution + ResetHigherLevel + interaction + density
+    BBX_CHECK(distribute(s, false, true, false));
+    const double delta = compute_delta(s, dt);
+    s->delta64 = delta;
+    StepConst st{};
+    st.dt = (float)dt;
+    st.dt_over_m = (float)(dt / s->mass);
+    st.delta = (float)delta;
+    if (warm == BBX_WARM_MLP) {
+        BBX_CHECK(run_nonpressure(s, st, 0, 1));
+        BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride, s->pwarm, s->stream));
+        BBX_CHECK(run_warm_init(s));
+        // f_p(p_warm, rho) and the k = 0 prediction
+        BBX_CHECK(run_pressure_force(s, st, 0));
+    } else {
+        BBX_CHECK(run_nonpressure(s, st, 1, 0));
+    }
+    int it = 0;
+    double ratio = 0;
+    for (int k = 0; k < max_it; k++) {
+        st.first = (k == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
+        BBX_CHECK(run_predict_pressure(s, st));
+        it = k + 1;
+        bool last = (mode == BBX_MODE_REFERENCE_EXACT) || (k == max_it - 1);
+        if (!last) {
+            BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_err, s->max_err, sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s->stream));
+            BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+            double e;
+            std::memcpy(&e, s->h_max_err, sizeof(double));
+            ratio = e / s->prm.target_density;
+            if (absd(ratio) < tol) last = true;
+        }
+        BBX_CHECK(run_pressure_force(s, st, last ? 1 : 0));
+        if (last) break;
+    }
+    BBX_CHECK(swap_after_integrate(s));
+    BBX_CHECK(run_xsph(s, st));
+    BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    bbx_stats local{};
+    BBX_CHECK(read_stats(s, &local));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    local.iterations = it;
+    local.gpu_ms = ms;
+    if (stats) *stats = local;
+    if (s->prof.enabled) s->prof.resolve();
+    return BBX_OK;
+}
+
+int bbx_pcisph_advance(bbx_solver *s, double interval, int max_it, double tol, int mode, int warm,
+                       int *substeps) {
+    BBX_CHECK(check_setup(s));
+    double remaining = interval;
+    int steps = 0;
+    while (remaining > (double)0.0001f) {
+        unsigned n = 0;
+        BBX_CHECK(time_steps(s, remaining, 5.0, &n));
+        if (n == 0) return fail(BBX_E_INVALID, "time-step rule returned 0");
+        double dt = remaining / (double)n;
+        BBX_CHECK(bbx_pcisph_substep(s, dt, max_it, tol, mode, warm, nullptr));
+        remaining -= dt;
+        steps++;
+    }
+    if (substeps) *substeps = steps;
+    return BBX_OK;
+}
+
+int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
+    BBX_CHECK(check_setup(s));
+    if (s->kind != BBX_SPH) return fail(BBX_E_INVALID, "not an SPH solver");
+    if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
+    BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, 1));
+    BBX_CHECK(distribute(s, false, true, true));
+    StepConst st{};
+    st.dt = (float)dt;
+    st.dt_over_m = (float)(dt / s->mass);
+    BBX_CHECK(run_sph_force(s));
+    BBX_CHECK(run_integrate(s, st));
+    BBX_CHECK(swap_after_integrate(s));
+    BBX_CHECK(run_xsph(s, st));
+    BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    bbx_stats local{};
+    BBX_CHECK(read_stats(s, &local));
+    local.max_density_error = 0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    local.gpu_ms = ms;
+    if (stats) *stats = local;
+    if (s->prof.enabled) s->prof.resolve();
+    return BBX_OK;
+}
+
+int bbx_sph_advance(bbx_solver *s, double interval, int *substeps) {
+    BBX_CHECK(check_setup(s));
+    double remaining = interval;
+    int steps = 0;
+    while (remaining > (double)0.0001f) {
+        unsigned n = 0;
+        BBX_CHECK(time_steps(s, remaining, 1.0, &n));
+        if (n == 0) return fail(BBX_E_INVALID, "time-step rule returned 0");
+        double dt = remaining / (double)n;
+        BBX_CHECK(bbx_sph_substep(s, dt, nullptr));
+        remaining -= dt;
+        steps++;
+    }
+    if (substeps) *substeps = steps;
+    return BBX_OK;
+}
+
+int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m) {
+    BBX_CHECK(check_solver(s));
+    s->mlp = m;
+    return BBX_OK;
+}
+
+int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,
+                       float *targets, bbx_stats *stats) {
+    BBX_CHECK(check_setup(s));
+    if (!features || !targets) return fail(BBX_E_INVALID, "null output");
+    const size_t n = (size_t)s->n;
+    // features at the start of the substep: distribute + density + non-pressure
+    // with feature output, then the plain substep from the same state.
+    // The features are computed on a throwaway pass that leaves the state
+    // untouched (distribute is idempotent for unchanged positions).
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(distribute(s, false, true, false));
+    const double delta = compute_delta(s, dt);
+    StepConst st{};
+    st.dt = (float)dt;
+    st.dt_over_m = (float)(dt / s->mass);
+    st.delta = (float)delta;
+    BBX_CHECK(run_nonpressure(s, st, 0, 1));
+    std::vector<int> ids;
+    BBX_CHECK(slot_ids(s, ids));
+    std::vector<float> h(n * BBX_NFEAT);
+    for (int f = 0; f < BBX_NFEAT; f++)
+        BBX_HIP_CHECK(hipMemcpy(&h[f * n], s->feat + (size_t)f * s->stride, sizeof(float) * n,
+                                hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++)
+        for (int f = 0; f < BBX_NFEAT; f++) features[(size_t)ids[i] * BBX_NFEAT + f] = h[f * n + i];
+    bbx_stats local{};
+    BBX_CHECK(bbx_pcisph_substep(s, dt, max_it, tol, BBX_MODE_TEXTBOOK, BBX_WARM_NONE, &local));
+    // converged pressure / (delta rho0), by id (slots moved only inside the substep's distribute)
+    BBX_CHECK(slot_ids(s, ids));
+    std::vector<float> p(n);
+    BBX_HIP_CHECK(hipMemcpy(p.data(), s->pres, sizeof(float) * n, hipMemcpyDeviceToHost));
+    const double pref = delta * s->prm.target_density;
+    for (size_t i = 0; i < n; i++) targets[ids[i]] = pref != 0 ? (float)(p[i] / pref) : 0.f;
+    if (stats) *stats = local;
+    return BBX_OK;
+}
+
+int bbx_profile_enable(bbx_solver *s, int enable) {
+    BBX_CHECK(check_solver(s));
+    s->prof.enabled = enable != 0;
+    return BBX_OK;
+}
+
+int bbx_profile_reset(bbx_solver *s) {
+    BBX_CHECK(check_solver(s));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->prof.reset();
+    return BBX_OK;
+}
+
+int bbx_profile_get(bbx_solver *s, char *names, int names_cap, double *total_ms, int *launches,
+                    int cap, int *count) {
+    BBX_CHECK(check_solver(s));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->prof.resolve();
+    int k = (int)s->prof.names.size();
+    if (count) *count = k;
+    std::string joined;
+    for (int i = 0; i < k; i++) {
+        if (i < cap) {
+            if (total_ms) total_ms[i] = s->prof.total_ms[i];
+            if (launches) launches[i] = s->prof.launches[i];
+        }
+        joined += s->prof.names[i];
+        joined += '\n';
+    }
+    if (names && names_cap > 0) {
+        std::strncpy(names, joined.c_str(), (size_t)names_cap - 1);
+        names[names_cap - 1] = '\0';
+    }
+    return BBX_OK;
+}
+
+int bbx_solver_synchronize(bbx_solver *s) {
+    BBX_CHECK(check_solver(s));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return BBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// bbx_common.h -- internal declarations shared by the HIP translation units
+// of the MI355X SPH / PCISPH hot path (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bubbles_amd.h"
+#include "bbx_features.h"
+
+namespace bbx {
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define BBX_HIP_CHECK(expr)                                                         \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::bbx::fail(BBX_E_HIP, std::string(#expr) + ": " +               \
+                                              hipGetErrorString(e_));               \
+    } while (0)
+
+#define BBX_CHECK(expr)                                                             \
+    do {                                                                            \
+        int r_ = (expr);                                                            \
+        if (r_ != BBX_OK) return r_;                                                \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// constants shared by host and device
+// ---------------------------------------------------------------------------
+constexpr int kBucket = BBX_BUCKET_SIZE;   // reference bucket cap
+constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
+constexpr int kScanItems = 8;              // items per thread in the cell scan
+
+// Grid geometry in FP64 (the hash is computed in the reference's precision).
+struct GridGeom {
+    int n[3];
+    unsigned total;
+    double len[3];
+    double minp[3];
+    double bmin[3];
+    double bmax[3];
+};
+
+// Per-solver scalars passed to kernels by value.
+struct SimConst {
+    int n;             // particles
+    int stride;        // neighbour-list row stride (>= n, multiple of 64)
+    GridGeom g;
+    // kernel radius and derived FP32 coefficients
+    float h, h2, inv_h, inv_h2;
+    float poly6;       // 315 / (64 pi h^3)
+    float spiky_grad;  // 45 / (pi h^4)
+    float spiky_lap;   // 90 / (pi h^5)
+    double h2d;        // FP64 kernel radius squared for the bucket rule
+    float lo2, hi2;    // FP32 prefilter band around the FP64 cutoff
+    float mass, mass2, rho0, visc, drag, nps;
+    float gx, gy, gz;
+    float radius;      // collision radius
+    float tank_c[3], tank_h[3];
+    float bmin[3], bmax[3];
+    float min_len;     // min cell length (0.9-cell flag)
+    float eos_scale, eos_exp;
+    float pseudo_visc;
+};
+
+// Per-call step constants.
+struct StepConst {
+    float dt, dt_over_m;
+    float delta;       // PCISPH stiffness (FP64 on host, rounded for the update)
+    int first;         // k == 0 without warm start: pressure starts at 0
+};
+
+// device-side counters
+enum {
+    CNT_OVERFLOW = 0,   // bucket inserts dropped by the cap
+    CNT_LOST = 1,       // particles in no chain
+    CNT_FULL = 2,       // last distribution was a full pass
+    CNT_BAD_HASH = 3,   // points hashing outside the grid
+    CNT_N = 8
+};
+
+// ---------------------------------------------------------------------------
+// kernel profiler (HIP events on the solver stream)
+// ---------------------------------------------------------------------------
+struct Profiler {
+    bool enabled = false;
+    struct Pending { int id; hipEvent_t a, b; };
+    std::vector<std::string> names;
+    std::vector<double> total_ms;
+    std::vector<int> launches;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    int name_id(const char *name);
+    void begin(hipStream_t s, const char *name, hipEvent_t *a);
+    void end(hipStream_t s, int id, hipEvent_t a);
+    int resolve();
+    void reset();
+    void destroy();
+};
+
+}  // namespace bbx
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+struct bbx_mlp;
+
+struct bbx_solver {
+    int kind = BBX_PCISPH;
+    bbx_params prm{};
+    int n = 0;
+    int stride = 0;
+    double mass = 1e-3, kr = 0, radius = 1e-3, delta_denom = 0, mass_over_rho2 = 0;
+    double delta64 = 0;   // PCISPH delta of the current substep (FP64)
+    bbx::GridGeom grid{};
+    bbx::SimConst sc{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool setup_done = false;
+
+    // storage-order particle state.  A = sorted by cell (current),
+    // B = same order after integration (input of the next redistribution).
+    float4 *posA = nullptr, *posB = nullptr;   // xyz, w = particle id bits
+    float4 *velA = nullptr, *velB = nullptr;   // xyz, w = last converged pressure
+    int *cellA = nullptr, *cellB = nullptr;    // chain membership (total = none)
+    int *home = nullptr;                       // hashed cell (neighbour lookups)
+    // cell table (total + 1 entries, the last one collects particles in no chain)
+    int *cell_count = nullptr, *cell_start = nullptr;
+    int *scan_tmp = nullptr;
+    size_t scan_tmp_elems = 0;
+    // sort scratch (indexed by old slot / new position)
+    unsigned long long *key = nullptr;
+    int *newc = nullptr, *rank_tmp = nullptr, *tmp_idx = nullptr, *hash_tmp = nullptr;
+    // neighbour lists: nbr[k * stride + i], k < 100
+    int *nbr = nullptr, *nbr_cnt = nullptr;
+    // per-particle fields
+    float *rho = nullptr, *pres = nullptr, *rho_star = nullptr, *q = nullptr;
+    float4 *force = nullptr, *fp = nullptr, *xstar = nullptr;
+    float *feat = nullptr;       // [16][stride]
+    float *pwarm = nullptr;      // MLP output
+    // reductions / flags
+    unsigned long long *max_err = nullptr;   // FP64 bits of max |error|
+    unsigned *max_force = nullptr;           // FP32 bits of max |f|
+    int *flag = nullptr;                     // requiresHigherLevelUpdate
+    int *counters = nullptr;                 // CNT_*
+    // pinned readback
+    unsigned long long *h_max_err = nullptr;
+    int *h_counters = nullptr;
+    unsigned *h_max_force = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    bbx_mlp *mlp = nullptr;
+    bbx::Profiler prof;
+};
+
+// ---------------------------------------------------------------------------
+// launchers (defined in the .hip files, called by the ABI layer)
+// ---------------------------------------------------------------------------
+namespace bbx {
+int build_grid_geom(const bbx_params &p, GridGeom *g);
+int compute_mass_and_delta(bbx_solver *s);
+void fill_sim_const(bbx_solver *s);
+
+int launch_fill_int(bbx_solver *s, int *p, int value, size_t n);
+int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure);
+int run_density_only(bbx_solver *s);
+int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features);
+int run_predict_pressure(bbx_solver *s, const StepConst &st);
+int run_pressure_force(bbx_solver *s, const StepConst &st, int integrate);
+int run_sph_force(bbx_solver *s);
+int run_integrate(bbx_solver *s, const StepConst &st);
+int run_xsph(bbx_solver *s, const StepConst &st);
+int run_warm_init(bbx_solver *s);
+int swap_after_integrate(bbx_solver *s);
+int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float *out,
+                     hipStream_t stream);
+}  // namespace bbx

@@ -1,0 +1,188 @@
+// bbx_device.h -- device-side helpers: FP64 hash and bucket rule with the
+// reference's rounding, FP32 SPH kernels, the reversed-box collider and
+// wave64 reductions.  Included by the .hip translation units only.
+#pragma once
+
+#include "bbx_common.h"
+
+namespace bbx {
+
+__device__ __forceinline__ int pid_of(float4 p) { return __float_as_int(p.w); }
+
+// Grid::GetHashedPosition per axis (src/core/grid.h:258-298): ExtremeEpsilon
+// adds +-Epsilon (the float literal 0.0001f) only within 1e-8 of the bounds,
+// then floor((p - min) / len).  IEEE FP64, no contraction.
+__device__ __forceinline__ int hash_axis(double p, double bmin, double bmax, double minp,
+                                         double len, int n, int *bad) {
+#pragma clang fp contract(off)
+    double eps = 0.0;
+    double d0 = p - bmin;
+    double d1 = p - bmax;
+    if ((d0 > 0 ? d0 : -d0) < 1e-8) eps = (double)0.0001f;
+    else if ((d1 > 0 ? d1 : -d1) < 1e-8) eps = -(double)0.0001f;
+    p += eps;
+    double dp = (p - minp) / len;
+    double f = floor(dp);
+    if (!(f >= 0.0)) { *bad = 1; return 0; }
+    if (f >= (double)n) { *bad = 1; return n - 1; }
+    return (int)f;
+}
+
+// Returns the linear cell id (x + y*nx + z*nx*ny) and the 3-D index; *bad is
+// set when the point hashes outside the grid (the index is clamped).
+__device__ __forceinline__ int hash_cell(const GridGeom &g, float4 p, int u[3], int *bad) {
+    u[0] = hash_axis((double)p.x, g.bmin[0], g.bmax[0], g.minp[0], g.len[0], g.n[0], bad);
+    u[1] = hash_axis((double)p.y, g.bmin[1], g.bmax[1], g.minp[1], g.len[1], g.n[1], bad);
+    u[2] = hash_axis((double)p.z, g.bmin[2], g.bmax[2], g.minp[2], g.len[2], g.n[2], bad);
+    return u[0] + u[1] * g.n[0] + u[2] * g.n[0] * g.n[1];
+}
+
+__device__ __forceinline__ void unlinear(const GridGeom &g, int id, int u[3]) {
+    int plane = g.n[0] * g.n[1];
+    u[2] = id / plane;
+    int rem = id - u[2] * plane;
+    u[1] = rem / g.n[0];
+    u[0] = rem - u[1] * g.n[0];
+}
+
+// IsWithinStd(Distance(pi, pj), h) in FP64 (src/core/kernel.cpp:229-234,
+// src/core/geometry.h:632-633): d = sqrt(dx^2+dy^2+dz^2), d2 = d*d,
+// inside iff !(|d2 - h2| < 1e-8 || d2 - h2 > 0).
+__device__ __forceinline__ bool within_std_fp64(float4 a, float4 b, double h2) {
+#pragma clang fp contract(off)
+    double dx = (double)a.x - (double)b.x;
+    double dy = (double)a.y - (double)b.y;
+    double dz = (double)a.z - (double)b.z;
+    double s = dx * dx + dy * dy + dz * dz;
+    double d = __builtin_sqrt(s);
+    double d2 = d * d;
+    double of = d2 - h2;
+    double aof = of > 0 ? of : -of;
+    return !(aof < 1e-8 || of > 0);
+}
+
+// Bucket membership: FP32 squared distance decides outside a narrow band
+// around the FP64 cutoff; inside the band the exact FP64 rule decides.
+__device__ __forceinline__ bool in_bucket(const SimConst &c, float4 a, float4 b) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < c.lo2) return true;
+    if (d2 > c.hi2) return false;
+    return within_std_fp64(a, b, c.h2d);
+}
+
+// poly6 W with its own cutoff (src/core/kernel.cpp:123-132), FP32.
+__device__ __forceinline__ float poly6_w(const SimConst &c, float d2) {
+    float of = d2 - c.h2;
+    if (fabsf(of) < 1e-8f || of > 0.f) return 0.f;
+    float x = 1.f - d2 * c.inv_h2;
+    return c.poly6 * x * x * x;
+}
+
+// spiky cutoff (src/core/kernel.cpp:181-226): zero when |r-h| < 1e-8 or r > h
+__device__ __forceinline__ bool spiky_in(const SimConst &c, float r) {
+    float of = r - c.h;
+    return !(fabsf(of) < 1e-8f || of > 0.f);
+}
+
+// Kahan compensated add (FP32)
+__device__ __forceinline__ void kahan_add(float &s, float &comp, float v) {
+    float y = v - comp;
+    float t = s + y;
+    comp = (t - s) - y;
+    s = t;
+}
+
+// Reversed-box collider (ColliderSet3 with one container; CollisionHandle,
+// BoxClosestPoint, IsInside: src/core/collider.cpp:3-44,238-264,
+// src/shapes/box.cpp:74-172, src/core/shape.cpp:275-282), FP32.
+__device__ __forceinline__ bool inside_box(float px, float py, float pz, const float lo[3],
+                                           const float hi[3]) {
+    bool rv = px >= lo[0] && px <= hi[0] && py >= lo[1] && py <= hi[1] && pz >= lo[2] &&
+              pz <= hi[2];
+    if (!rv) {
+        float x = fminf(fabsf(lo[0] - px), fabsf(hi[0] - px));
+        float y = fminf(fabsf(lo[1] - py), fabsf(hi[1] - py));
+        float z = fminf(fabsf(lo[2] - pz), fabsf(hi[2] - pz));
+        rv = x < 1e-6f || y < 1e-6f || z < 1e-6f;
+    }
+    return rv;
+}
+
+__device__ __forceinline__ void collide(const SimConst &c, float e, float3 &p, float3 &v) {
+    const float lo[3] = {-c.tank_h[0], -c.tank_h[1], -c.tank_h[2]};
+    const float hi[3] = {c.tank_h[0], c.tank_h[1], c.tank_h[2]};
+    float l[3] = {p.x - c.tank_c[0], p.y - c.tank_c[1], p.z - c.tank_c[2]};
+    float cl[3];
+    int axis = 0;
+    float sgn = 1.f;  // outward normal of the chosen face: sgn * e_axis
+    float dist;
+    bool in = inside_box(l[0], l[1], l[2], lo, hi);
+    if (in) {
+        // nearest of the six planes, order +x,+y,+z,-x,-y,-z, strict <
+        float best = (hi[0] - l[0]) * (hi[0] - l[0]);
+        for (int f = 1; f < 6; f++) {
+            int a = f % 3;
+            float d = (f < 3) ? (hi[a] - l[a]) : (lo[a] - l[a]);
+            float d2 = d * d;
+            if (d2 < best) { best = d2; axis = a; sgn = (f < 3) ? 1.f : -1.f; }
+        }
+        cl[0] = l[0]; cl[1] = l[1]; cl[2] = l[2];
+        cl[axis] = (sgn > 0) ? hi[axis] : lo[axis];
+        dist = -sqrtf(best);
+    } else {
+        cl[0] = fminf(fmaxf(l[0], lo[0]), hi[0]);
+        cl[1] = fminf(fmaxf(l[1], lo[1]), hi[1]);
+        cl[2] = fminf(fmaxf(l[2], lo[2]), hi[2]);
+        float cov[3] = {l[0] - cl[0], l[1] - cl[1], l[2] - cl[2]};
+        float maxc = cov[0];
+        for (int f = 1; f < 6; f++) {
+            int a = f % 3;
+            float cs = (f < 3) ? cov[a] : -cov[a];
+            if (cs > maxc) { maxc = cs; axis = a; sgn = (f < 3) ? 1.f : -1.f; }
+        }
+        float dx = cl[0] - l[0], dy = cl[1] - l[1], dz = cl[2] - l[2];
+        dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+    // reversed orientation: the normal points into the tank
+    float n[3] = {0.f, 0.f, 0.f};
+    n[axis] = -sgn;
+    // IsInside(reversed) = !(ClosestDistance < 0); penetrating if that or |d| < r
+    bool is_inside = !(dist < 0.f);
+    if (!(is_inside || fabsf(dist) < c.radius)) return;
+    float tp[3] = {cl[0] + c.tank_c[0] + c.radius * n[0], cl[1] + c.tank_c[1] + c.radius * n[1],
+                   cl[2] + c.tank_c[2] + c.radius * n[2]};
+    float ndr = n[0] * v.x + n[1] * v.y + n[2] * v.z;
+    if (ndr < 0.f) {
+        // relN = ndr * n -> -e * relN; relT unchanged (friction 0)
+        float rn = ndr * (-e) - ndr;   // change along n
+        v.x += rn * n[0];
+        v.y += rn * n[1];
+        v.z += rn * n[2];
+    }
+    p.x = tp[0];
+    p.y = tp[1];
+    p.z = tp[2];
+}
+
+// wave64 reductions
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace bbx

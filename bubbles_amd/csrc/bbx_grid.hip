@@ -1,0 +1,389 @@
+// bbx_grid.hip -- neighbour-grid build on gfx950.
+//
+// Replaces the reference's per-cell linked-list redistribution
+// (UpdateGridDistributionKernel / SwapGridStatesKernel / UpdateParticlesBuckets,
+// src/equations/sph_equations3.cpp:487-539 over src/core/grid.h:389-500,422-447)
+// with a counting sort into cell-contiguous storage:
+//
+//   1. hash      one thread per particle: FP64 cell hash, sort key, atomic count
+//   2. scan      exclusive prefix sum of the cell counts (wave64 shuffles + LDS)
+//   3. scatter   place each particle in its cell's segment (arrival order)
+//   4. rank+move rank every particle inside its cell by key and move x, v
+//   5. buckets   per particle, walk the 27 cells in (y,x,z) order, apply the
+//                reference's FP64 cutoff, keep the first 100 -- fused with the
+//                density sum over the same members
+//
+// Reference chain order is reproduced by the key: a full distribution orders
+// each cell by particle id (DistributeByParticle); an incremental one
+// (DistributeToCellOpt) concatenates the 27 neighbour cells' old chains in
+// (y,x,z) order, i.e. key = (direction of the old cell seen from the new cell,
+// old storage slot), since storage is the old chain order.
+#include "bbx_device.h"
+
+namespace bbx {
+
+// ---------------------------------------------------------------------------
+// fills and scan
+// ---------------------------------------------------------------------------
+__global__ void k_fill_int(int *__restrict__ p, int value, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+int launch_fill_int(bbx_solver *s, int *p, int value, size_t n) {
+    if (n == 0) return BBX_OK;
+    size_t blocks = (n + kBlock - 1) / kBlock;
+    k_fill_int<<<dim3((unsigned)blocks), dim3(kBlock), 0, s->stream>>>(p, value, n);
+    BBX_HIP_CHECK(hipGetLastError());
+    return BBX_OK;
+}
+
+// Block-wide exclusive scan of kScanItems items per thread.  Each workgroup
+// scans kBlock*kScanItems elements and writes its total to block_sums.
+__global__ void __launch_bounds__(kBlock) k_scan_block(const int *__restrict__ in,
+                                                       int *__restrict__ out,
+                                                       int *__restrict__ block_sums, size_t n) {
+    __shared__ int wave_tot[kBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kBlock * kScanItems + (size_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        size_t i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        sum += v[k];
+    }
+    // inclusive scan of thread sums inside the wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    int wave_off = 0;
+    for (int w = 0; w < wave; w++) wave_off += wave_tot[w];
+    int run = wave_off + incl - sum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        size_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kBlock - 1) {
+        int tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) tot += wave_tot[w];
+        block_sums[blockIdx.x] = tot;
+    }
+}
+
+__global__ void k_scan_add(int *__restrict__ out, const int *__restrict__ block_off, size_t n) {
+    const size_t per = (size_t)kBlock * kScanItems;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += block_off[i / per];
+}
+
+// Exclusive scan, recursive over block totals.  Needs scan_tmp sized by
+// scan_scratch_elems(n).
+static size_t blocks_for(size_t n) { return (n + (size_t)kBlock * kScanItems - 1) / ((size_t)kBlock * kScanItems); }
+
+size_t scan_scratch_elems(size_t n) {
+    size_t total = 0;
+    while (n > 1) {
+        size_t b = blocks_for(n);
+        total += 2 * b;
+        n = b;
+    }
+    return total + 2;
+}
+
+static int scan_rec(bbx_solver *s, const int *in, int *out, size_t n, int *tmp) {
+    size_t b = blocks_for(n);
+    int *sums = tmp;
+    int *sums_scanned = tmp + b;
+    k_scan_block<<<dim3((unsigned)b), dim3(kBlock), 0, s->stream>>>(in, out, sums, n);
+    BBX_HIP_CHECK(hipGetLastError());
+    if (b > 1) {
+        BBX_CHECK(scan_rec(s, sums, sums_scanned, b, tmp + 2 * b));
+        size_t blocks = (n + kBlock - 1) / kBlock;
+        k_scan_add<<<dim3((unsigned)blocks), dim3(kBlock), 0, s->stream>>>(out, sums_scanned, n);
+        BBX_HIP_CHECK(hipGetLastError());
+    }
+    return BBX_OK;
+}
+
+int launch_scan(bbx_solver *s, const int *in, int *out, size_t n) {
+    if (n == 0) return BBX_OK;
+    if (scan_scratch_elems(n) > s->scan_tmp_elems) return fail(BBX_E_RANGE, "scan scratch too small");
+    return scan_rec(s, in, out, n, s->scan_tmp);
+}
+
+// ---------------------------------------------------------------------------
+// 1. hash + key + count
+// ---------------------------------------------------------------------------
+// Full mode (flag set, or first distribution): key = particle id.
+// Incremental mode: key = (27-neighbour direction index of the old cell as
+// seen from the new cell, old slot); a particle whose cell moved by more than
+// one in any axis, or that was in no chain, stays in no chain (the reference
+// only looks at the 27 neighbours' chains).
+__global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__restrict__ pos,
+                                                 const int *__restrict__ cell_old,
+                                                 const int *__restrict__ flag, int force_full,
+                                                 int *__restrict__ newc, int *__restrict__ hcell,
+                                                 unsigned long long *__restrict__ key,
+                                                 int *__restrict__ count,
+                                                 int *__restrict__ rank_tmp,
+                                                 int *__restrict__ counters) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c.n) return;
+    const int none = (int)c.g.total;
+    float4 p = pos[s];
+    int u[3], bad = 0;
+    int cid = hash_cell(c.g, p, u, &bad);
+    bool full = force_full || (*flag != 0);
+    int nc;
+    unsigned long long k;
+    if (full) {
+        nc = bad ? none : cid;
+        k = (unsigned long long)(unsigned)pid_of(p);
+    } else {
+        int co = cell_old[s];
+        nc = none;
+        k = (unsigned long long)s;
+        if (!bad && co != none) {
+            int uo[3];
+            unlinear(c.g, co, uo);
+            int dx = uo[0] - u[0], dy = uo[1] - u[1], dz = uo[2] - u[2];
+            if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+                nc = cid;
+                unsigned d = (unsigned)((dy + 1) * 9 + (dx + 1) * 3 + (dz + 1));
+                k = ((unsigned long long)d << 32) | (unsigned long long)(unsigned)s;
+            }
+        }
+    }
+    if (bad) atomicAdd(&counters[CNT_BAD_HASH], 1);
+    newc[s] = nc;
+    // a point outside the grid has no home cell: its bucket stays empty
+    hcell[s] = bad ? -1 : cid;
+    key[s] = k;
+    rank_tmp[s] = atomicAdd(&count[nc], 1);
+}
+
+// ---------------------------------------------------------------------------
+// 3. scatter into cell segments (arrival order), record full/lost stats
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_scatter(SimConst c, const int *__restrict__ newc,
+                                                    const int *__restrict__ rank_tmp,
+                                                    const int *__restrict__ start,
+                                                    const int *__restrict__ count,
+                                                    int *__restrict__ tmp_idx,
+                                                    int *__restrict__ flag, int force_full,
+                                                    int *__restrict__ counters) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        counters[CNT_FULL] = (force_full || *flag) ? 1 : 0;
+        counters[CNT_LOST] = count[c.g.total];
+        *flag = 0;  // ResetHigherLevel
+    }
+    if (s >= c.n) return;
+    tmp_idx[start[newc[s]] + rank_tmp[s]] = s;
+}
+
+// ---------------------------------------------------------------------------
+// 4. rank inside the cell by key and move the particle state
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__restrict__ tmp_idx,
+                                                      const int *__restrict__ newc,
+                                                      const int *__restrict__ hcell,
+                                                      const unsigned long long *__restrict__ key,
+                                                      const int *__restrict__ start,
+                                                      const int *__restrict__ count,
+                                                      const float4 *__restrict__ pos_in,
+                                                      const float4 *__restrict__ vel_in,
+                                                      float4 *__restrict__ pos_out,
+                                                      float4 *__restrict__ vel_out,
+                                                      int *__restrict__ cell_out,
+                                                      int *__restrict__ home_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n) return;
+    int s = tmp_idx[t];
+    int cc = newc[s];
+    unsigned long long k = key[s];
+    int b = start[cc];
+    int e = b + count[cc];
+    int dest;
+    if (e - b <= 4096) {
+        int rank = 0;
+        for (int q = b; q < e; q++) rank += key[tmp_idx[q]] < k ? 1 : 0;
+        dest = b + rank;
+    } else {
+        dest = t;  // pathological occupancy: keep arrival order
+    }
+    pos_out[dest] = pos_in[s];
+    vel_out[dest] = vel_in[s];
+    cell_out[dest] = cc;
+    home_out[dest] = hcell[s];
+}
+
+// ---------------------------------------------------------------------------
+// 5. buckets + density (+ Tait pressure for SPH)
+// ---------------------------------------------------------------------------
+// ComputeDensityFor (src/equations/sph_equations3.cpp:25-58) sums poly6 over
+// the bucket in bucket order; the sum runs as the bucket is built.
+template <bool kPressure>
+__global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
+                                                            const float4 *__restrict__ pos,
+                                                            const int *__restrict__ home,
+                                                            const int *__restrict__ start,
+                                                            const int *__restrict__ count,
+                                                            int *__restrict__ nbr,
+                                                            int *__restrict__ nbr_cnt,
+                                                            float *__restrict__ rho,
+                                                            float *__restrict__ pres,
+                                                            int *__restrict__ counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = pos[i];
+    const int hc = home[i];
+    int u[3] = {0, 0, 0};
+    if (hc >= 0) unlinear(c.g, hc, u);
+    int cnt = 0, dropped = 0;
+    float sum = 0.f, comp = 0.f;
+    const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2];
+    const int ylo = hc >= 0 ? u[1] - 1 : 1, yhi = hc >= 0 ? u[1] + 1 : 0;
+    for (int y = ylo; y <= yhi; y++) {
+        if (y < 0 || y >= ny) continue;
+        for (int x = u[0] - 1; x <= u[0] + 1; x++) {
+            if (x < 0 || x >= nx) continue;
+            for (int z = u[2] - 1; z <= u[2] + 1; z++) {
+                if (z < 0 || z >= nz) continue;
+                int nc = x + y * nx + z * nx * ny;
+                int b = start[nc], e = b + count[nc];
+                for (int j = b; j < e; j++) {
+                    float4 pj = pos[j];
+                    if (!in_bucket(c, pi, pj)) continue;
+                    if (cnt < kBucket) {
+                        nbr[(size_t)cnt * c.stride + i] = j;
+                        cnt++;
+                        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+                    } else {
+                        dropped++;
+                    }
+                }
+            }
+        }
+    }
+    nbr_cnt[i] = cnt;
+    float di = sum * c.mass;
+    rho[i] = di;
+    if (kPressure) {
+        // ComputePressureValue (src/equations/sph_equations3.cpp:7-18)
+        float p = c.eos_scale / c.eos_exp * (powf(di / c.rho0, c.eos_exp) - 1.f);
+        if (p < 0.f) p *= c.nps;
+        pres[i] = p;
+    }
+    if (dropped) atomicAdd(&counters[CNT_OVERFLOW], dropped);
+}
+
+// density over the existing buckets (after an upload of positions)
+__global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__restrict__ pos,
+                                                    const int *__restrict__ nbr,
+                                                    const int *__restrict__ nbr_cnt,
+                                                    float *__restrict__ rho) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = pos[i];
+    float sum = 0.f, comp = 0.f;
+    int cnt = nbr_cnt[i];
+    for (int k = 0; k < cnt; k++) {
+        float4 pj = pos[nbr[(size_t)k * c.stride + i]];
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+    }
+    rho[i] = sum * c.mass;
+}
+
+static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+#define PROF_BEGIN(name)                                                \
+    hipEvent_t ev_a_ = nullptr;                                         \
+    int ev_id_ = -1;                                                    \
+    if (s->prof.enabled) {                                              \
+        ev_id_ = s->prof.name_id(name);                                 \
+        s->prof.begin(s->stream, name, &ev_a_);                         \
+    }
+#define PROF_END()                                                      \
+    if (s->prof.enabled) s->prof.end(s->stream, ev_id_, ev_a_);
+
+// UpdateGridDistribution: full or incremental chain update, then buckets.
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure) {
+    SimConst &c = s->sc;
+    const int n = s->n;
+    const size_t cells = (size_t)c.g.total + 1;
+    BBX_CHECK(launch_fill_int(s, s->cell_count, 0, cells));
+    if (n > 0) {
+        {
+            PROF_BEGIN("grid_hash");
+            k_hash<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->posA, s->cellA, s->flag,
+                                                         force_full ? 1 : 0, s->newc,
+                                                         s->hash_tmp, s->key, s->cell_count,
+                                                         s->rank_tmp, s->counters);
+            BBX_HIP_CHECK(hipGetLastError());
+            PROF_END();
+        }
+        {
+            PROF_BEGIN("grid_scan");
+            BBX_CHECK(launch_scan(s, s->cell_count, s->cell_start, cells));
+            PROF_END();
+        }
+        {
+            PROF_BEGIN("grid_scatter");
+            k_scatter<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->newc, s->rank_tmp,
+                                                            s->cell_start, s->cell_count,
+                                                            s->tmp_idx, s->flag,
+                                                            force_full ? 1 : 0, s->counters);
+            BBX_HIP_CHECK(hipGetLastError());
+            PROF_END();
+        }
+        {
+            PROF_BEGIN("grid_rank_move");
+            k_rank_move<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->tmp_idx, s->newc,
+                                                              s->hash_tmp, s->key,
+                                                              s->cell_start, s->cell_count,
+                                                              s->posA, s->velA, s->posB,
+                                                              s->velB, s->cellB, s->home);
+            BBX_HIP_CHECK(hipGetLastError());
+            PROF_END();
+        }
+        std::swap(s->posA, s->posB);
+        std::swap(s->velA, s->velB);
+        std::swap(s->cellA, s->cellB);
+    }
+    if (build_buckets && n > 0) {
+        PROF_BEGIN("buckets_density");
+        if (with_pressure)
+            k_buckets_density<true><<<nblocks(n), kBlock, 0, s->stream>>>(
+                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr, s->nbr_cnt, s->rho,
+                s->pres, s->counters);
+        else
+            k_buckets_density<false><<<nblocks(n), kBlock, 0, s->stream>>>(
+                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr, s->nbr_cnt, s->rho,
+                s->pres, s->counters);
+        BBX_HIP_CHECK(hipGetLastError());
+        PROF_END();
+    }
+    return BBX_OK;
+}
+
+int run_density_only(bbx_solver *s) {
+    if (s->n == 0) return BBX_OK;
+    PROF_BEGIN("density");
+    k_density<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->posA, s->nbr, s->nbr_cnt, s->rho);
+    BBX_HIP_CHECK(hipGetLastError());
+    PROF_END();
+    return BBX_OK;
+}
+
+}  // namespace bbx

@@ -1,0 +1,226 @@
+/*
+ * bubbles_amd.h -- C ABI of the MI355X (gfx950) SPH / PCISPH hot path.
+ *
+ * The reference exposes this path only as a C++ class API (PciSphSolver3,
+ * SphSolver3, Grid3, ParticleSet3, Bucket; src/core/pcisph_solver.h:56-83,
+ * src/core/sph_solver.h:76-96, src/core/grid.h:197-600,
+ * src/core/particle.h:154-401).  Each entry point below is the flat form of
+ * one of those calls; the citation names the reference interface it
+ * replaces.  The C++ facade in include/bubbles/ (same class names) and any
+ * FFI (ctypes, cgo, JNI) bind to these symbols.
+ *
+ * Conventions
+ *   - every function returns BBX_OK (0) or a negative BBX_E* status; the
+ *     message of the last failure on the calling thread is bbx_last_error();
+ *   - host arrays are plain pointers in the reference's precision (double)
+ *     and are indexed by particle id (the order particles were uploaded in),
+ *     never by device storage order;
+ *   - one solver owns one HIP stream and all of its device memory; calls on
+ *     one handle are not thread-safe; nothing calls exit().
+ */
+#ifndef BUBBLES_AMD_H
+#define BUBBLES_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBX_OK 0
+#define BBX_E_INVALID (-1)   /* bad argument or call order            */
+#define BBX_E_HIP (-2)       /* HIP runtime error                      */
+#define BBX_E_NOMEM (-3)     /* device allocation failed               */
+#define BBX_E_RANGE (-4)     /* size beyond a documented limit         */
+#define BBX_E_NODEVICE (-5)  /* no gfx950 device visible               */
+
+#define BBX_BUCKET_SIZE 100  /* MaximumParticlesPerBucket, src/core/particle.h:10 */
+
+/* solver kinds */
+#define BBX_PCISPH 0
+#define BBX_SPH 1
+
+/* PCISPH loop modes (see ComputePressureForceAndIntegrate,
+ * src/equations/pcisph_equations3.cpp:218-256).  REFERENCE_EXACT runs one
+ * predict-correct iteration per substep, which is what the shipped loop does
+ * because its error array is never written; TEXTBOOK iterates until the max
+ * compression error / rho0 < tol or max_it is reached. */
+#define BBX_MODE_REFERENCE_EXACT 0
+#define BBX_MODE_TEXTBOOK 1
+
+/* pressure warm start */
+#define BBX_WARM_NONE 0
+#define BBX_WARM_MLP 1
+
+/* downloadable / uploadable per-particle fields (by particle id) */
+#define BBX_FIELD_POSITION 0          /* 3 doubles */
+#define BBX_FIELD_VELOCITY 1          /* 3 doubles */
+#define BBX_FIELD_FORCE 2             /* 3 doubles: last total force */
+#define BBX_FIELD_DENSITY 3           /* 1 double */
+#define BBX_FIELD_PRESSURE 4          /* 1 double */
+#define BBX_FIELD_DENSITY_PREDICTED 5 /* 1 double */
+#define BBX_FIELD_PRESSURE_FORCE 6    /* 3 doubles */
+#define BBX_FIELD_TEMP_POSITION 7     /* 3 doubles: predicted x* */
+#define BBX_FIELD_FEATURES 8          /* 16 doubles: warm-start features */
+
+typedef struct bbx_solver bbx_solver;
+typedef struct bbx_mlp bbx_mlp;
+
+/* Setup inputs + SphSolverData3 fields (src/core/sph_solver.h:31-50,
+ * DefaultSphSolverData3 src/solvers/sph_solver3.cpp:124-151). */
+typedef struct bbx_params {
+    double target_density;          /* Setup(targetDensity, ...)            */
+    double target_spacing;          /* Setup(..., targetSpacing, ...)       */
+    double relative_radius;         /* Setup(..., relativeRadius, ...)      */
+    double tank_center[3];          /* reversed MakeBox container centre    */
+    double tank_size[3];            /* container size                       */
+    double grid_spacing;            /* UtilBuildGridForDomain(b, spacing, s) */
+    double grid_scale;
+    double eos_exponent;            /* 7                                    */
+    double negative_pressure_scale; /* 0                                    */
+    double viscosity;               /* 0.04                                 */
+    double pseudo_viscosity;        /* 10                                   */
+    double sound_speed;             /* 100                                  */
+    double drag_coefficient;        /* 1e-4                                 */
+    double gravity[3];              /* (0, -9.8f, 0)                        */
+} bbx_params;
+
+typedef struct bbx_grid_desc {      /* Grid3 after Build (src/core/grid.h:626-666) */
+    unsigned usizes[3];
+    unsigned total;
+    double cells_len[3];
+    double min_point[3];
+    double bounds_min[3];
+    double bounds_max[3];
+} bbx_grid_desc;
+
+typedef struct bbx_info {
+    int count;
+    double mass;                    /* ParticleSet3::GetMass                */
+    double kernel_radius;           /* SphParticleSet3::GetKernelRadius     */
+    double radius;                  /* ParticleSet3::GetRadius              */
+    double delta_denom;             /* PciSphSolver3::deltaDenom            */
+    bbx_grid_desc grid;
+} bbx_info;
+
+typedef struct bbx_stats {
+    int iterations;                 /* predict-correct iterations this substep   */
+    int full_redistribution;        /* 1 if the 0.9-cell flag forced a full pass */
+    int bucket_overflows;           /* neighbours dropped by the 100 cap         */
+    int lost_particles;             /* particles outside every cell chain        */
+    double max_density_error;       /* max compression error / rho0 (FP64)       */
+    double gpu_ms;                  /* device time of the substep                */
+} bbx_stats;
+
+/* ---- library ---- */
+const char *bbx_last_error(void);
+int bbx_version(int *major, int *minor);
+int bbx_device_count(int *count);
+
+/* ---- defaults ----
+ * DefaultSphSolverData3(true) (src/solvers/sph_solver3.cpp:124-151) */
+int bbx_default_params(bbx_params *out);
+
+/* ---- solver lifetime ----
+ * replaces PciSphSolver3::Initialize / SphSolver3::Initialize plus
+ * SphParticleSet3FromBuilder(ParticleSetBuilder3::AddParticle...) and
+ * UtilBuildGridForDomain (src/solvers/pcisph_solver3.cpp:9-14,
+ * src/core/particle.cpp:21-27, src/core/util.cpp:269-287).
+ * pos/vel: n*3 doubles (vel may be NULL = zero). */
+int bbx_solver_create(int kind, const bbx_params *params, const double *pos,
+                      const double *vel, int n, bbx_solver **out);
+int bbx_solver_destroy(bbx_solver *s);
+
+/* PciSphSolver3::Setup / SphSolver3::Setup (src/solvers/pcisph_solver3.cpp:103-146,
+ * src/solvers/sph_solver3.cpp:96-122): mass, delta denominator, full
+ * distribution into ascending-id chains. */
+int bbx_solver_setup(bbx_solver *s);
+int bbx_solver_info(bbx_solver *s, bbx_info *out);
+
+/* ParticleSet3::Get/SetParticle* (src/core/particle.h:316-386), by id. */
+int bbx_solver_download(bbx_solver *s, int field, double *out);
+int bbx_solver_upload(bbx_solver *s, int field, const double *in);
+
+/* SphParticleSet3::requiresHigherLevelUpdate (src/core/particle.h:525-527) */
+int bbx_solver_get_flag(bbx_solver *s, int *flag);
+int bbx_solver_set_flag(bbx_solver *s, int flag);
+
+/* ---- grid and neighbour query ----
+ * UpdateGridDistributionGPU (src/equations/sph_equations3.cpp:511-539):
+ * full or incremental chain update (flag-dependent), then every particle's
+ * bucket; then ResetHigherLevel. */
+int bbx_grid_distribute(bbx_solver *s, bbx_stats *stats);
+/* Cell::GetChain walk for every cell (src/core/grid.h:56-69): CSR in chain
+ * order; cell_start has total+1 entries, members up to count entries. */
+int bbx_grid_get_chains(bbx_solver *s, int *cell_start, int *members);
+/* Replace the chain state (cell membership and order) with the given CSR. */
+int bbx_grid_set_chains(bbx_solver *s, const int *cell_start, const int *members);
+/* Grid3::GetLinearHashedPosition (src/core/grid.h:307-321) for n points;
+ * returns -1 for points outside the grid. */
+int bbx_grid_hash(bbx_solver *s, const double *points, int n, int *cell_ids);
+/* Grid3::GetNeighborsOf (src/core/grid.h:502-510): <=27 ids, (y,x,z) order */
+int bbx_grid_cell_neighbors(bbx_solver *s, int cell, int *out27, int *count);
+/* ParticleSet3::GetParticleBucket for all particles (src/core/particle.h:276-279,
+ * Bucket::Get :52-59): counts[n], ids[n*100] (row i = particle i). */
+int bbx_neighbors_get(bbx_solver *s, int *counts, int *ids);
+
+/* ---- stages (one reference wrapper each; used by parity tests) ---- */
+int bbx_stage_density(bbx_solver *s);          /* ComputeParticleInteraction + ComputeDensityGPU (src/equations/sph_equations3.cpp:541-553,585-597) */
+int bbx_stage_nonpressure(bbx_solver *s);      /* ComputeNonPressureForceGPU (:608-620) */
+
+/* ---- solver steps ----
+ * One PCISPH substep of fixed dt: AdvanceTimeStep (src/solvers/pcisph_solver3.cpp:42-65). */
+int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
+                       bbx_stats *stats);
+/* PciSphSolver3::Advance (src/solvers/pcisph_solver3.cpp:67-89): reference
+ * substep rule with scale 5. */
+int bbx_pcisph_advance(bbx_solver *s, double interval, int max_it, double tol, int mode,
+                       int warm, int *substeps);
+/* SphSolver3 AdvanceTimeStep / Advance (src/solvers/sph_solver3.cpp:48-93),
+ * race-free two-phase force/integration. */
+int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats);
+int bbx_sph_advance(bbx_solver *s, double interval, int *substeps);
+/* PciSphSolver3::ComputeDelta (src/solvers/pcisph_solver3.cpp:182-185) */
+int bbx_pcisph_delta(bbx_solver *s, double dt, double *delta);
+/* SphParticleSet3::ComputeNumberOfTimeSteps (src/core/particle.h:584-608) */
+int bbx_number_of_time_steps(bbx_solver *s, double interval, double scale, unsigned *n);
+
+/* ---- learned pressure warm start (no reference counterpart) ----
+ * 16 -> 64 -> 64 -> 1 MLP, GELU, BF16 weights, FP32 accumulate, on MFMA. */
+int bbx_mlp_create(int seed, bbx_mlp **out);
+int bbx_mlp_destroy(bbx_mlp *m);
+/* params layout: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1] in FP32
+ * (master copy), then feature mean[16], feature inv_std[16], out_scale[1]. */
+int bbx_mlp_param_count(int *count);
+int bbx_mlp_get_params(bbx_mlp *m, float *out);
+int bbx_mlp_set_params(bbx_mlp *m, const float *in);
+/* y[n] = MLP(x[n*16]) on the device (host arrays). */
+int bbx_mlp_infer(bbx_mlp *m, const float *x, int n, float *y);
+/* gradients of 0.5*mean((y - t)^2) w.r.t. the FP32 params (host arrays) */
+int bbx_mlp_gradients(bbx_mlp *m, const float *x, const float *t, int n, float *grad,
+                      float *loss);
+/* one Adam step on a batch (host arrays); returns the batch loss */
+int bbx_mlp_train_step(bbx_mlp *m, const float *x, const float *t, int n, float lr,
+                       float *loss);
+/* train for `steps` Adam steps on device-resident samples drawn from the
+ * given host arrays (copied once); batch sampled by a counter-based RNG */
+int bbx_mlp_train(bbx_mlp *m, const float *x, const float *t, int n, int steps, int batch,
+                  float lr, int seed, float *final_loss);
+int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m);
+/* Harvest (features, converged pressure / P) pairs of one TEXTBOOK substep:
+ * runs the substep and writes n*16 features and n targets (by particle id). */
+int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,
+                       float *targets, bbx_stats *stats);
+
+/* ---- profiling ----
+ * Average device time (ms) and launch count of every kernel of this solver
+ * since the last reset, measured with HIP events on the solver's stream. */
+int bbx_profile_enable(bbx_solver *s, int enable);
+int bbx_profile_reset(bbx_solver *s);
+/* names: '\n'-separated kernel names; ms/launches arrays of up to cap entries */
+int bbx_profile_get(bbx_solver *s, char *names, int names_cap, double *total_ms,
+                    int *launches, int cap, int *count);
+int bbx_solver_synchronize(bbx_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,302 @@
+"""ctypes front-end of the FP64 CPU oracle (libbbx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+benchmark's CPU-baseline leg.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbbx_oracle.so")
+
+MODE_REFERENCE_EXACT = 0
+MODE_TEXTBOOK = 1
+SPH_TWO_PHASE = 0
+SPH_SEQUENTIAL = 1
+BUCKET = 100
+
+
+class Params(C.Structure):
+    _fields_ = [("target_density", C.c_double), ("target_spacing", C.c_double),
+                ("relative_radius", C.c_double), ("tank_center", C.c_double * 3),
+                ("tank_size", C.c_double * 3), ("grid_spacing", C.c_double),
+                ("grid_scale", C.c_double), ("eos_exponent", C.c_double),
+                ("negative_pressure_scale", C.c_double), ("viscosity", C.c_double),
+                ("pseudo_viscosity", C.c_double), ("sound_speed", C.c_double),
+                ("drag_coefficient", C.c_double), ("gravity", C.c_double * 3),
+                ("threads", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("max_density_error", C.c_double),
+                ("full_redistribution", C.c_int), ("bucket_overflows", C.c_int),
+                ("lost_particles", C.c_int)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("usizes", C.c_uint * 3), ("total", C.c_uint), ("cells_len", C.c_double * 3),
+                ("min_point", C.c_double * 3), ("bounds_min", C.c_double * 3),
+                ("bounds_max", C.c_double * 3)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, i, u, vp = C.c_double, C.c_int, C.c_uint, C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        for name in ("bbo_poly6_W", "bbo_poly6_dW", "bbo_poly6_d2W", "bbo_spiky_W",
+                     "bbo_spiky_dW", "bbo_spiky_d2W"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d, d]
+        for name in ("bbo_within_std", "bbo_within_spiky"):
+            getattr(L, name).restype = i
+            getattr(L, name).argtypes = [d, d]
+        L.bbo_grid_build.argtypes = [C.POINTER(u), dp, dp, C.POINTER(GridDesc)]
+        L.bbo_grid_for_domain.argtypes = [dp, dp, d, d, C.POINTER(GridDesc)]
+        L.bbo_grid_hash.restype = C.c_longlong
+        L.bbo_grid_hash.argtypes = [C.POINTER(GridDesc), dp, ip]
+        L.bbo_grid_neighbors.restype = i
+        L.bbo_grid_neighbors.argtypes = [C.POINTER(GridDesc), u, ip]
+        L.bbo_bcc_generate.restype = i
+        L.bbo_bcc_generate.argtypes = [dp, dp, d, dp, i]
+        L.bbo_compute_mass.restype = d
+        L.bbo_compute_mass.argtypes = [d, d, d]
+        L.bbo_delta_denom.restype = d
+        L.bbo_delta_denom.argtypes = [d, d]
+        L.bbo_default_params.argtypes = [C.POINTER(Params)]
+        L.bbo_sim_create.restype = vp
+        L.bbo_sim_create.argtypes = [C.POINTER(Params), i, dp, dp]
+        L.bbo_sim_destroy.argtypes = [vp]
+        L.bbo_sim_setup.argtypes = [vp]
+        L.bbo_sim_info.argtypes = [vp, dp, dp, dp, dp, C.POINTER(GridDesc)]
+        L.bbo_sim_set_threads.argtypes = [vp, i]
+        L.bbo_sim_get.argtypes = [vp, C.c_char_p, dp]
+        L.bbo_sim_set.argtypes = [vp, C.c_char_p, dp]
+        L.bbo_sim_get_flag.restype = i
+        L.bbo_sim_get_flag.argtypes = [vp]
+        L.bbo_sim_set_flag.argtypes = [vp, i]
+        L.bbo_sim_count.restype = i
+        L.bbo_sim_count.argtypes = [vp]
+        L.bbo_sim_get_chains.argtypes = [vp, ip, ip]
+        L.bbo_sim_set_chains.argtypes = [vp, ip, ip]
+        L.bbo_sim_get_buckets.argtypes = [vp, ip, ip]
+        L.bbo_stage_distribute.argtypes = [vp, C.POINTER(Stats)]
+        L.bbo_stage_interaction.argtypes = [vp]
+        L.bbo_stage_density.argtypes = [vp, i]
+        L.bbo_stage_nonpressure.argtypes = [vp]
+        L.bbo_stage_predict.argtypes = [vp, d, i]
+        L.bbo_stage_predict_pressure.argtypes = [vp, d, i]
+        L.bbo_stage_pressure_force.argtypes = [vp]
+        L.bbo_stage_accumulate_integrate.argtypes = [vp, d]
+        L.bbo_stage_xsph.argtypes = [vp, d]
+        L.bbo_compute_delta.restype = d
+        L.bbo_compute_delta.argtypes = [vp, d]
+        L.bbo_number_of_time_steps.restype = u
+        L.bbo_number_of_time_steps.argtypes = [vp, d, d]
+        L.bbo_pcisph_substep.argtypes = [vp, d, i, d, i, dp, C.POINTER(Stats)]
+        L.bbo_pcisph_advance.restype = i
+        L.bbo_pcisph_advance.argtypes = [vp, d, i, d, i]
+        L.bbo_sph_substep.argtypes = [vp, d, i, C.POINTER(Stats)]
+        L.bbo_sph_advance.restype = i
+        L.bbo_sph_advance.argtypes = [vp, d, i]
+        L.bbo_features.argtypes = [vp, d, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+# ---- free functions ----
+def kernel(name, h, r):
+    return getattr(lib(), "bbo_" + name)(float(h), float(r))
+
+
+def grid_build(res, p0, p1):
+    g = GridDesc()
+    lib().bbo_grid_build((C.c_uint * 3)(*res), (C.c_double * 3)(*p0), (C.c_double * 3)(*p1), C.byref(g))
+    return g
+
+
+def grid_for_domain(dmin, dmax, spacing, scale):
+    g = GridDesc()
+    lib().bbo_grid_for_domain((C.c_double * 3)(*dmin), (C.c_double * 3)(*dmax), float(spacing),
+                              float(scale), C.byref(g))
+    return g
+
+
+def grid_hash(g, p):
+    u = (C.c_int * 3)()
+    cid = lib().bbo_grid_hash(C.byref(g), (C.c_double * 3)(*p), u)
+    return int(cid), tuple(u)
+
+
+def grid_neighbors(g, cell):
+    out = (C.c_int * 27)()
+    n = lib().bbo_grid_neighbors(C.byref(g), int(cell), out)
+    return list(out[:n])
+
+
+def bcc_points(pmin, pmax, spacing, maxn=-1):
+    n = lib().bbo_bcc_generate((C.c_double * 3)(*pmin), (C.c_double * 3)(*pmax), float(spacing),
+                               None, -1 if maxn < 0 else maxn)
+    pts = np.zeros((max(n, 1), 3))
+    lib().bbo_bcc_generate((C.c_double * 3)(*pmin), (C.c_double * 3)(*pmax), float(spacing),
+                           _dp(pts), -1 if maxn < 0 else maxn)
+    return pts[:n]
+
+
+def compute_mass(rho0, spacing, kr):
+    return lib().bbo_compute_mass(rho0, spacing, kr)
+
+
+def delta_denom(spacing, kr):
+    return lib().bbo_delta_denom(spacing, kr)
+
+
+def params_for(scene, threads=1, **over):
+    p = Params()
+    lib().bbo_default_params(C.byref(p))
+    p.target_density = scene.target_density
+    p.target_spacing = scene.target_spacing
+    p.relative_radius = scene.relative_radius
+    p.tank_center[:] = list(scene.tank_center)
+    p.tank_size[:] = list(scene.tank_size)
+    p.grid_spacing = scene.grid_spacing
+    p.grid_scale = scene.grid_scale
+    p.viscosity = scene.viscosity
+    p.threads = threads
+    for k, v in over.items():
+        if k == "gravity":
+            p.gravity[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Sim:
+    """One oracle simulation (PciSphSolver3 / SphSolver3 state after Setup)."""
+
+    def __init__(self, scene, threads=1, setup=True, **over):
+        self.scene = scene
+        self.params = params_for(scene, threads, **over)
+        pos = np.ascontiguousarray(scene.positions, dtype=np.float64)
+        vel = np.ascontiguousarray(scene.velocities, dtype=np.float64)
+        self.n = pos.shape[0]
+        self.h = lib().bbo_sim_create(C.byref(self.params), self.n, _dp(pos), _dp(vel))
+        if setup:
+            lib().bbo_sim_setup(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bbo_sim_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        m, kr, r, dd = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        g = GridDesc()
+        lib().bbo_sim_info(self.h, C.byref(m), C.byref(kr), C.byref(r), C.byref(dd), C.byref(g))
+        return {"mass": m.value, "kernel_radius": kr.value, "radius": r.value,
+                "delta_denom": dd.value, "grid": g}
+
+    def get(self, field):
+        vec = field in ("position", "velocity", "force", "interaction", "temp_position",
+                        "temp_velocity", "pressure_force")
+        out = np.zeros((self.n, 3) if vec else (self.n,), dtype=np.float64)
+        lib().bbo_sim_get(self.h, field.encode(), _dp(out))
+        return out
+
+    def set(self, field, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        lib().bbo_sim_set(self.h, field.encode(), _dp(a))
+
+    @property
+    def flag(self):
+        return lib().bbo_sim_get_flag(self.h)
+
+    @flag.setter
+    def flag(self, v):
+        lib().bbo_sim_set_flag(self.h, int(v))
+
+    def set_threads(self, t):
+        lib().bbo_sim_set_threads(self.h, int(t))
+
+    def chains(self):
+        total = self.info()["grid"].total
+        start = np.zeros(total + 1, dtype=np.int32)
+        members = np.zeros(max(self.n, 1), dtype=np.int32)
+        lib().bbo_sim_get_chains(self.h, _ip(start), _ip(members))
+        return start, members[: start[-1]]
+
+    def set_chains(self, start, members):
+        s = np.ascontiguousarray(start, dtype=np.int32)
+        m = np.ascontiguousarray(members, dtype=np.int32)
+        if m.size == 0:
+            m = np.zeros(1, dtype=np.int32)
+        lib().bbo_sim_set_chains(self.h, _ip(s), _ip(m))
+
+    def buckets(self):
+        counts = np.zeros(self.n, dtype=np.int32)
+        ids = np.zeros((self.n, BUCKET), dtype=np.int32)
+        lib().bbo_sim_get_buckets(self.h, _ip(counts), _ip(ids))
+        return counts, ids
+
+    def distribute(self):
+        st = Stats()
+        lib().bbo_stage_distribute(self.h, C.byref(st))
+        return st
+
+    def stage(self, name, *args):
+        return getattr(lib(), "bbo_stage_" + name)(self.h, *args)
+
+    def delta(self, dt):
+        return lib().bbo_compute_delta(self.h, dt)
+
+    def number_of_time_steps(self, interval, scale):
+        return lib().bbo_number_of_time_steps(self.h, interval, scale)
+
+    def pcisph_substep(self, dt, max_it=5, tol=0.01, mode=MODE_REFERENCE_EXACT, warm=None):
+        st = Stats()
+        w = None
+        if warm is not None:
+            warm = np.ascontiguousarray(warm, dtype=np.float64)
+            w = _dp(warm)
+        lib().bbo_pcisph_substep(self.h, dt, max_it, tol, mode, w, C.byref(st))
+        return st
+
+    def pcisph_advance(self, interval, max_it=5, tol=0.01, mode=MODE_REFERENCE_EXACT):
+        return lib().bbo_pcisph_advance(self.h, interval, max_it, tol, mode)
+
+    def sph_substep(self, dt, sph_mode=SPH_TWO_PHASE):
+        st = Stats()
+        lib().bbo_sph_substep(self.h, dt, sph_mode, C.byref(st))
+        return st
+
+    def sph_advance(self, interval, sph_mode=SPH_TWO_PHASE):
+        return lib().bbo_sph_advance(self.h, interval, sph_mode)
+
+    def features(self, dt):
+        out = np.zeros((self.n, 16), dtype=np.float64)
+        lib().bbo_features(self.h, dt, _dp(out))
+        return out

@@ -1,0 +1,191 @@
+"""GPU parity: neighbour grid (cell ids, chain order, buckets) bit-exact
+against the FP64 oracle on the same FP32 positions.
+
+Reference semantics under test: full distribution = ascending id per cell
+(src/core/grid.h:389-407); incremental = concatenation of the 27 neighbour
+cells' old chains in (y,x,z) order (src/core/grid.h:449-494); buckets = the
+first 100 chain members within the FP64 cutoff (src/core/grid.h:422-447).
+"""
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_dam(order="sorted", seed=2):
+    sc = scenes.dam_break(32, 16, 32, spacing=0.01, tank=(0.5, 0.25, 0.5), seed=seed)
+    if order == "shuffled":
+        sc = sc.shuffled(seed)
+    return sc
+
+
+def gpu_positions_scene(sc, pos):
+    """Scene whose positions are the FP32 device positions (widened)."""
+    import dataclasses
+
+    return dataclasses.replace(sc, positions=np.asarray(pos, dtype=np.float64).copy())
+
+
+def assert_chains_equal(a, b):
+    sa, ma = a
+    sb, mb = b
+    assert np.array_equal(sa, sb), "cell_start differs"
+    assert np.array_equal(ma, mb), "chain members / order differ"
+
+
+def assert_buckets_equal(a, b):
+    ca, ia = a
+    cb, ib = b
+    assert np.array_equal(ca, cb), f"bucket counts differ at {np.nonzero(ca != cb)[0][:10]}"
+    n, k = ia.shape
+    mask = np.arange(k)[None, :] < ca[:, None]
+    assert np.array_equal(np.where(mask, ia, -1), np.where(mask, ib, -1)), "bucket ids/order differ"
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_setup_scalars_and_grid(bbx, oracle, order):
+    sc = small_dam(order)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(sc)
+    gi, oi = g.info(), o.info()
+    # FP64 host scalars are computed by the same method: exact
+    assert gi["mass"] == oi["mass"]
+    assert gi["kernel_radius"] == oi["kernel_radius"]
+    assert gi["radius"] == oi["radius"]
+    assert gi["delta_denom"] == oi["delta_denom"]
+    gg, og = gi["grid"], oi["grid"]
+    assert list(gg.usizes) == list(og.usizes) and gg.total == og.total
+    for f in ("cells_len", "min_point", "bounds_min", "bounds_max"):
+        assert list(getattr(gg, f)) == list(getattr(og, f))
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_setup_full_distribution_exact(bbx, oracle, order):
+    sc = small_dam(order)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")))
+    assert_chains_equal(g.chains(), o.chains())
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_incremental_distribution_exact_over_steps(bbx, oracle, order):
+    """Run GPU PCISPH substeps; before each one, replay the GPU's own chain
+    state and positions through the oracle's distribution and compare."""
+    sc = small_dam(order)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")))
+    for step in range(12):
+        pre_chain = g.chains()
+        pre_pos = g.get("position")
+        flag = g.flag
+        gst = g.distribute()
+        o.set_chains(*pre_chain)
+        o.set("position", pre_pos)
+        o.flag = flag
+        ost = o.distribute()
+        assert gst.full_redistribution == ost.full_redistribution
+        assert gst.lost_particles == ost.lost_particles
+        assert gst.bucket_overflows == ost.bucket_overflows
+        assert_chains_equal(g.chains(), o.chains())
+        assert_buckets_equal(g.buckets(), o.buckets())
+        g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+
+
+def test_forced_full_redistribution_exact(bbx, oracle):
+    """A velocity kick above 0.9 cell / dt sets requiresHigherLevelUpdate in
+    integration; the next distribution is a full one (ascending ids)."""
+    sc = small_dam("shuffled", seed=5)
+    g = bbx.Solver(sc)
+    g.pcisph_substep(sc.dt, 3, 0.01, bbx.MODE_TEXTBOOK)
+    # a uniform kick (no relative velocity, so viscosity does not damp it)
+    v = g.get("velocity")
+    v[:, 0] += 0.95 * 0.02 / sc.dt * 1.2
+    g.set("velocity", v)
+    g.pcisph_substep(sc.dt, 3, 0.01, bbx.MODE_TEXTBOOK)
+    assert g.flag == 1
+    pre_chain = g.chains()
+    pre_pos = g.get("position")
+    gst = g.distribute()
+    assert gst.full_redistribution == 1
+    o = oracle.Sim(gpu_positions_scene(sc, pre_pos))
+    o.set_chains(*pre_chain)
+    o.flag = 1
+    ost = o.distribute()
+    assert ost.full_redistribution == 1
+    assert_chains_equal(g.chains(), o.chains())
+    assert_buckets_equal(g.buckets(), o.buckets())
+    # full distribution is ascending id inside each cell
+    start, members = g.chains()
+    for c in np.nonzero(np.diff(start) > 1)[0][:200]:
+        seg = members[start[c]:start[c + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+def test_bucket_cutoff_edge_cases(bbx, oracle):
+    """Pairs placed within a few ulps of the kernel radius exercise the FP64
+    branch of the bucket rule (IsWithinStd, src/core/kernel.cpp:229-234)."""
+    import dataclasses
+
+    base = small_dam("sorted")
+    h = base.kernel_radius
+    rng = np.random.default_rng(11)
+    pts = []
+    for k in range(400):
+        c = rng.uniform(-0.09, 0.09, size=3)  # pairs stay inside the grid
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = h * (1.0 + rng.choice([-1, 1]) * rng.choice([1e-9, 1e-8, 3e-7, 2e-6, 1e-5]))
+        r = r - 1e-8 / (2 * h) * rng.choice([0.0, 1.0, 0.5])  # around h^2 - 1e-8
+        pts.append(c)
+        pts.append(c + r * d)
+    pos = np.asarray(pts, dtype=np.float32).astype(np.float64)
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")))
+    g.distribute()
+    o.distribute()
+    assert_buckets_equal(g.buckets(), o.buckets())
+
+
+def test_host_hash_matches_oracle(bbx, oracle):
+    sc = small_dam("sorted")
+    g = bbx.Solver(sc)
+    gd = g.info()["grid"]
+    rng = np.random.default_rng(3)
+    lo, hi = np.array(gd.bounds_min), np.array(gd.bounds_max)
+    pts = rng.uniform(lo, hi, size=(5000, 3))
+    # points exactly on and within 1e-8 of the bounds take the Epsilon branch
+    pts[:50, 0] = lo[0]
+    pts[50:100, 1] = hi[1]
+    pts[100:150, 2] = hi[2] - 5e-9
+    ids = g.hash(pts)
+    og = oracle.Sim(sc).info()["grid"]
+    ref = np.array([oracle.grid_hash(og, p)[0] for p in pts])
+    assert np.array_equal(ids, ref)
+
+
+def test_cell_neighbors_match_oracle(bbx, oracle):
+    sc = small_dam("sorted")
+    g = bbx.Solver(sc)
+    og = oracle.Sim(sc).info()["grid"]
+    total = og.total
+    for c in [0, 1, total - 1, total // 2, 25, 25 * 13, 137]:
+        assert g.cell_neighbors(c) == oracle.grid_neighbors(og, c)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_tiny_particle_counts(bbx, oracle, n):
+    import dataclasses
+
+    base = small_dam("sorted")
+    pos = base.positions[:n].copy()
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc)
+    st = g.pcisph_substep(sc.dt, 3, 0.01, bbx.MODE_TEXTBOOK)
+    assert st.iterations >= 1
+    if n:
+        o = oracle.Sim(gpu_positions_scene(sc, pos.astype(np.float32).astype(np.float64)))
+        o.pcisph_substep(sc.dt, 3, 0.01, oracle.MODE_TEXTBOOK)
+        np.testing.assert_allclose(g.get("position"), o.get("position"), rtol=0, atol=1e-6)

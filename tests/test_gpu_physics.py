@@ -1,0 +1,201 @@
+"""GPU parity: SPH / PCISPH physics (FP32 + Kahan) against the FP64 oracle.
+
+Tolerances are stated per quantity; the oracle runs the reference method in
+IEEE double on the same (FP32-rounded) inputs.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+# stated tolerances (FP32 device vs FP64 oracle)
+RTOL_DENSITY = 2e-6        # relative, per particle (Kahan-summed poly6)
+RTOL_FORCE = 2e-5          # relative to max |f|
+RTOL_PRESSURE = 2e-3       # relative to max |p|; p = delta * (rho* - rho0) cancels
+ATOL_POS_STEP = 2e-7       # absolute, metres, one substep (tank ~ 1 m, FP32 ulp 6e-8)
+
+
+def small_dam(seed=2, **kw):
+    return scenes.dam_break(32, 16, 32, spacing=0.01, tank=(0.5, 0.25, 0.5), seed=seed, **kw)
+
+
+def paired(bbx, oracle, sc, **over):
+    g = bbx.Solver(sc, **over)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")), **over)
+    return g, o
+
+
+def test_density_matches_oracle(bbx, oracle):
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    g.distribute()
+    o.distribute()
+    o.stage("interaction")
+    o.stage("density", 1)
+    gd, od = g.get("density"), o.get("density")
+    np.testing.assert_allclose(gd, od, rtol=RTOL_DENSITY, atol=0)
+
+
+def test_isolated_particle_density_closed_form(bbx, oracle):
+    """One particle: rho = m * 315 / (64 pi h^3) (poly6 at r = 0)."""
+    base = small_dam()
+    pos = np.array([[0.01, 0.02, -0.03]])
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc)
+    g.distribute()
+    inf = g.info()
+    h, m = inf["kernel_radius"], inf["mass"]
+    expect = m * 315.0 / (64.0 * np.pi * h ** 3)
+    assert abs(g.get("density")[0] - expect) <= 1e-6 * expect
+
+
+def test_nonpressure_force_matches_oracle(bbx, oracle):
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    # give the block some motion first so viscosity is non-zero
+    for _ in range(5):
+        g.pcisph_substep(sc.dt, 4, 0.01, bbx.MODE_TEXTBOOK)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")))
+    o.set("velocity", g.get("velocity"))
+    o.set_chains(*g.chains())
+    o.flag = g.flag
+    g.distribute()
+    g.stage_nonpressure()
+    o.distribute()
+    o.stage("interaction")
+    o.stage("density", 1)
+    o.stage("nonpressure")
+    gf, of = g.get("force"), o.get("force")
+    scale = np.abs(of).max()
+    assert np.abs(gf - of).max() <= RTOL_FORCE * scale
+
+
+@pytest.mark.parametrize("mode", ["exact", "textbook"])
+def test_one_pcisph_substep_matches_oracle(bbx, oracle, mode):
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    for _ in range(20):
+        g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")))
+    o.set("velocity", g.get("velocity"))
+    o.set_chains(*g.chains())
+    o.flag = g.flag
+    m = bbx.MODE_REFERENCE_EXACT if mode == "exact" else bbx.MODE_TEXTBOOK
+    gs = g.pcisph_substep(sc.dt, 8, 0.01, m)
+    os_ = o.pcisph_substep(sc.dt, 8, 0.01, m)
+    if mode == "exact":
+        assert gs.iterations == 1 and os_.iterations == 1
+    else:
+        assert abs(gs.iterations - os_.iterations) <= 1
+    assert abs(gs.max_density_error - os_.max_density_error) <= 1e-4
+    if gs.iterations == os_.iterations:
+        gp, op = g.get("pressure"), o.get("pressure")
+        assert np.abs(gp - op).max() <= RTOL_PRESSURE * max(np.abs(op).max(), 1.0)
+        np.testing.assert_allclose(g.get("position"), o.get("position"), rtol=0, atol=ATOL_POS_STEP)
+        dv = np.abs(g.get("velocity") - o.get("velocity")).max()
+        assert dv <= ATOL_POS_STEP / sc.dt
+
+
+def test_pcisph_multistep_drift_reference_exact(bbx, oracle):
+    """60 REFERENCE_EXACT substeps (fixed iteration count) from the same
+    start: FP32 + Kahan drifts from FP64 by less than 1e-3 h."""
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    for _ in range(60):
+        g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
+        o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+    dx = np.abs(g.get("position") - o.get("position")).max()
+    assert dx <= 1e-3 * sc.kernel_radius, dx
+    rel = np.abs(g.get("density") - o.get("density")) / o.get("density")
+    assert rel.max() <= 1e-4, rel.max()
+
+
+def test_pcisph_multistep_textbook_iterations(bbx, oracle):
+    """60 TEXTBOOK substeps: the convergence decision (FP64 max error vs 1%)
+    matches the oracle on most substeps; a borderline decision changes the
+    pressure by one correction, so positions are held to 1e-2 h."""
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    iters_g, iters_o, err_g, err_o = [], [], [], []
+    for _ in range(60):
+        a = g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+        b = o.pcisph_substep(sc.dt, 8, 0.01, oracle.MODE_TEXTBOOK)
+        iters_g.append(a.iterations)
+        iters_o.append(b.iterations)
+        err_g.append(a.max_density_error)
+        err_o.append(b.max_density_error)
+    agree = np.mean(np.array(iters_g) == np.array(iters_o))
+    assert agree >= 0.9, (iters_g, iters_o)
+    assert abs(np.mean(iters_g) - np.mean(iters_o)) <= 0.25
+    assert max(err_g) <= 0.01 + 1e-3 or max(iters_g) == 8
+    dx = np.abs(g.get("position") - o.get("position")).max()
+    assert dx <= 1e-2 * sc.kernel_radius, dx
+
+
+def test_free_fall_closed_form(bbx):
+    """Single particle, drag 0: symplectic Euler gives
+    y_n = y0 + g dt^2 n (n + 1) / 2 (src/equations/sph_equations3.cpp:283-339)."""
+    base = small_dam()
+    pos = np.array([[0.0, 0.05, 0.0]])
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc, drag_coefficient=0.0)
+    dt, n = 5e-4, 40
+    for _ in range(n):
+        g.pcisph_substep(dt, 3, 0.01, bbx.MODE_TEXTBOOK)
+    gy = float(np.float32(-9.8))
+    y = g.get("position")[0, 1]
+    expect = 0.05 + gy * dt * dt * n * (n + 1) / 2
+    assert abs(y - expect) < 2e-6
+    vy = g.get("velocity")[0, 1]
+    assert abs(vy - gy * dt * n) < 1e-5
+
+
+def test_wall_bounce_restitution(bbx, oracle):
+    """A particle crossing the floor is put at floor + radius with its normal
+    velocity reflected by restitution 0.6 (src/core/collider.cpp:3-44)."""
+    base = small_dam()
+    r = base.target_spacing
+    floor = -0.125
+    pos = np.array([[0.0, floor + r + 1e-4, 0.0]])
+    vel = np.array([[0.3, -2.0, 0.0]])
+    sc = dataclasses.replace(base, positions=pos, velocities=vel)
+    g = bbx.Solver(sc, drag_coefficient=0.0, gravity=(0.0, 0.0, 0.0))
+    g.pcisph_substep(5e-4, 3, 0.01, bbx.MODE_TEXTBOOK)
+    p, v = g.get("position")[0], g.get("velocity")[0]
+    assert abs(p[1] - (floor + r)) < 1e-6
+    assert abs(v[1] - 0.6 * 2.0) < 1e-5
+    assert abs(v[0] - 0.3) < 1e-6
+    o = oracle.Sim(dataclasses.replace(sc, positions=pos.astype(np.float32).astype(np.float64)),
+                   drag_coefficient=0.0, gravity=(0.0, 0.0, 0.0))
+    o.pcisph_substep(5e-4, 3, 0.01, oracle.MODE_TEXTBOOK)
+    np.testing.assert_allclose(p, o.get("position")[0], atol=1e-6)
+    np.testing.assert_allclose(v, o.get("velocity")[0], atol=1e-5)
+
+
+def test_sph_substep_matches_oracle(bbx, oracle):
+    sc = scenes.sph_dam_break()
+    sc = dataclasses.replace(sc, positions=sc.positions[:4096].copy(),
+                             velocities=sc.velocities[:4096].copy())
+    g = bbx.Solver(sc, kind=bbx.SPH)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")))
+    dt = 1.6e-4
+    for _ in range(3):
+        g.sph_substep(dt)
+        o.sph_substep(dt, oracle.SPH_TWO_PHASE)
+    np.testing.assert_allclose(g.get("position"), o.get("position"), rtol=0, atol=5e-6)
+
+
+def test_timestep_rule_matches_oracle(bbx, oracle):
+    sc = small_dam()
+    g, o = paired(bbx, oracle, sc)
+    # before any force: speed limit only
+    assert g.number_of_time_steps(1e-3, 5.0) == o.number_of_time_steps(1e-3, 5.0)
+    assert g.number_of_time_steps(1e-3, 1.0) == o.number_of_time_steps(1e-3, 1.0)
+    # PciSphSolver3::Advance reproduces the substep count of the oracle
+    ng = g.pcisph_advance(1.0 / 240.0, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
+    no = o.pcisph_advance(1.0 / 240.0, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+    assert ng == no
