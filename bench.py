@@ -100,6 +100,9 @@ def run_gpu(args):
                    "launches": v["launches"], "total_ms": round(v["total_ms"], 3)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
     solver.close()
+    gbs, ms = abi.stream_copy(2.0 ** 30, 20)
+    roofline["measured_stream_copy_GBs"] = round(gbs, 1)
+    roofline["frac_of_measured"] = round(achieved / gbs, 4) if gbs > 0 else None
     return sc, value, elapsed, kbar, float(np.max(errs)), roofline, whole, kernels
 
 
@@ -117,7 +120,14 @@ def run_cpu_baseline(sc, args):
         t += time.perf_counter() - t0
         done += 1
     del sim
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(sc.n * done / t, 1), "unit": "particle-steps/s", "cores": threads,
+            "cpu_model": cpu_model,
             "kind": "port",
             "sample": f"{done} timed substeps (after 1 warm-up) of the same {sc.n}-particle dam "
                       f"break, FP64 oracle with linked-list cells and 100-slot buckets, "
@@ -127,7 +137,7 @@ def run_cpu_baseline(sc, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--mode", choices=["textbook", "exact"], default="textbook")
     ap.add_argument("--warm", choices=["none", "mlp"], default="none")

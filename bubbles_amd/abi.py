@@ -124,6 +124,7 @@ def load():
         "bbx_profile_reset": (i, [vp]),
         "bbx_profile_get": (i, [vp, C.c_char_p, i, dp, ip, i, ip]),
         "bbx_solver_synchronize": (i, [vp]),
+        "bbx_stream_copy": (i, [d, i, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -143,6 +144,13 @@ def device_count() -> int:
     n = C.c_int(0)
     check(load().bbx_device_count(C.byref(n)), "bbx_device_count")
     return n.value
+
+
+def stream_copy(nbytes: float, iters: int = 20):
+    """(GB/s, avg ms) of a device copy of nbytes (read + write)."""
+    g, ms = C.c_double(), C.c_double()
+    check(load().bbx_stream_copy(float(nbytes), iters, C.byref(g), C.byref(ms)), "bbx_stream_copy")
+    return g.value, ms.value
 
 
 def _dp(a):

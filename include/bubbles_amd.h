@@ -219,6 +219,11 @@ int bbx_profile_reset(bbx_solver *s);
 int bbx_profile_get(bbx_solver *s, char *names, int names_cap, double *total_ms,
                     int *launches, int cap, int *count);
 int bbx_solver_synchronize(bbx_solver *s);
+/* STREAM-style device copy of `bytes` (read + write), `iters` launches:
+ * achievable HBM bandwidth (GB/s, 2*bytes per launch) and the average launch
+ * time.  Used to re-measure the bandwidth roof and to calibrate the
+ * FETCH_SIZE / WRITE_SIZE counters.  No counterpart in the reference. */
+int bbx_stream_copy(double bytes, int iters, double *gbps, double *avg_ms);
 
 #ifdef __cplusplus
 }

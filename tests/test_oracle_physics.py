@@ -1,0 +1,327 @@
+"""Pin the oracle's SPH / PCISPH physics with closed forms, invariants and
+independent brute force (the reference ships no numeric tests for these).
+
+  density        isolated particle closed form; perfect-lattice sum by brute force
+  mass, delta    BCC lattice sums recomputed independently with numpy
+  pressure force pairwise antisymmetry: sum_i f_p,i = 0
+  integration    free fall y_n = y0 + g dt^2 n(n+1)/2; drag decay (1 - dt c/m)^n
+  collider       wall hit: x = wall + r, v_n -> -e v_n (e = 0.6; 0 in predict)
+  PCISPH loop    REFERENCE_EXACT = one iteration; TEXTBOOK error decreases
+  time-step rule closed form of ComputeNumberOfTimeSteps
+  Tait EOS       p(rho0) = 0, p(1.1 rho0) closed form
+  XSPH           a uniform velocity field is a fixed point
+"""
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+
+GY = float(np.float32(-9.8))
+
+
+def base_scene(**kw):
+    return scenes.dam_break(16, 8, 16, spacing=0.01, tank=(0.4, 0.3, 0.4), seed=2, **kw)
+
+
+def with_points(sc, pos, vel=None):
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    vel = np.zeros_like(pos) if vel is None else np.asarray(vel, dtype=np.float64).reshape(-1, 3)
+    return dataclasses.replace(sc, positions=pos, velocities=vel)
+
+
+def poly6(h, r):
+    r = np.asarray(r, dtype=np.float64)
+    d2 = r * r
+    of = d2 - h * h
+    x = 1.0 - d2 / (h * h)
+    w = 315.0 / (64.0 * math.pi * h ** 3) * x ** 3
+    return np.where((np.abs(of) < 1e-8) | (of > 0), 0.0, w)
+
+
+def bcc_points_numpy(kr, spacing):
+    """BCC lattice over [-1.5 kr, 1.5 kr]^3: layers every spacing/2 along z,
+    odd layers offset by spacing/2 in x and y (src/generator/bcclattice.cpp)."""
+    lo, ext = -1.5 * kr, 3.0 * kr
+    half = spacing / 2
+    pts = []
+    k = 0
+    while k * half <= ext:
+        off = half if k % 2 else 0.0
+        j = 0
+        while j * spacing + off <= ext:
+            i = 0
+            while i * spacing + off <= ext:
+                pts.append((i * spacing + off + lo, j * spacing + off + lo, k * half + lo))
+                i += 1
+            j += 1
+        k += 1
+    return np.array(pts)
+
+
+def test_isolated_particle_density(oracle):
+    sc = with_points(base_scene(), [[0.01, 0.02, 0.03]])
+    s = oracle.Sim(sc)
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    inf = s.info()
+    expect = inf["mass"] * 315.0 / (64.0 * math.pi * inf["kernel_radius"] ** 3)
+    assert s.get("density")[0] == pytest.approx(expect, rel=1e-14)
+
+
+def test_perfect_lattice_density_brute_force(oracle):
+    """Interior particle of a perfect simple-cubic lattice: rho = m * sum of
+    poly6 over lattice offsets inside the radius (counted by brute force)."""
+    sc = scenes.dam_break(20, 20, 20, spacing=0.01, tank=(0.4, 0.4, 0.4), seed=0)
+    pos = scenes.lattice_block((20, 20, 20), 0.01, (-0.1, -0.1, -0.1), 0, jitter=0.0)
+    s = oracle.Sim(with_points(sc, pos))
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    inf = s.info()
+    h, m = inf["kernel_radius"], inf["mass"]
+    i = 10 + 20 * (10 + 20 * 10)
+    d = pos - pos[i]
+    r = np.sqrt((d[:, 0] ** 2 + d[:, 1] ** 2) + d[:, 2] ** 2)
+    expect = m * poly6(h, r).sum()
+    assert s.get("density")[i] == pytest.approx(expect, rel=1e-12)
+    # with the BCC-calibrated mass and targetSpacing = s 2^(1/3) the interior
+    # of the simple-cubic block sits within 1% of rest density
+    assert abs(s.get("density")[i] / 1000.0 - 1.0) < 0.01
+
+
+@pytest.mark.parametrize("spacing,rel", [(0.01 * 2 ** (1 / 3), 2 ** (2 / 3)), (0.02, 2.0), (0.1, 1.8)])
+def test_mass_and_delta_denominator_brute_force(oracle, spacing, rel):
+    kr = spacing * rel
+    pts = bcc_points_numpy(kr, spacing)
+    # mass: rho0 / max_i sum_j W(|p_i - p_j|) over <= 1024 BCC points
+    pm = pts[:1024]
+    d = pm[:, None, :] - pm[None, :, :]
+    r = np.sqrt((d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2)
+    nden = poly6(kr, r).sum(axis=1).max()
+    assert oracle.compute_mass(1000.0, spacing, kr) == pytest.approx(1000.0 / nden, rel=1e-12)
+    # delta denominator: -(sum grad)^2 - sum grad^2 over points with |p| < h
+    r0 = np.sqrt((pts[:, 0] ** 2 + pts[:, 1] ** 2) + pts[:, 2] ** 2)
+    sel = r0 * r0 < kr * kr
+    rr = r0[sel]
+    dirs = np.where(rr[:, None] > 0, pts[sel] / np.where(rr > 0, rr, 1)[:, None], 0.0)
+    x = 1.0 - rr / kr
+    dW = -45.0 / (math.pi * kr ** 4) * x * x
+    grad = -dW[:, None] * dirs
+    g1 = grad.sum(axis=0)
+    denom = -(g1 @ g1) - (grad * grad).sum()
+    assert oracle.delta_denom(spacing, kr) == pytest.approx(denom, rel=1e-10)
+
+
+def test_bcc_generator_counts(oracle):
+    pts = oracle.bcc_points((-1, -1, -1), (1, 1, 1), 1.0)
+    ref = bcc_points_numpy(2.0 / 3.0, 1.0)  # same box: [-1, 1]^3
+    assert len(pts) == len(ref)
+    np.testing.assert_allclose(np.sort(pts, axis=0), np.sort(ref, axis=0), atol=1e-15)
+
+
+def test_pressure_force_antisymmetric(oracle):
+    """sum_i f_p,i = 0 for any pressures: m^2 (p_i/rho_i^2 + p_j/rho_j^2) is
+    symmetric and gradW is antisymmetric in (i, j)."""
+    sc = base_scene()
+    s = oracle.Sim(sc)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        s.pcisph_substep(sc.dt, 3, 0.01, oracle.MODE_TEXTBOOK)
+    s.set("pressure", rng.uniform(0, 500, size=sc.n))
+    s.set("density_predicted", 1000 + rng.uniform(-5, 5, size=sc.n))
+    s.stage("pressure_force")
+    fp = s.get("pressure_force")
+    assert np.abs(fp.sum(axis=0)).max() <= 1e-12 * np.abs(fp).sum()
+    assert np.abs(fp).max() > 0
+
+
+def test_pressure_force_pair_direction(oracle):
+    """Two particles with positive pressure repel along the line between them."""
+    sc = with_points(base_scene(), [[0.0, 0.0, 0.0], [0.011, 0.0, 0.0]])
+    s = oracle.Sim(sc)
+    s.distribute()
+    s.set("pressure", np.array([100.0, 50.0]))
+    s.set("density_predicted", np.array([1000.0, 1000.0]))
+    s.stage("pressure_force")
+    fp = s.get("pressure_force")
+    assert fp[0, 0] < 0 < fp[1, 0]
+    assert fp[0, 0] == pytest.approx(-fp[1, 0], rel=1e-14)
+    inf = s.info()
+    h, m, r = inf["kernel_radius"], inf["mass"], 0.011
+    grad = 45.0 / (math.pi * h ** 4) * (1 - r / h) ** 2
+    assert fp[1, 0] == pytest.approx(m * m * (100.0 / 1e6 + 50.0 / 1e6) * grad, rel=1e-12)
+
+
+def test_free_fall_closed_form(oracle):
+    sc = with_points(base_scene(), [[0.0, 0.1, 0.0]])
+    s = oracle.Sim(sc, drag_coefficient=0.0)
+    dt, n = 5e-4, 50
+    for _ in range(n):
+        s.pcisph_substep(dt, 3, 0.01, oracle.MODE_TEXTBOOK)
+    y = s.get("position")[0, 1]
+    assert y == pytest.approx(0.1 + GY * dt * dt * n * (n + 1) / 2, abs=1e-14)
+    assert s.get("velocity")[0, 1] == pytest.approx(GY * dt * n, rel=1e-13)
+
+
+def test_drag_decay_closed_form(oracle):
+    sc = with_points(base_scene(), [[0.0, 0.0, 0.0]], [[1.0, 0.0, 0.0]])
+    s = oracle.Sim(sc, gravity=(0.0, 0.0, 0.0), drag_coefficient=1e-4)
+    m = s.info()["mass"]
+    dt, n = 5e-4, 30
+    for _ in range(n):
+        s.pcisph_substep(dt, 3, 0.01, oracle.MODE_TEXTBOOK)
+    assert s.get("velocity")[0, 0] == pytest.approx((1.0 - dt * 1e-4 / m) ** n, rel=1e-12)
+
+
+@pytest.mark.parametrize("axis,sign", [(1, -1), (0, 1), (2, -1)])
+def test_wall_hit_restitution(oracle, axis, sign):
+    sc = base_scene()
+    half = sc.tank_size[axis] / 2
+    r = sc.target_spacing
+    p = np.zeros(3)
+    p[axis] = sign * (half - r - 1e-4)
+    v = np.zeros(3)
+    v[axis] = sign * 2.0
+    v[(axis + 1) % 3] = 0.25
+    s = oracle.Sim(with_points(sc, [p], [v]), gravity=(0.0, 0.0, 0.0), drag_coefficient=0.0)
+    s.pcisph_substep(5e-4, 1, 0.01, oracle.MODE_TEXTBOOK)
+    # predict used restitution 0: the normal predicted velocity is zero
+    assert s.get("temp_velocity")[0, axis] == pytest.approx(0.0, abs=1e-15)
+    x, u = s.get("position")[0], s.get("velocity")[0]
+    assert x[axis] == pytest.approx(sign * (half - r), abs=1e-15)
+    assert u[axis] == pytest.approx(-sign * 0.6 * 2.0, rel=1e-14)
+    assert u[(axis + 1) % 3] == pytest.approx(0.25, rel=1e-14)
+
+
+def test_corner_hit_uses_nearest_face_only(oracle):
+    """Only the single nearest face is resolved per call
+    (ColliderSet3::ResolveCollision picks one collider, BoxClosestPoint one plane)."""
+    sc = base_scene()
+    hx, hy = sc.tank_size[0] / 2, sc.tank_size[1] / 2
+    r = sc.target_spacing
+    p = [hx - r + 0.004, -hy + r - 0.001, 0.0]   # deeper into the x wall
+    s = oracle.Sim(with_points(sc, [p]), gravity=(0.0, 0.0, 0.0), drag_coefficient=0.0)
+    s.pcisph_substep(5e-4, 1, 0.01, oracle.MODE_TEXTBOOK)
+    x = s.get("position")[0]
+    assert x[0] == pytest.approx(hx - r, abs=1e-15)       # pushed off the nearest (x) face
+    assert x[1] == pytest.approx(-hy + r - 0.001, abs=1e-15)  # y face untouched this step
+
+
+def test_reference_exact_is_one_iteration(oracle):
+    sc = base_scene()
+    s = oracle.Sim(sc)
+    for _ in range(5):
+        st = s.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+        assert st.iterations == 1
+
+
+def test_textbook_error_decreases_with_iterations(oracle):
+    """On a block compressed to 0.97 of its spacing (~10% over rest density)
+    the predicted max error falls with each of the first correction
+    iterations; uniform interior compression has no pressure gradient, so the
+    loop cannot reach 1% in one substep and stops at maxIt."""
+    sc = scenes.dam_break(12, 12, 12, spacing=0.0097, tank=(0.4, 0.4, 0.4), seed=3)
+    sc = dataclasses.replace(sc, target_spacing=0.01 * 2 ** (1 / 3),
+                             relative_radius=0.02 / (0.01 * 2 ** (1 / 3)))
+    errs = []
+    for k in range(1, 6):
+        s = oracle.Sim(sc)
+        st = s.pcisph_substep(5e-4, k, 0.0, oracle.MODE_TEXTBOOK)
+        assert st.iterations == k
+        errs.append(st.max_density_error)
+    assert errs[0] > 0.01
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+    s = oracle.Sim(sc)
+    st = s.pcisph_substep(5e-4, 8, 0.01, oracle.MODE_TEXTBOOK)
+    assert st.iterations == 8 and st.max_density_error > 0.01
+
+
+def test_textbook_converges_at_rest(oracle):
+    """A block at rest density converges in one iteration below 1%."""
+    sc = base_scene()
+    s = oracle.Sim(sc)
+    st = s.pcisph_substep(sc.dt, 8, 0.01, oracle.MODE_TEXTBOOK)
+    assert st.iterations == 1 and 0 <= st.max_density_error < 0.01
+
+
+def test_timestep_rule_closed_form(oracle):
+    sc = base_scene()
+    s = oracle.Sim(sc)
+    kr, m = s.info()["kernel_radius"], s.info()["mass"]
+    # forces zero: speed limit 0.4 h / c
+    n = s.number_of_time_steps(1e-3, 5.0)
+    assert n == math.ceil(1e-3 / (5.0 * 0.4 * kr / 100.0))
+    f = np.zeros((sc.n, 3))
+    f[7] = [0.0, 3.0, 4.0]  # |f| = 5
+    s.set("force", f)
+    lim = min(0.25 * math.sqrt(kr * m / 5.0), 0.4 * kr / 100.0)
+    assert s.number_of_time_steps(1e-3, 1.0) == math.ceil(1e-3 / lim)
+
+
+def test_tait_pressure_closed_form(oracle):
+    sc = with_points(base_scene(), [[0.0, 0.0, 0.0]])
+    s = oracle.Sim(sc)
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    rho = s.get("density")[0]
+    expect = 1000.0 * 100.0 ** 2 / 7.0 * ((rho / 1000.0) ** 7 - 1.0)
+    expect = 0.0 if expect < 0 else expect  # negativePressureScale = 0
+    assert s.get("pressure")[0] == pytest.approx(expect, abs=1e-9)
+    s2 = oracle.Sim(sc, negative_pressure_scale=1.0)
+    s2.distribute()
+    s2.stage("interaction")
+    s2.stage("density", 1)
+    e2 = 1000.0 * 100.0 ** 2 / 7.0 * ((rho / 1000.0) ** 7 - 1.0)
+    assert s2.get("pressure")[0] == pytest.approx(e2, rel=1e-12)
+
+
+def test_xsph_uniform_field_is_fixed_point(oracle):
+    """XSPH smoothing only runs for pseudoViscosity * dt > 0.1 and leaves a
+    uniform velocity field unchanged (normalised weights)."""
+    sc = base_scene()
+    s = oracle.Sim(sc)
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    v = np.tile([0.3, -0.2, 0.1], (sc.n, 1))
+    s.set("velocity", v)
+    s.stage("xsph", 0.02)  # 10 * 0.02 = 0.2 > 0.1: active
+    np.testing.assert_allclose(s.get("velocity"), v, rtol=1e-12)
+    w = v.copy()
+    w[::3] *= -1
+    s.set("velocity", w)
+    s.stage("xsph", 0.005)  # 0.05 < 0.1: inactive
+    assert np.array_equal(s.get("velocity"), w)
+    s.stage("xsph", 0.02)
+    assert not np.array_equal(s.get("velocity"), w)
+
+
+def test_sph_modes_close(oracle):
+    """Sequential in-place integration (the reference's single-thread loop) and
+    the race-free two-phase form differ only at neighbour-update order."""
+    sc = scenes.sph_dam_break()
+    sc = dataclasses.replace(sc, positions=sc.positions[:2048].copy(),
+                             velocities=sc.velocities[:2048].copy())
+    a = oracle.Sim(sc)
+    b = oracle.Sim(sc)
+    for _ in range(5):
+        a.sph_substep(1.6e-4, oracle.SPH_TWO_PHASE)
+        b.sph_substep(1.6e-4, oracle.SPH_SEQUENTIAL)
+    d = np.abs(a.get("position") - b.get("position")).max()
+    assert 0 < d < 1e-4
+
+
+def test_threads_do_not_change_results(oracle):
+    sc = base_scene()
+    a = oracle.Sim(sc, threads=1)
+    b = oracle.Sim(sc, threads=4)
+    for _ in range(4):
+        a.pcisph_substep(sc.dt, 6, 0.01, oracle.MODE_TEXTBOOK)
+        b.pcisph_substep(sc.dt, 6, 0.01, oracle.MODE_TEXTBOOK)
+    assert np.array_equal(a.get("position"), b.get("position"))
+    assert np.array_equal(a.chains()[1], b.chains()[1])
