@@ -25,8 +25,9 @@ BUCKET = 100
 NFEAT = 16
 
 FIELD = {"position": 0, "velocity": 1, "force": 2, "density": 3, "pressure": 4,
-         "density_predicted": 5, "pressure_force": 6, "temp_position": 7, "features": 8}
-FIELD_WIDTH = {0: 3, 1: 3, 2: 3, 3: 1, 4: 1, 5: 1, 6: 3, 7: 3, 8: 16}
+         "density_predicted": 5, "pressure_force": 6, "temp_position": 7, "features": 8,
+         "warm_pressure": 9}
+FIELD_WIDTH = {0: 3, 1: 3, 2: 3, 3: 1, 4: 1, 5: 1, 6: 3, 7: 3, 8: 16, 9: 1}
 
 
 class BbxError(RuntimeError):

@@ -602,6 +602,7 @@ int bbx_solver_download(bbx_solver *s, int field, double *out) {
         case BBX_FIELD_DENSITY: return scal(s->rho);
         case BBX_FIELD_PRESSURE: return scal(s->pres);
         case BBX_FIELD_DENSITY_PREDICTED: return scal(s->rho_star);
+        case BBX_FIELD_WARM_PRESSURE: return scal(s->pwarm);
         case BBX_FIELD_FEATURES: {
             std::vector<float> h(n * BBX_NFEAT);
             for (int f = 0; f < BBX_NFEAT; f++)
@@ -892,7 +893,8 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     st.delta = (float)delta;
     if (warm == BBX_WARM_MLP) {
         BBX_CHECK(run_nonpressure(s, st, 0, 1));
-        BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride, s->pwarm, s->stream));
+        BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride,
+                                   (float)(delta * s->prm.target_density), s->pwarm, s->stream));
         BBX_CHECK(run_warm_init(s));
         // f_p(p_warm, rho) and the k = 0 prediction
         BBX_CHECK(run_pressure_force(s, st, 0));

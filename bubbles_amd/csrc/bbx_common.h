@@ -182,6 +182,6 @@ int run_integrate(bbx_solver *s, const StepConst &st);
 int run_xsph(bbx_solver *s, const StepConst &st);
 int run_warm_init(bbx_solver *s);
 int swap_after_integrate(bbx_solver *s);
-int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float *out,
+int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float scale, float *out,
                      hipStream_t stream);
 }  // namespace bbx

@@ -1,0 +1,187 @@
+"""Learned PCISPH pressure warm start: harvest, train, evaluate.
+
+Data: (16 features at the start of a substep -> converged pressure / P) pairs
+harvested from plain TEXTBOOK PCISPH runs on seeded dam-break and
+block-drop scenes (P = delta * rho0, the pressure one correction assigns to a
+100% density error).  The network (16 -> 64 -> 64 -> 1, GELU, BF16 MFMA,
+FP32 accumulation) is trained on the device with Adam by the library's own
+forward/backward kernels.  At run time the solver starts its predict-correct
+loop from p = P * max(0, MLP(features)) and the unchanged loop still iterates
+to the same tolerance.
+
+CLI:
+  python -m bubbles_amd.learned train  [--out PATH] [--quick]
+  python -m bubbles_amd.learned eval   [--weights PATH] [--steps N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import abi, scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "data", "warmstart_mlp.npz")
+
+TRAIN_SEEDS = list(range(1000, 1032))
+HELDOUT_SEEDS = list(range(2000, 2008))
+
+
+def training_scenes(seeds=TRAIN_SEEDS, scale=1.0):
+    """Half dam breaks, half block drops, sizes varied by seed; spacing 0.01
+    and dt 5e-4 like the benchmark scene."""
+    out = []
+    rng = np.random.default_rng(12345)
+    for k, seed in enumerate(seeds):
+        if k % 2 == 0:
+            nx = int(round(rng.integers(48, 97) * scale))
+            ny = int(round(rng.integers(24, 65) * scale))
+            nz = int(round(rng.integers(48, 97) * scale))
+            out.append(scenes.dam_break(max(nx, 8), max(ny, 8), max(nz, 8), spacing=0.01,
+                                        tank=(2.0, 1.0, 2.0), seed=seed))
+        else:
+            out.append(scenes.block_drop(scale=0.25 * scale, seed=seed, spacing=0.01, dt=5e-4))
+    return out
+
+
+def harvest(scene_list, steps=400, every=25, start=25, per_snapshot=8192, max_it=8, tol=0.01,
+            seed=0, log=print):
+    """Run plain TEXTBOOK PCISPH on each scene; every `every` substeps record a
+    random subset of (features, converged p / P)."""
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    for si, sc in enumerate(scene_list):
+        s = abi.Solver(sc)
+        iters = []
+        for step in range(steps):
+            if step >= start and (step - start) % every == 0:
+                f, t, st = s.harvest(sc.dt, max_it, tol)
+                k = min(per_snapshot, sc.n)
+                sel = rng.choice(sc.n, size=k, replace=False)
+                xs.append(f[sel])
+                ts.append(t[sel])
+            else:
+                st = s.pcisph_substep(sc.dt, max_it, tol, abi.MODE_TEXTBOOK)
+            iters.append(st.iterations)
+        s.close()
+        log(f"harvest scene {si} {sc.name} n={sc.n} mean_it={np.mean(iters):.2f}")
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+
+
+def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0):
+    net = abi.Mlp(seed)
+    p = net.get_params()
+    from_ = abi.Mlp.param_count()
+    mean = x.mean(axis=0)
+    std = x.std(axis=0)
+    std[std < 1e-12] = 1.0
+    nf = abi.NFEAT
+    off_mean = from_ - 2 * nf - 1
+    p[off_mean:off_mean + nf] = mean
+    p[off_mean + nf:off_mean + 2 * nf] = 1.0 / std
+    p[-1] = max(float(t.std()), 1e-6)
+    net.set_params(p)
+    loss = net.train(x, t, steps=steps, batch=batch, lr=lr, seed=seed)
+    return net, loss
+
+
+def save(net, path=DEFAULT_WEIGHTS, meta=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=net.get_params(), meta=json.dumps(meta or {}))
+
+
+def load(path=DEFAULT_WEIGHTS):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path}: train it with `python -m bubbles_amd.learned train`")
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"].astype(np.float32)
+    net = abi.Mlp(0)
+    net.set_params(params)
+    return net
+
+
+def load_or_train(solver=None, scene=None, path=DEFAULT_WEIGHTS):
+    """Trained weights from `path` (the benchmark never trains inside timing)."""
+    return load(path)
+
+
+def compare(scene, net, steps=200, warmup=0, max_it=8, tol=0.01):
+    """Plain vs warm-started TEXTBOOK PCISPH on the same scene: iterations,
+    errors, time and the final state difference."""
+    res = {}
+    finals = {}
+    for label, warm in (("plain", abi.WARM_NONE), ("learned", abi.WARM_MLP)):
+        s = abi.Solver(scene)
+        if warm == abi.WARM_MLP:
+            s.attach_mlp(net)
+        for _ in range(warmup):
+            s.pcisph_substep(scene.dt, max_it, tol, abi.MODE_TEXTBOOK, abi.WARM_NONE)
+        s.synchronize()
+        it, err, ms = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = s.pcisph_substep(scene.dt, max_it, tol, abi.MODE_TEXTBOOK, warm)
+            it.append(st.iterations)
+            err.append(st.max_density_error)
+            ms.append(st.gpu_ms)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        res[label] = {"mean_iterations": float(np.mean(it)), "max_iterations": int(np.max(it)),
+                      "max_density_error": float(np.max(err)),
+                      "converged_fraction": float(np.mean(np.array(err) < tol)),
+                      "particle_steps_per_s": scene.n * steps / el, "gpu_ms_per_step": float(np.mean(ms))}
+        rho = s.get("density")
+        res[label]["mean_abs_density_dev"] = float(np.mean(np.abs(rho / scene.target_density - 1.0)))
+        finals[label] = s.get("position")
+        s.close()
+    res["speedup"] = res["learned"]["particle_steps_per_s"] / res["plain"]["particle_steps_per_s"]
+    res["max_position_diff_over_h"] = float(np.abs(finals["plain"] - finals["learned"]).max() / scene.kernel_radius)
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    tr = sub.add_parser("train")
+    tr.add_argument("--out", default=DEFAULT_WEIGHTS)
+    tr.add_argument("--quick", action="store_true")
+    tr.add_argument("--steps", type=int, default=20000)
+    tr.add_argument("--batch", type=int, default=4096)
+    tr.add_argument("--lr", type=float, default=1e-3)
+    ev = sub.add_parser("eval")
+    ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ev.add_argument("--steps", type=int, default=480)
+    ev.add_argument("--warmup", type=int, default=20)
+    ev.add_argument("--nx", type=int, default=128)
+    ev.add_argument("--ny", type=int, default=64)
+    ev.add_argument("--nz", type=int, default=128)
+    ev.add_argument("--seed", type=int, default=HELDOUT_SEEDS[0])
+    args = ap.parse_args(argv)
+    if args.cmd == "train":
+        t0 = time.perf_counter()
+        seeds = TRAIN_SEEDS[:4] if args.quick else TRAIN_SEEDS
+        scale = 0.5 if args.quick else 1.0
+        x, t = harvest(training_scenes(seeds, scale), steps=200 if args.quick else 400,
+                       per_snapshot=2048 if args.quick else 8192)
+        t1 = time.perf_counter()
+        print(f"harvested {x.shape[0]} samples in {t1 - t0:.1f} s; target mean {t.mean():.4f} std {t.std():.4f}")
+        net, loss = fit(x, t, steps=200 if args.quick else args.steps, batch=args.batch, lr=args.lr)
+        t2 = time.perf_counter()
+        print(f"trained {args.steps} Adam steps in {t2 - t1:.1f} s, final batch loss {loss:.5f}")
+        save(net, args.out, {"samples": int(x.shape[0]), "steps": args.steps, "batch": args.batch,
+                             "lr": args.lr, "seeds": seeds, "final_loss": loss})
+        print("saved", args.out)
+    else:
+        net = load(args.weights)
+        sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=args.seed)
+        print(json.dumps(compare(sc, net, steps=args.steps, warmup=args.warmup)))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

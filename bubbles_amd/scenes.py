@@ -126,13 +126,15 @@ def sph_dam_break(seed=1) -> Scene:
                   0.02, 0.04, (1.0, 1.0, 1.0), seed, 0.04, 1e-3)
 
 
-def block_drop(scale=1.0, seed=3, spacing=0.005, tank=(4.0, 2.0, 4.0), dt=2.5e-4) -> Scene:
-    """PCISPH block drop: a cube falling onto a resting layer.  At scale 1 the
-    cube is 64^3 and the layer 512x32x496, 2^23 particles in total; smaller
-    ``scale`` shrinks every count (and the tank) for parity-size runs."""
+def block_drop(scale=1.0, seed=3, spacing=0.005, dt=2.5e-4) -> Scene:
+    """PCISPH block drop: a cube falling onto a resting layer.  At scale 1 and
+    spacing 0.005 the cube is 64^3 and the layer 512x32x496 (2^23 particles)
+    in a 4x2x4 tank; ``scale`` shrinks every count, and the tank follows
+    scale * spacing / 0.005 so the layer always fits."""
     c = max(2, int(round(64 * scale)))
     lx, ly, lz = max(2, int(round(512 * scale))), max(2, int(round(32 * scale))), max(2, int(round(496 * scale)))
-    tank = tuple(t * scale for t in tank) if scale != 1.0 else tank
+    f = scale * spacing / 0.005
+    tank = (4.0 * f, 2.0 * f, 4.0 * f)
     layer_h = ly * spacing
     drop_y = layer_h + 0.4 * tank[1]
     cx = 0.5 * tank[0] - 0.5 * c * spacing

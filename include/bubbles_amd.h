@@ -60,6 +60,7 @@ extern "C" {
 #define BBX_FIELD_PRESSURE_FORCE 6    /* 3 doubles */
 #define BBX_FIELD_TEMP_POSITION 7     /* 3 doubles: predicted x* */
 #define BBX_FIELD_FEATURES 8          /* 16 doubles: warm-start features */
+#define BBX_FIELD_WARM_PRESSURE 9     /* 1 double: MLP initial pressure of the last warm substep */
 
 typedef struct bbx_solver bbx_solver;
 typedef struct bbx_mlp bbx_mlp;

@@ -1,0 +1,157 @@
+"""GPU parity of the BF16-MFMA warm-start MLP against the numpy oracle
+(oracle/mlp_ref.py, BF16 rounding emulated at the device's operand points).
+
+Stated tolerances: outputs within 2e-4 of max |y|; gradients within 2e-3 of
+the per-tensor max |g| (FP32 accumulation order and the exp-based tanh are
+the only differences left after emulation).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+TENSORS = {"W1": (M.OFF_W1, M.OFF_B1), "b1": (M.OFF_B1, M.OFF_W2), "W2": (M.OFF_W2, M.OFF_B2),
+           "b2": (M.OFF_B2, M.OFF_W3), "W3": (M.OFF_W3, M.OFF_B3), "b3": (M.OFF_B3, M.N_TRAIN)}
+
+
+def random_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(M.N_PARAMS, dtype=np.float32)
+    p[M.OFF_W1:M.OFF_B1] = rng.normal(scale=0.3, size=M.H * M.F)
+    p[M.OFF_B1:M.OFF_W2] = rng.normal(scale=0.1, size=M.H)
+    p[M.OFF_W2:M.OFF_B2] = rng.normal(scale=0.15, size=M.H * M.H)
+    p[M.OFF_B2:M.OFF_W3] = rng.normal(scale=0.1, size=M.H)
+    p[M.OFF_W3:M.OFF_B3] = rng.normal(scale=0.2, size=M.H)
+    p[M.OFF_B3] = 0.05
+    p[M.OFF_MEAN:M.OFF_ISTD] = rng.normal(scale=0.5, size=M.F)
+    p[M.OFF_ISTD:M.OFF_OSCALE] = rng.uniform(0.5, 2.0, size=M.F)
+    p[M.OFF_OSCALE] = 1.7
+    return p
+
+
+def test_param_count(bbx):
+    assert bbx.Mlp.param_count() == M.N_PARAMS
+
+
+def test_infer_matches_oracle(bbx):
+    rng = np.random.default_rng(0)
+    p = random_params(0)
+    net = bbx.Mlp(0)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    x = rng.normal(size=(4100, M.F)).astype(np.float32)  # not a multiple of 16
+    y = net.infer(x)
+    ref = M.predict(p, x, emulate=True)
+    assert np.abs(y - ref).max() <= 2e-4 * np.abs(ref).max()
+    ref64 = M.predict(p, x, emulate=False)
+    assert np.abs(y - ref64).max() <= 5e-2 * np.abs(ref64).max()
+
+
+@pytest.mark.parametrize("n", [16, 1000, 5000])
+def test_gradients_match_oracle(bbx, n):
+    rng = np.random.default_rng(n)
+    p = random_params(1)
+    net = bbx.Mlp(1)
+    net.set_params(p)
+    x = rng.normal(size=(n, M.F)).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    g, loss = net.gradients(x, t)
+    lref, gref = M.loss_and_grad(p, x, t, emulate=True)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    for name, (a, b) in TENSORS.items():
+        scale = np.abs(gref[a:b]).max()
+        assert np.abs(g[a:b] - gref[a:b]).max() <= 2e-3 * scale + 1e-9, name
+    assert np.all(g[M.N_TRAIN:] == 0)
+
+
+def test_gradients_deterministic(bbx):
+    rng = np.random.default_rng(5)
+    net = bbx.Mlp(2)
+    net.set_params(random_params(2))
+    x = rng.normal(size=(3000, M.F)).astype(np.float32)
+    t = rng.normal(size=3000).astype(np.float32)
+    g1, l1 = net.gradients(x, t)
+    g2, l2 = net.gradients(x, t)
+    assert np.array_equal(g1, g2) and l1 == l2
+
+
+def test_adam_step_matches_oracle(bbx):
+    rng = np.random.default_rng(7)
+    p = random_params(3)
+    net = bbx.Mlp(3)
+    net.set_params(p)
+    x = rng.normal(size=(2048, M.F)).astype(np.float32)
+    t = rng.normal(size=2048).astype(np.float32)
+    lr = 1e-3
+    net.train_step(x, t, lr)
+    got = net.get_params()
+    _, g = M.loss_and_grad(p, x, t, emulate=True)
+    ref, _, _ = M.adam_step(p, g, np.zeros(M.N_TRAIN), np.zeros(M.N_TRAIN), 1, lr)
+    clear = np.abs(g[:M.N_TRAIN]) > 1e-2 * np.abs(g[:M.N_TRAIN]).max()
+    d = np.abs(got[:M.N_TRAIN] - ref[:M.N_TRAIN])[clear]
+    assert d.max() <= 1e-2 * lr
+    assert np.array_equal(got[M.N_TRAIN:], p[M.N_TRAIN:])
+
+
+def test_training_fits_smooth_target(bbx):
+    """Adam on the device fits a smooth synthetic target (loss / 10)."""
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(65536, M.F)).astype(np.float32)
+    t = (np.tanh(x[:, 0] - 0.5 * x[:, 3]) + 0.3 * x[:, 7] ** 2).astype(np.float32)
+    net = bbx.Mlp(4)
+    p = net.get_params()
+    p[M.OFF_MEAN:M.OFF_ISTD] = x.mean(axis=0)
+    p[M.OFF_ISTD:M.OFF_OSCALE] = 1.0 / x.std(axis=0)
+    p[M.OFF_OSCALE] = t.std()
+    net.set_params(p)
+    _, l0 = net.gradients(x[:8192], t[:8192])
+    net.train(x, t, steps=1500, batch=2048, lr=2e-3, seed=1)
+    _, l1 = net.gradients(x[:8192], t[:8192])
+    assert l1 < 0.1 * l0, (l0, l1)
+
+
+def test_warm_start_substep_matches_oracle(bbx, oracle):
+    """A warm-started PCISPH substep equals the oracle's loop started from the
+    same initial pressures (features recomputed by the oracle in FP64)."""
+    sc = scenes.dam_break(32, 16, 32, spacing=0.01, tank=(0.5, 0.25, 0.5), seed=2)
+    g = bbx.Solver(sc)
+    for _ in range(10):
+        g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+    net = bbx.Mlp(5)
+    p = net.get_params()
+    p[M.OFF_W3:M.OFF_B3] *= 0.1
+    p[M.OFF_B3] = 0.02
+    p[M.OFF_OSCALE] = 1.0
+    net.set_params(p)
+    g.attach_mlp(net)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")))
+    o.set("velocity", g.get("velocity"))
+    o.set("pressure", g.get("pressure"))
+    o.set_chains(*g.chains())
+    o.flag = g.flag
+    a = g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK, bbx.WARM_MLP)
+    pw = g.get("warm_pressure")
+    feats = g.get("features")
+    assert np.all(pw >= 0) and pw.max() > 0
+    # features at the start of the substep
+    o2 = oracle.Sim(dataclasses.replace(sc, positions=o.get("position")))
+    o2.set("velocity", o.get("velocity"))
+    o2.set("pressure", o.get("pressure"))
+    o2.set_chains(*o.chains())
+    o2.flag = o.flag
+    o2.distribute()
+    o2.stage("interaction")
+    o2.stage("density", 1)
+    o2.stage("nonpressure")
+    of = o2.features(sc.dt)
+    scale = np.abs(of).max(axis=0) + 1e-12
+    assert (np.abs(feats - of) / scale).max() <= 1e-3
+    b = o.pcisph_substep(sc.dt, 8, 0.01, oracle.MODE_TEXTBOOK, warm=pw)
+    assert abs(a.iterations - b.iterations) <= 1
+    if a.iterations == b.iterations:
+        np.testing.assert_allclose(g.get("position"), o.get("position"), atol=5e-7)
