@@ -101,6 +101,8 @@ def load():
         "bbx_grid_set_chains": (i, [vp, ip, ip]),
         "bbx_grid_hash": (i, [vp, dp, i, ip]),
         "bbx_grid_cell_neighbors": (i, [vp, i, ip, ip]),
+        "bbx_grid_describe": (i, [C.POINTER(Params), C.POINTER(GridDesc)]),
+        "bbx_grid_hash_desc": (i, [C.POINTER(GridDesc), dp, i, ip, ip]),
         "bbx_neighbors_get": (i, [vp, ip, ip]),
         "bbx_stage_density": (i, [vp]),
         "bbx_stage_nonpressure": (i, [vp]),

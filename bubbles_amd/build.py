@@ -62,6 +62,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_dam_break.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_dam_break")
+
+
+def build_facade_example() -> str:
+    """Compile the C++ facade scene (host code only) against the library."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, FACADE_SRC, "-o", FACADE_BIN,
+           "-L" + PKG, "-lbubbles_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../bubbles_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"facade example failed to build:\n{res.stdout}\n{res.stderr}")
+    return FACADE_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_facade_example()
     sys.exit(0)

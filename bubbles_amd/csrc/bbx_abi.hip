@@ -334,7 +334,7 @@ static void free_all(bbx_solver *s) {
     dfree(s->cellA); dfree(s->cellB); dfree(s->home);
     dfree(s->cell_count); dfree(s->cell_start); dfree(s->scan_tmp);
     dfree(s->key); dfree(s->newc); dfree(s->rank_tmp); dfree(s->tmp_idx); dfree(s->hash_tmp);
-    dfree(s->nbr); dfree(s->nbr_cnt);
+    dfree(s->nbr16); dfree(s->nbr32); dfree(s->nbr_cnt); dfree(s->tile_tab);
     dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->q);
     dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
     dfree(s->max_err); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
@@ -461,8 +461,11 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->rank_tmp, N);
     if (!rc) rc = dalloc(&s->tmp_idx, N);
     if (!rc) rc = dalloc(&s->hash_tmp, N);
-    if (!rc) rc = dalloc(&s->nbr, N * kBucket);
+    s->ntiles = (int)((N + kBlock - 1) / kBlock);
+    if (!rc) rc = dalloc(&s->nbr16, N * kBucket);
+    if (!rc) rc = dalloc(&s->nbr32, N * kBucket);
     if (!rc) rc = dalloc(&s->nbr_cnt, N);
+    if (!rc) rc = dalloc(&s->tile_tab, (size_t)s->ntiles * kTabIntsHost);
     if (!rc) rc = dalloc(&s->rho, N);
     if (!rc) rc = dalloc(&s->pres, N);
     if (!rc) rc = dalloc(&s->rho_star, N);
@@ -508,6 +511,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (e == hipSuccess) e = hipMemset(s->rho_star, 0, sizeof(float) * N);
     if (e == hipSuccess) e = hipMemset(s->q, 0, sizeof(float) * N);
     if (e == hipSuccess) e = hipMemset(s->nbr_cnt, 0, sizeof(int) * N);
+    if (e == hipSuccess) e = hipMemset(s->tile_tab, 0, sizeof(int) * (size_t)s->ntiles * kTabIntsHost);
     if (e == hipSuccess) e = hipMemset(s->feat, 0, sizeof(float) * N * BBX_NFEAT);
     if (e == hipSuccess) e = hipMemset(s->max_force, 0, sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(s->max_err, 0, sizeof(unsigned long long));
@@ -792,6 +796,55 @@ int bbx_grid_cell_neighbors(bbx_solver *s, int cell, int *out27, int *count) {
     return BBX_OK;
 }
 
+static void desc_from_geom(const GridGeom &g, bbx_grid_desc *out) {
+    for (int k = 0; k < 3; k++) {
+        out->usizes[k] = (unsigned)g.n[k];
+        out->cells_len[k] = g.len[k];
+        out->min_point[k] = g.minp[k];
+        out->bounds_min[k] = g.bmin[k];
+        out->bounds_max[k] = g.bmax[k];
+    }
+    out->total = g.total;
+}
+
+int bbx_grid_describe(const bbx_params *params, bbx_grid_desc *out) {
+    if (!params || !out) return fail(BBX_E_INVALID, "null argument");
+    GridGeom g{};
+    BBX_CHECK(build_grid_geom(*params, &g));
+    desc_from_geom(g, out);
+    return BBX_OK;
+}
+
+int bbx_grid_hash_desc(const bbx_grid_desc *grid, const double *points, int n, int *cell_ids, int *u) {
+    if (!grid || n < 0 || (n > 0 && (!points || !cell_ids))) return fail(BBX_E_INVALID, "bad arguments");
+    GridGeom g{};
+    for (int k = 0; k < 3; k++) {
+        g.n[k] = (int)grid->usizes[k];
+        g.len[k] = grid->cells_len[k];
+        g.minp[k] = grid->min_point[k];
+        g.bmin[k] = grid->bounds_min[k];
+        g.bmax[k] = grid->bounds_max[k];
+    }
+    g.total = grid->total;
+    for (int i = 0; i < n; i++) {
+        const double *p = &points[3 * (size_t)i];
+        cell_ids[i] = (int)host_hash(g, p);
+        if (u) {
+            long long id = cell_ids[i];
+            if (id < 0) {
+                u[3 * i] = u[3 * i + 1] = u[3 * i + 2] = -1;
+            } else {
+                int plane = g.n[0] * g.n[1];
+                u[3 * i + 2] = (int)(id / plane);
+                int rem = (int)(id - (long long)u[3 * i + 2] * plane);
+                u[3 * i + 1] = rem / g.n[0];
+                u[3 * i] = rem - u[3 * i + 1] * g.n[0];
+            }
+        }
+    }
+    return BBX_OK;
+}
+
 int bbx_neighbors_get(bbx_solver *s, int *counts, int *out_ids) {
     BBX_CHECK(check_solver(s));
     if (!counts) return fail(BBX_E_INVALID, "null output");
@@ -803,14 +856,31 @@ int bbx_neighbors_get(bbx_solver *s, int *counts, int *out_ids) {
     BBX_HIP_CHECK(hipMemcpy(cnt.data(), s->nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) counts[ids[i]] = cnt[i];
     if (out_ids) {
-        std::vector<int> row(n);
+        // per-tile tables translate LDS-local entries back to storage slots
+        const int ntiles = (int)((n + kBlock - 1) / kBlock);
+        std::vector<int> tab((size_t)ntiles * kTabIntsHost);
+        BBX_HIP_CHECK(hipMemcpy(tab.data(), s->tile_tab, sizeof(int) * tab.size(), hipMemcpyDeviceToHost));
+        std::vector<int> row32(n);
+        std::vector<unsigned short> row16(n);
         for (int k = 0; k < kBucket; k++) {
-            BBX_HIP_CHECK(hipMemcpy(row.data(), s->nbr + (size_t)k * s->stride, sizeof(int) * n,
+            BBX_HIP_CHECK(hipMemcpy(row32.data(), s->nbr32 + (size_t)k * s->stride, sizeof(int) * n,
                                     hipMemcpyDeviceToHost));
+            BBX_HIP_CHECK(hipMemcpy(row16.data(), s->nbr16 + (size_t)k * s->stride,
+                                    sizeof(unsigned short) * n, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < n; i++) {
                 int v = -1;
                 if (k < cnt[i]) {
-                    int j = row[i];
+                    const int *t = &tab[(i / kBlock) * kTabIntsHost];
+                    int j = -1;
+                    if (t[1]) {  // staged tile: local index -> interval -> slot
+                        const int jl = row16[i];
+                        for (int q = 0; q < t[0]; q++) {
+                            const int st = t[3 + q], len = t[3 + 9 + q], off = t[3 + 18 + q];
+                            if (jl >= off && jl < off + len) j = st + (jl - off);
+                        }
+                    } else {
+                        j = row32[i];
+                    }
                     v = (j >= 0 && (size_t)j < n) ? ids[j] : -2;
                 }
                 out_ids[(size_t)ids[i] * kBucket + k] = v;

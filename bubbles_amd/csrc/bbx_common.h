@@ -39,6 +39,7 @@ int fail(int code, const std::string &msg);
 constexpr int kBucket = BBX_BUCKET_SIZE;   // reference bucket cap
 constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
 constexpr int kScanItems = 8;              // items per thread in the cell scan
+constexpr int kTabIntsHost = 32;           // ints per tile table (see bbx_tile.h)
 
 // Grid geometry in FP64 (the hash is computed in the reference's precision).
 struct GridGeom {
@@ -140,8 +141,12 @@ struct bbx_solver {
     // sort scratch (indexed by old slot / new position)
     unsigned long long *key = nullptr;
     int *newc = nullptr, *rank_tmp = nullptr, *tmp_idx = nullptr, *hash_tmp = nullptr;
-    // neighbour lists: nbr[k * stride + i], k < 100
-    int *nbr = nullptr, *nbr_cnt = nullptr;
+    // neighbour lists (bucket slot k of particle i at [k * stride + i], k < 100):
+    // 16-bit LDS-local indices for staged tiles, 32-bit storage indices otherwise
+    unsigned short *nbr16 = nullptr;
+    int *nbr32 = nullptr, *nbr_cnt = nullptr;
+    int *tile_tab = nullptr;   // per-tile interval tables (kTabInts ints each)
+    int ntiles = 0;
     // per-particle fields
     float *rho = nullptr, *pres = nullptr, *rho_star = nullptr, *q = nullptr;
     float4 *force = nullptr, *fp = nullptr, *xstar = nullptr;

@@ -18,7 +18,7 @@
 // (DistributeToCellOpt) concatenates the 27 neighbour cells' old chains in
 // (y,x,z) order, i.e. key = (direction of the old cell seen from the new cell,
 // old storage slot), since storage is the old chain order.
-#include "bbx_device.h"
+#include "bbx_tile.h"
 
 namespace bbx {
 
@@ -231,21 +231,50 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
 // ---------------------------------------------------------------------------
 // ComputeDensityFor (src/equations/sph_equations3.cpp:25-58) sums poly6 over
 // the bucket in bucket order; the sum runs as the bucket is built.
+// The tile's candidate intervals are staged in LDS; the bucket of a staged
+// tile is written as LDS-local indices (nbr16), otherwise as storage indices
+// (nbr32).  Candidates are visited in the reference's order: neighbour cells
+// y outer, x, z inner (src/core/grid.h:568-591), each cell in chain order.
 template <bool kPressure>
 __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             const float4 *__restrict__ pos,
                                                             const int *__restrict__ home,
                                                             const int *__restrict__ start,
                                                             const int *__restrict__ count,
-                                                            int *__restrict__ nbr,
+                                                            unsigned short *__restrict__ nbr16,
+                                                            int *__restrict__ nbr32,
                                                             int *__restrict__ nbr_cnt,
                                                             float *__restrict__ rho,
                                                             float *__restrict__ pres,
+                                                            int *__restrict__ tile_tab,
                                                             int *__restrict__ counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
+    __shared__ float4 sx[kTileCap];
+    __shared__ TileTab tt;
+    __shared__ int rdelta[9];
+    __shared__ int smin, smax;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < c.n;
+    const int hc = valid ? home[i] : -1;
+    if (threadIdx.x == 0) {
+        smin = 0x7fffffff;
+        smax = -1;
+    }
+    __syncthreads();
+    if (hc >= 0) {
+        atomicMin(&smin, hc);
+        atomicMax(&smax, hc);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        tab_build(c.g, smin, smax, start, count, tt, rdelta);
+        tab_store(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
+    }
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
+    __syncthreads();
+    if (!valid) return;
     const float4 pi = pos[i];
-    const int hc = home[i];
     int u[3] = {0, 0, 0};
     if (hc >= 0) unlinear(c.g, hc, u);
     int cnt = 0, dropped = 0;
@@ -258,13 +287,16 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             if (x < 0 || x >= nx) continue;
             for (int z = u[2] - 1; z <= u[2] + 1; z++) {
                 if (z < 0 || z >= nz) continue;
-                int nc = x + y * nx + z * nx * ny;
-                int b = start[nc], e = b + count[nc];
+                const int nc = x + y * nx + z * nx * ny;
+                const int b = start[nc], e = b + count[nc];
+                const int shift = staged ? rdelta[(z - u[2] + 1) * 3 + (y - u[1] + 1)] : 0;
                 for (int j = b; j < e; j++) {
-                    float4 pj = pos[j];
+                    const int jl = j + shift;
+                    const float4 pj = staged ? sx[jl] : pos[j];
                     if (!in_bucket(c, pi, pj)) continue;
                     if (cnt < kBucket) {
-                        nbr[(size_t)cnt * c.stride + i] = j;
+                        if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)jl;
+                        else nbr32[(size_t)cnt * c.stride + i] = j;
                         cnt++;
                         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                         kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
@@ -289,16 +321,26 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
 
 // density over the existing buckets (after an upload of positions)
 __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__restrict__ pos,
-                                                    const int *__restrict__ nbr,
+                                                    const unsigned short *__restrict__ nbr16,
+                                                    const int *__restrict__ nbr32,
                                                     const int *__restrict__ nbr_cnt,
+                                                    const int *__restrict__ tile_tab,
                                                     float *__restrict__ rho) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sx[kTileCap];
+    __shared__ TileTab tt;
+    tab_load_shared(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = pos[i];
     float sum = 0.f, comp = 0.f;
-    int cnt = nbr_cnt[i];
+    const int cnt = nbr_cnt[i];
     for (int k = 0; k < cnt; k++) {
-        float4 pj = pos[nbr[(size_t)k * c.stride + i]];
+        const size_t at = (size_t)k * c.stride + i;
+        const float4 pj = staged ? sx[nbr16[at]] : pos[nbr32[at]];
         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
     }
@@ -365,12 +407,12 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
         PROF_BEGIN("buckets_density");
         if (with_pressure)
             k_buckets_density<true><<<nblocks(n), kBlock, 0, s->stream>>>(
-                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr, s->nbr_cnt, s->rho,
-                s->pres, s->counters);
+                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
+                s->rho, s->pres, s->tile_tab, s->counters);
         else
             k_buckets_density<false><<<nblocks(n), kBlock, 0, s->stream>>>(
-                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr, s->nbr_cnt, s->rho,
-                s->pres, s->counters);
+                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
+                s->rho, s->pres, s->tile_tab, s->counters);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     }
@@ -380,7 +422,8 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
 int run_density_only(bbx_solver *s) {
     if (s->n == 0) return BBX_OK;
     PROF_BEGIN("density");
-    k_density<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->posA, s->nbr, s->nbr_cnt, s->rho);
+    k_density<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->posA, s->nbr16, s->nbr32, s->nbr_cnt,
+                                                        s->tile_tab, s->rho);
     BBX_HIP_CHECK(hipGetLastError());
     PROF_END();
     return BBX_OK;

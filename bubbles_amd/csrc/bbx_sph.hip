@@ -1,10 +1,13 @@
 // bbx_sph.hip -- SPH / PCISPH gathers, prediction and integration on gfx950.
 //
-// Every kernel is one thread per particle in cell-sorted storage order and
-// walks the particle's bucket (the reference neighbour list) from the
-// slot-major table nbr[k * stride + i], so a wavefront reads bucket slot k of
-// 64 consecutive particles with one coalesced load.  Fusions relative to the
-// reference's launch sequence (src/equations/pcisph_equations3.cpp:218-256):
+// Every kernel is one thread per particle in cell-sorted storage order.  A
+// workgroup's 256 particles form a tile: the tile's candidate intervals
+// (recorded by the bucket kernel) are first copied into LDS with coalesced
+// loads, then each thread walks its bucket -- LDS-local 16-bit indices,
+// slot-major table nbr16[k * stride + i] -- reading neighbours from LDS.
+// Tiles too large to stage walk 32-bit storage indices and read global
+// memory instead.  Fusions relative to the reference's launch sequence
+// (src/equations/pcisph_equations3.cpp:218-256):
 //
 //   nonpressure + predict(k=0)       ComputeNonPressureForceFor + PredictVelocityAndPositionFor
 //   predicted density + pressure     PredictPressureFor, FP64 max-error reduction
@@ -12,7 +15,7 @@
 //   pressure force + integrate       PredictPressureForceFor + AccumulateForcesFor + TimeIntegrationFor
 //
 // Arithmetic is FP32; density and pressure-force sums use Kahan compensation.
-#include "bbx_device.h"
+#include "bbx_tile.h"
 
 namespace bbx {
 
@@ -28,6 +31,14 @@ namespace bbx {
 
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// neighbour tables of the substep
+struct NbrTab {
+    const unsigned short *__restrict__ n16;
+    const int *__restrict__ n32;
+    const int *__restrict__ cnt;
+    const int *__restrict__ tab;
+};
+
 // PredictVelocityAndPositionFor (src/equations/pcisph_equations3.cpp:3-28):
 // v* = v + dt/m (f + f_p), x* = x + dt v*, collider with restitution 0.
 __device__ __forceinline__ float4 predict_position(const SimConst &c, const StepConst &st,
@@ -40,40 +51,81 @@ __device__ __forceinline__ float4 predict_position(const SimConst &c, const Step
     return make_float4(tp.x, tp.y, tp.z, 0.f);
 }
 
+// TimeIntegrationFor (src/equations/sph_equations3.cpp:283-339): symplectic
+// Euler, collider with restitution 0.6, clamp into the domain, 0.9-cell flag.
+__device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float4 xi, float4 vi,
+                                              float Fx, float Fy, float Fz, float3 &p, float3 &v,
+                                              int *__restrict__ flag) {
+    const float inv_m = 1.f / c.mass;
+    v = make_float3(vi.x + dt * (Fx * inv_m), vi.y + dt * (Fy * inv_m), vi.z + dt * (Fz * inv_m));
+    p = make_float3(xi.x + dt * v.x, xi.y + dt * v.y, xi.z + dt * v.z);
+    collide(c, 0.6f, p, v);
+    if (!inside_box(p.x, p.y, p.z, c.bmin, c.bmax)) {
+        p.x = fminf(fmaxf(p.x, c.bmin[0] + c.radius), c.bmax[0] - c.radius);
+        p.y = fminf(fmaxf(p.y, c.bmin[1] + c.radius), c.bmax[1] - c.radius);
+        p.z = fminf(fmaxf(p.z, c.bmin[2] + c.radius), c.bmax[2] - c.radius);
+    }
+    float ddx = p.x - xi.x, ddy = p.y - xi.y, ddz = p.z - xi.z;
+    if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) >= c.min_len * 0.9f) *flag = 1;
+}
+
 // ---------------------------------------------------------------------------
 // non-pressure force (src/equations/sph_equations3.cpp:80-110)
 // mode: 0 = store force only, 1 = + predict(k=0) with f_p = 0,
 //       2 = + warm-start features
+// LDS: neighbour position, (velocity, density)
 // ---------------------------------------------------------------------------
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st,
+__global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
                                                         const float4 *__restrict__ pos,
                                                         const float4 *__restrict__ vel,
                                                         const float *__restrict__ rho,
-                                                        const int *__restrict__ nbr,
-                                                        const int *__restrict__ nbr_cnt,
                                                         float4 *__restrict__ force,
                                                         float4 *__restrict__ xstar,
                                                         float *__restrict__ rho_star,
                                                         float *__restrict__ feat,
                                                         unsigned long long *__restrict__ max_err) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sx[kTileCap];
+    __shared__ float4 sv[kTileCap];
+    __shared__ TileTab tt;
+    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    if (staged) {
+        tab_stage(tt, sx, [&](int j) { return pos[j]; });
+        tab_stage(tt, sv, [&](int j) {
+            float4 v = vel[j];
+            return make_float4(v.x, v.y, v.z, rho[j]);
+        });
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0 && max_err) *max_err = 0ull;
     if (i >= c.n) return;
+    const int self = staged ? tab_local(tt, i) : i;
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     const float coef = c.visc * c.mass2;
     float fx = c.mass * c.gx - c.drag * vi.x;
     float fy = c.mass * c.gy - c.drag * vi.y;
     float fz = c.mass * c.gz - c.drag * vi.z;
-    const int cnt = nbr_cnt[i];
-    // feature accumulators
+    const int cnt = nb.cnt[i];
     float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
     for (int k = 0; k < cnt; k++) {
-        int j = nbr[(size_t)k * c.stride + i];
-        float4 xj = pos[j];
-        float4 vj = vel[j];
-        float rj = rho[j];
+        const size_t at = (size_t)k * c.stride + i;
+        int j;
+        float4 xj, vj;
+        if (staged) {
+            j = nb.n16[at];
+            xj = sx[j];
+            vj = sv[j];
+        } else {
+            j = nb.n32[at];
+            xj = pos[j];
+            float4 v = vel[j];
+            vj = make_float4(v.x, v.y, v.z, rho[j]);
+        }
+        const float rj = vj.w;
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);
         float w = 0.f, x = 0.f;
@@ -85,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
         fx += (vj.x - vi.x) * s;
         fy += (vj.y - vi.y) * s;
         fz += (vj.z - vi.z) * s;
-        if (kMode == 2 && j != i) {
+        if (kMode == 2 && j != self) {
             float u = r * c.inv_h;
             m1 += u;
             m2 += u * u;
@@ -94,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
             cy -= dy * c.inv_h;
             cz -= dz * c.inv_h;
             if (r > 0.f && x > 0.f) {
-                // grad_i W = dW/dr (x_i - x_j)/r, dW/dr = -spiky_grad x^2
+                // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
                 float g = -c.spiky_grad * x * x / r;
                 float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
                 divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
@@ -142,25 +194,33 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
 
 // ---------------------------------------------------------------------------
 // predicted density and pressure update (PredictPressureFor,
-// src/equations/pcisph_equations3.cpp:60-90) + FP64 max-error reduction
+// src/equations/pcisph_equations3.cpp:60-90) + FP64 max-error reduction.
+// LDS: predicted positions.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepConst st, double delta,
+                                                             NbrTab nb,
                                                              const float4 *__restrict__ xstar,
-                                                             const int *__restrict__ nbr,
-                                                             const int *__restrict__ nbr_cnt,
                                                              float *__restrict__ pres,
                                                              float *__restrict__ rho_star,
                                                              float *__restrict__ q,
                                                              unsigned long long *__restrict__ max_err) {
+    __shared__ float4 sx[kTileCap];
+    __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kBlock / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    if (staged) tab_stage(tt, sx, [&](int j) { return xstar[j]; });
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long ebits = 0ull;
     if (i < c.n) {
         const float4 xi = xstar[i];
         float sum = 0.f, comp = 0.f;
-        const int cnt = nbr_cnt[i];
+        const int cnt = nb.cnt[i];
         for (int k = 0; k < cnt; k++) {
-            float4 xj = xstar[nbr[(size_t)k * c.stride + i]];
+            const size_t at = (size_t)k * c.stride + i;
+            const float4 xj = staged ? sx[nb.n16[at]] : xstar[nb.n32[at]];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
         }
@@ -192,19 +252,18 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
 
 // ---------------------------------------------------------------------------
 // pressure force (PredictPressureForceFor, src/equations/pcisph_equations3.cpp:114-155)
-// at the current positions with predicted densities; then either predict the
-// next iteration's x* (kIntegrate = 0) or accumulate and integrate
-// (AccumulateForcesFor + TimeIntegrationFor, src/equations/sph_equations3.cpp:283-339).
+// at the current positions with predicted densities (q = p / rho*^2); then
+// either predict the next iteration's x* (kIntegrate = 0) or accumulate and
+// integrate (AccumulateForcesFor + TimeIntegrationFor).
+// LDS: (position, q).
 // ---------------------------------------------------------------------------
 template <int kIntegrate>
-__global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst st,
+__global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst st, NbrTab nb,
                                                            const float4 *__restrict__ pos,
                                                            const float4 *__restrict__ vel,
                                                            const float4 *__restrict__ force,
                                                            const float *__restrict__ q,
                                                            const float *__restrict__ pres,
-                                                           const int *__restrict__ nbr,
-                                                           const int *__restrict__ nbr_cnt,
                                                            float4 *__restrict__ xstar,
                                                            float4 *__restrict__ fp_out,
                                                            float4 *__restrict__ pos_out,
@@ -213,25 +272,45 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            int *__restrict__ flag,
                                                            unsigned *__restrict__ max_force,
                                                            unsigned long long *__restrict__ max_err) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sx[kTileCap];
+    __shared__ TileTab tt;
+    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    if (staged)
+        tab_stage(tt, sx, [&](int j) {
+            float4 p = pos[j];
+            return make_float4(p.x, p.y, p.z, q[j]);
+        });
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     if (!kIntegrate && i == 0) *max_err = 0ull;
     unsigned fbits = 0u;
     if (i < c.n) {
+        const int self = staged ? tab_local(tt, i) : i;
         const float4 xi = pos[i];
         const float qi = q[i];
         float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-        const int cnt = nbr_cnt[i];
+        const int cnt = nb.cnt[i];
         for (int k = 0; k < cnt; k++) {
-            int j = nbr[(size_t)k * c.stride + i];
-            if (j == i) continue;
-            float4 xj = pos[j];
-            float qj = q[j];
+            const size_t at = (size_t)k * c.stride + i;
+            int j;
+            float4 xj;
+            if (staged) {
+                j = nb.n16[at];
+                xj = sx[j];
+            } else {
+                j = nb.n32[at];
+                float4 p = pos[j];
+                xj = make_float4(p.x, p.y, p.z, q[j]);
+            }
+            if (j == self) continue;
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
             float r = sqrtf(dx * dx + dy * dy + dz * dz);
             if (!spiky_in(c, r) || !(r > 0.f) || fabsf(r) < 1e-8f) continue;
             float x = 1.f - r * c.inv_h;
             // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r
-            float g = c.mass2 * (qi + qj) * c.spiky_grad * x * x / r;
+            float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
             kahan_add(tx, ex, g * dx);
             kahan_add(ty, ey, g * dy);
             kahan_add(tz, ez, g * dz);
@@ -242,19 +321,9 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         if (!kIntegrate) {
             xstar[i] = predict_position(c, st, xi, vi, fi.x + fpx, fi.y + fpy, fi.z + fpz);
         } else {
-            float Fx = fi.x + fpx, Fy = fi.y + fpy, Fz = fi.z + fpz;
-            float inv_m = 1.f / c.mass;
-            float3 v = make_float3(vi.x + st.dt * (Fx * inv_m), vi.y + st.dt * (Fy * inv_m),
-                                   vi.z + st.dt * (Fz * inv_m));
-            float3 p = make_float3(xi.x + st.dt * v.x, xi.y + st.dt * v.y, xi.z + st.dt * v.z);
-            collide(c, 0.6f, p, v);
-            if (!inside_box(p.x, p.y, p.z, c.bmin, c.bmax)) {
-                p.x = fminf(fmaxf(p.x, c.bmin[0] + c.radius), c.bmax[0] - c.radius);
-                p.y = fminf(fmaxf(p.y, c.bmin[1] + c.radius), c.bmax[1] - c.radius);
-                p.z = fminf(fmaxf(p.z, c.bmin[2] + c.radius), c.bmax[2] - c.radius);
-            }
-            float ddx = p.x - xi.x, ddy = p.y - xi.y, ddz = p.z - xi.z;
-            if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) >= c.min_len * 0.9f) *flag = 1;
+            const float Fx = fi.x + fpx, Fy = fi.y + fpy, Fz = fi.z + fpz;
+            float3 p, v;
+            integrate_one(c, st.dt, xi, vi, Fx, Fy, Fz, p, v, flag);
             pos_out[i] = make_float4(p.x, p.y, p.z, xi.w);
             vel_out[i] = make_float4(v.x, v.y, v.z, pres[i]);
             force_out[i] = make_float4(Fx, Fy, Fz, 0.f);
@@ -271,16 +340,37 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
 // ---------------------------------------------------------------------------
 // SPH (WCSPH) forces: ComputeAllForcesFor without the in-loop integration
 // (src/equations/sph_equations3.cpp:184-278); integration is a separate pass.
+// LDS: (position, p / rho^2), (velocity, density).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, const float4 *__restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
+                                                      const float4 *__restrict__ pos,
                                                       const float4 *__restrict__ vel,
                                                       const float *__restrict__ rho,
                                                       const float *__restrict__ pres,
-                                                      const int *__restrict__ nbr,
-                                                      const int *__restrict__ nbr_cnt,
                                                       float4 *__restrict__ force) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sx[kTileCap];
+    __shared__ float4 sv[kTileCap];
+    __shared__ TileTab tt;
+    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    auto xq = [&](int j) {
+        float4 p = pos[j];
+        float d = rho[j];
+        return make_float4(p.x, p.y, p.z, pres[j] / (d * d));
+    };
+    auto vr = [&](int j) {
+        float4 v = vel[j];
+        return make_float4(v.x, v.y, v.z, rho[j]);
+    };
+    if (staged) {
+        tab_stage(tt, sx, xq);
+        tab_stage(tt, sv, vr);
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= c.n) return;
+    const int self = staged ? tab_local(tt, i) : i;
     const float4 xi = pos[i], vi = vel[i];
     const float di = rho[i];
     const float qi = pres[i] / (di * di);
@@ -289,23 +379,32 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, const float4 *
     float fy = c.mass * c.gy - c.drag * vi.y;
     float fz = c.mass * c.gz - c.drag * vi.z;
     float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-    const int cnt = nbr_cnt[i];
+    const int cnt = nb.cnt[i];
     for (int k = 0; k < cnt; k++) {
-        int j = nbr[(size_t)k * c.stride + i];
-        if (j == i) continue;
-        float4 xj = pos[j], vj = vel[j];
-        float dj = rho[j];
+        const size_t at = (size_t)k * c.stride + i;
+        int j;
+        float4 xj, vj;
+        if (staged) {
+            j = nb.n16[at];
+            xj = sx[j];
+            vj = sv[j];
+        } else {
+            j = nb.n32[at];
+            xj = xq(j);
+            vj = vr(j);
+        }
+        if (j == self) continue;
         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);
         if (!spiky_in(c, r)) continue;
         float x = 1.f - r * c.inv_h;
         if (r > 0.f && !(fabsf(r) < 1e-8f)) {
-            float g = c.mass2 * (qi + pres[j] / (dj * dj)) * c.spiky_grad * x * x / r;
+            float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
             kahan_add(tx, ex, g * dx);
             kahan_add(ty, ey, g * dy);
             kahan_add(tz, ez, g * dz);
         }
-        float s = coef * c.spiky_lap * x / dj;
+        float s = coef * c.spiky_lap * x / vj.w;
         fx += (vj.x - vi.x) * s;
         fy += (vj.y - vi.y) * s;
         fz += (vj.z - vi.z) * s;
@@ -313,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, const float4 *
     force[i] = make_float4(fx - tx, fy - ty, fz - tz, 0.f);
 }
 
-// TimeIntegrationFor (src/equations/sph_equations3.cpp:283-339)
+// TimeIntegrationFor as a separate pass (SPH two-phase form)
 __global__ void __launch_bounds__(kBlock) k_integrate(SimConst c, StepConst st,
                                                       const float4 *__restrict__ pos,
                                                       const float4 *__restrict__ vel,
@@ -326,18 +425,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate(SimConst c, StepConst st,
     unsigned fbits = 0u;
     if (i < c.n) {
         const float4 xi = pos[i], vi = vel[i], fi = force[i];
-        float inv_m = 1.f / c.mass;
-        float3 v = make_float3(vi.x + st.dt * (fi.x * inv_m), vi.y + st.dt * (fi.y * inv_m),
-                               vi.z + st.dt * (fi.z * inv_m));
-        float3 p = make_float3(xi.x + st.dt * v.x, xi.y + st.dt * v.y, xi.z + st.dt * v.z);
-        collide(c, 0.6f, p, v);
-        if (!inside_box(p.x, p.y, p.z, c.bmin, c.bmax)) {
-            p.x = fminf(fmaxf(p.x, c.bmin[0] + c.radius), c.bmax[0] - c.radius);
-            p.y = fminf(fmaxf(p.y, c.bmin[1] + c.radius), c.bmax[1] - c.radius);
-            p.z = fminf(fmaxf(p.z, c.bmin[2] + c.radius), c.bmax[2] - c.radius);
-        }
-        float ddx = p.x - xi.x, ddy = p.y - xi.y, ddz = p.z - xi.z;
-        if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) >= c.min_len * 0.9f) *flag = 1;
+        float3 p, v;
+        integrate_one(c, st.dt, xi, vi, fi.x, fi.y, fi.z, p, v, flag);
         pos_out[i] = make_float4(p.x, p.y, p.z, xi.w);
         vel_out[i] = make_float4(v.x, v.y, v.z, vi.w);
         fbits = __float_as_uint(sqrtf(fi.x * fi.x + fi.y * fi.y + fi.z * fi.z));
@@ -347,23 +436,45 @@ __global__ void __launch_bounds__(kBlock) k_integrate(SimConst c, StepConst st,
 }
 
 // XSPH velocity smoothing (src/equations/sph_equations3.cpp:341-382), active
-// only when pseudoViscosity * dt > 0.1.
-__global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st,
+// only when pseudoViscosity * dt > 0.1.  LDS: position, (velocity, density).
+__global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTab nb,
                                                  const float4 *__restrict__ pos,
                                                  const float4 *__restrict__ vel,
                                                  const float *__restrict__ rho,
-                                                 const int *__restrict__ nbr,
-                                                 const int *__restrict__ nbr_cnt,
                                                  float4 *__restrict__ vel_out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sx[kTileCap];
+    __shared__ float4 sv[kTileCap];
+    __shared__ TileTab tt;
+    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
+    auto vr = [&](int j) {
+        float4 v = vel[j];
+        return make_float4(v.x, v.y, v.z, rho[j]);
+    };
+    if (staged) {
+        tab_stage(tt, sx, [&](int j) { return pos[j]; });
+        tab_stage(tt, sv, vr);
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const float4 xi = pos[i], vi = vel[i];
-    float sx = 0.f, sy = 0.f, sz = 0.f, ws = 0.f;
+    float sx_ = 0.f, sy_ = 0.f, sz_ = 0.f, ws = 0.f;
     const float spiky_w = c.spiky_lap * c.h * c.h / 6.f;  // 15 / (pi h^3)
-    const int cnt = nbr_cnt[i];
+    const int cnt = nb.cnt[i];
     for (int k = 0; k < cnt; k++) {
-        int j = nbr[(size_t)k * c.stride + i];
-        float4 xj = pos[j], vj = vel[j];
+        const size_t at = (size_t)k * c.stride + i;
+        float4 xj, vj;
+        if (staged) {
+            const int j = nb.n16[at];
+            xj = sx[j];
+            vj = sv[j];
+        } else {
+            const int j = nb.n32[at];
+            xj = pos[j];
+            vj = vr(j);
+        }
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);
         float w = 0.f;
@@ -371,19 +482,21 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st,
             float x = 1.f - r * c.inv_h;
             w = spiky_w * x * x * x;
         }
-        float wj = c.mass / rho[j] * w;
+        float wj = c.mass / vj.w * w;
         ws += wj;
-        sx += wj * vj.x;
-        sy += wj * vj.y;
-        sz += wj * vj.z;
+        sx_ += wj * vj.x;
+        sy_ += wj * vj.y;
+        sz_ += wj * vj.z;
     }
     if (ws > 0.f) {
         float inv = 1.f / ws;
-        sx *= inv; sy *= inv; sz *= inv;
+        sx_ *= inv;
+        sy_ *= inv;
+        sz_ *= inv;
     }
     float f = fminf(fmaxf(st.dt * c.pseudo_visc, 0.f), 1.f);
-    vel_out[i] = make_float4((1.f - f) * vi.x + f * sx, (1.f - f) * vi.y + f * sy,
-                             (1.f - f) * vi.z + f * sz, vi.w);
+    vel_out[i] = make_float4((1.f - f) * vi.x + f * sx_, (1.f - f) * vi.y + f * sy_,
+                             (1.f - f) * vi.z + f * sz_, vi.w);
 }
 
 // warm start: p = MLP output, densityPredicted = density (k = 0 init)
@@ -403,28 +516,30 @@ __global__ void k_warm_init(SimConst c, const float *__restrict__ pwarm,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static NbrTab nbr_tab(const bbx_solver *s) {
+    return NbrTab{s->nbr16, s->nbr32, s->nbr_cnt, s->tile_tab};
+}
+
 int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features) {
     if (s->n == 0) return BBX_OK;
     const unsigned nb = nblocks(s->n);
+    const NbrTab t = nbr_tab(s);
     if (features) {
         PROF_BEGIN("nonpressure_features");
-        k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->rho, s->nbr,
-                                                       s->nbr_cnt, s->force, s->xstar, s->rho_star,
-                                                       s->feat, s->max_err);
+        k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+                                                       s->xstar, s->rho_star, s->feat, s->max_err);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     } else if (predict) {
         PROF_BEGIN("nonpressure_predict");
-        k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->rho, s->nbr,
-                                                       s->nbr_cnt, s->force, s->xstar, s->rho_star,
-                                                       s->feat, s->max_err);
+        k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+                                                       s->xstar, s->rho_star, s->feat, s->max_err);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     } else {
         PROF_BEGIN("nonpressure");
-        k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->rho, s->nbr,
-                                                       s->nbr_cnt, s->force, s->xstar, s->rho_star,
-                                                       s->feat, nullptr);
+        k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+                                                       s->xstar, s->rho_star, s->feat, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     }
@@ -435,9 +550,8 @@ int run_predict_pressure(bbx_solver *s, const StepConst &st) {
     if (s->n == 0) return BBX_OK;
     // the pressure increment uses the host's FP64 delta (s->delta64)
     PROF_BEGIN("predict_pressure");
-    k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, s->xstar, s->nbr,
-                                                                s->nbr_cnt, s->pres, s->rho_star,
-                                                                s->q, s->max_err);
+    k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
+                                                                s->pres, s->rho_star, s->q, s->max_err);
     BBX_HIP_CHECK(hipGetLastError());
     PROF_END();
     return BBX_OK;
@@ -446,20 +560,19 @@ int run_predict_pressure(bbx_solver *s, const StepConst &st) {
 int run_pressure_force(bbx_solver *s, const StepConst &st, int integrate) {
     if (s->n == 0) return BBX_OK;
     const unsigned nb = nblocks(s->n);
+    const NbrTab t = nbr_tab(s);
     if (integrate) {
         PROF_BEGIN("pressure_force_integrate");
-        k_pressure_force<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->force, s->q,
-                                                          s->pres, s->nbr, s->nbr_cnt, s->xstar, s->fp,
-                                                          s->posB, s->velB, s->force, s->flag,
-                                                          s->max_force, s->max_err);
+        k_pressure_force<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->force, s->q,
+                                                          s->pres, s->xstar, s->fp, s->posB, s->velB,
+                                                          s->force, s->flag, s->max_force, s->max_err);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     } else {
         PROF_BEGIN("pressure_force_predict");
-        k_pressure_force<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->force, s->q,
-                                                          s->pres, s->nbr, s->nbr_cnt, s->xstar, s->fp,
-                                                          s->posB, s->velB, s->force, s->flag,
-                                                          s->max_force, s->max_err);
+        k_pressure_force<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->force, s->q,
+                                                          s->pres, s->xstar, s->fp, s->posB, s->velB,
+                                                          s->force, s->flag, s->max_force, s->max_err);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     }
@@ -469,8 +582,8 @@ int run_pressure_force(bbx_solver *s, const StepConst &st, int integrate) {
 int run_sph_force(bbx_solver *s) {
     if (s->n == 0) return BBX_OK;
     PROF_BEGIN("sph_force");
-    k_sph_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->posA, s->velA, s->rho, s->pres,
-                                                         s->nbr, s->nbr_cnt, s->force);
+    k_sph_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, nbr_tab(s), s->posA, s->velA, s->rho,
+                                                         s->pres, s->force);
     BBX_HIP_CHECK(hipGetLastError());
     PROF_END();
     return BBX_OK;
@@ -496,8 +609,8 @@ int run_xsph(bbx_solver *s, const StepConst &st) {
     if (s->n == 0) return BBX_OK;
     if (!(s->prm.pseudo_viscosity * (double)st.dt > 0.1)) return BBX_OK;
     PROF_BEGIN("xsph");
-    k_xsph<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->rho, s->nbr,
-                                                    s->nbr_cnt, s->velB);
+    k_xsph<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, nbr_tab(s), s->posA, s->velA, s->rho,
+                                                    s->velB);
     BBX_HIP_CHECK(hipGetLastError());
     PROF_END();
     std::swap(s->velA, s->velB);

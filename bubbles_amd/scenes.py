@@ -111,12 +111,12 @@ def _scene(name, blocks, spacing, h, tank_size, seed, viscosity, dt, jitter=1e-4
 
 
 def dam_break(nx=128, ny=64, nz=128, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2,
-              dt=5e-4, viscosity=0.04) -> Scene:
+              dt=5e-4, viscosity=0.04, jitter=1e-4) -> Scene:
     """PCISPH dam break: an ``nx*ny*nz`` block in the low corner of the tank,
     kernel radius 2*spacing, grid cell = kernel radius (1M particles at the
     defaults: 128x64x128 at 0.01 in a 2x1x2 tank, dt 5e-4)."""
     return _scene(f"dam_break_{nx}x{ny}x{nz}", [((nx, ny, nz), (0.0, 0.0, 0.0))],
-                  spacing, 2.0 * spacing, tank, seed, viscosity, dt)
+                  spacing, 2.0 * spacing, tank, seed, viscosity, dt, jitter)
 
 
 def sph_dam_break(seed=1) -> Scene:

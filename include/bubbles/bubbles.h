@@ -1,0 +1,533 @@
+/*
+ * bubbles/bubbles.h -- C++ facade of the MI355X SPH / PCISPH hot path with
+ * the reference's own class and function names, so scene code written
+ * against Bubbles' solver API compiles against this library.
+ *
+ * Mirrors (reference file:line):
+ *   vec3f / vec3ui / Float / Bounds3f ......... src/core/geometry.h:69,497-650,1463-1615
+ *   Bucket ..................................... src/core/particle.h:22-60
+ *   ParticleSet3, SphParticleSet3 ............. src/core/particle.h:154-401,508-609
+ *   ParticleSetBuilder3, SphParticleSet3FromBuilder  src/core/particle.h:611-710, src/core/particle.cpp:21-27
+ *   Grid3, UtilBuildGridForDomain .............. src/core/grid.h:197-600, src/core/util.cpp:269-287
+ *   Transform (translation), MakeBox, Shape .... src/shapes/box.cpp:6-10
+ *   ColliderSetBuilder3, ColliderSet3 .......... src/core/collider.h
+ *   SphSolverData3, DefaultSphSolverData3 ...... src/core/sph_solver.h:31-50, src/solvers/sph_solver3.cpp:124-151
+ *   PciSphSolver3 .............................. src/core/pcisph_solver.h:56-83
+ *   SphSolver3 ................................. src/core/sph_solver.h:76-96
+ *
+ * Differences from the reference, by design:
+ *   - data lives in device memory owned by the solver; per-particle
+ *     accessors download lazily (cached until the next step) and setters
+ *     upload immediately;
+ *   - errors throw bubbles_error instead of printing and exiting;
+ *   - containers are reversed axis-aligned boxes placed by a translation;
+ *   - PciSphSolver3 exposes the predict-correct loop settings (the reference
+ *     hard-codes maxIt 5, tol 0.01 and, in effect, one iteration).
+ * The device path is FP32 (+ Kahan); Float stays double at this API.
+ */
+#ifndef BUBBLES_FACADE_H
+#define BUBBLES_FACADE_H
+
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../bubbles_amd.h"
+
+typedef double Float;
+
+struct bubbles_error : std::runtime_error {
+    explicit bubbles_error(const std::string &m) : std::runtime_error(m) {}
+};
+
+inline void bbx_throw_if(int rc, const char *what) {
+    if (rc != BBX_OK) throw bubbles_error(std::string(what) + ": " + bbx_last_error());
+}
+
+struct vec3f {
+    Float x, y, z;
+    vec3f() : x(0), y(0), z(0) {}
+    explicit vec3f(Float a) : x(a), y(a), z(a) {}
+    vec3f(Float a, Float b, Float c) : x(a), y(b), z(c) {}
+    Float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    Float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    vec3f operator+(const vec3f &v) const { return vec3f(x + v.x, y + v.y, z + v.z); }
+    vec3f operator-(const vec3f &v) const { return vec3f(x - v.x, y - v.y, z - v.z); }
+    vec3f operator*(Float s) const { return vec3f(x * s, y * s, z * s); }
+    Float LengthSquared() const { return x * x + y * y + z * z; }
+    Float Length() const { return std::sqrt(LengthSquared()); }
+};
+inline vec3f operator*(Float s, const vec3f &v) { return v * s; }
+
+struct vec3ui {
+    unsigned int x, y, z;
+    vec3ui() : x(0), y(0), z(0) {}
+    vec3ui(unsigned a, unsigned b, unsigned c) : x(a), y(b), z(c) {}
+    unsigned operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+struct Bounds3f {
+    vec3f pMin, pMax;
+    Bounds3f() {}
+    Bounds3f(const vec3f &a, const vec3f &b)
+        : pMin(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)),
+          pMax(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)) {}
+    Float ExtentOn(int i) const { return std::fabs(pMax[i] - pMin[i]); }
+    vec3f Center() const { return (pMin + pMax) * 0.5; }
+};
+
+/* Translation-only transform (the reference's container transform in every
+ * PCISPH scene is Transform() or Translate(...)). */
+struct Transform {
+    vec3f t;
+    Transform() {}
+    explicit Transform(const vec3f &tr) : t(tr) {}
+};
+inline Transform Translate(Float x, Float y, Float z) { return Transform(vec3f(x, y, z)); }
+inline Transform Translate(const vec3f &v) { return Transform(v); }
+
+struct Shape {
+    vec3f center, size;
+    bool reverseOrientation = false;
+    /* Shape::BoxGetBounds (src/shapes/box.cpp:59-64) */
+    Bounds3f GetBounds() const {
+        vec3f h(size.x / 2.0, size.y / 2.0, size.z / 2.0);
+        return Bounds3f(vec3f(-h.x, -h.y, -h.z) + center, h + center);
+    }
+};
+
+/* MakeBox (src/shapes/box.cpp:6-10) */
+inline Shape *MakeBox(const Transform &toWorld, const vec3f &size, bool reverseOrientation = false) {
+    Shape *s = new Shape();
+    s->center = toWorld.t;
+    s->size = size;
+    s->reverseOrientation = reverseOrientation;
+    return s;
+}
+
+struct ColliderSet3 {
+    std::vector<Shape *> shapes;
+};
+
+struct ColliderSetBuilder3 {
+    std::vector<Shape *> shapes;
+    void AddCollider3(Shape *shape) { shapes.push_back(shape); }
+    ColliderSet3 *GetColliderSet() {
+        ColliderSet3 *c = new ColliderSet3();
+        c->shapes = shapes;
+        return c;
+    }
+};
+
+/* Grid3 geometry (Build + hash + 27-cell lists) */
+class Grid3 {
+  public:
+    bbx_params prm{};
+    bbx_grid_desc desc{};
+    unsigned int GetCellCount() const { return desc.total; }
+    vec3ui GetIndexCount() const { return vec3ui(desc.usizes[0], desc.usizes[1], desc.usizes[2]); }
+    vec3f GetCellSize() const { return vec3f(desc.cells_len[0], desc.cells_len[1], desc.cells_len[2]); }
+    Bounds3f GetBounds() const {
+        return Bounds3f(vec3f(desc.bounds_min[0], desc.bounds_min[1], desc.bounds_min[2]),
+                        vec3f(desc.bounds_max[0], desc.bounds_max[1], desc.bounds_max[2]));
+    }
+    int GetDimensions() const { return 3; }
+    vec3ui GetHashedPosition(const vec3f &p) const {
+        double q[3] = {p.x, p.y, p.z};
+        int id, u[3];
+        bbx_throw_if(bbx_grid_hash_desc(&desc, q, 1, &id, u), "GetHashedPosition");
+        if (id < 0) throw bubbles_error("GetHashedPosition: point outside the grid");
+        return vec3ui((unsigned)u[0], (unsigned)u[1], (unsigned)u[2]);
+    }
+    unsigned int GetLinearHashedPosition(const vec3f &p) const {
+        double q[3] = {p.x, p.y, p.z};
+        int id;
+        bbx_throw_if(bbx_grid_hash_desc(&desc, q, 1, &id, nullptr), "GetLinearHashedPosition");
+        if (id < 0) throw bubbles_error("GetLinearHashedPosition: point outside the grid");
+        return (unsigned)id;
+    }
+    vec3ui GetCellIndex(unsigned int i) const {
+        unsigned plane = desc.usizes[0] * desc.usizes[1];
+        unsigned z = i / plane, rem = i - z * plane;
+        return vec3ui(rem % desc.usizes[0], rem / desc.usizes[0], z);
+    }
+    /* GetNeighborsOf (src/core/grid.h:502-510): fills an internal list */
+    int GetNeighborsOf(int id, int **neighbors) {
+        vec3ui u = GetCellIndex((unsigned)id);
+        int c = 0;
+        const int nx = (int)desc.usizes[0], ny = (int)desc.usizes[1], nz = (int)desc.usizes[2];
+        for (int j = (int)u.y - 1; j <= (int)u.y + 1; j++) {
+            if (j < 0 || j >= ny) continue;
+            for (int i = (int)u.x - 1; i <= (int)u.x + 1; i++) {
+                if (i < 0 || i >= nx) continue;
+                for (int k = (int)u.z - 1; k <= (int)u.z + 1; k++) {
+                    if (k < 0 || k >= nz) continue;
+                    list_[c++] = i + j * nx + k * nx * ny;
+                }
+            }
+        }
+        if (neighbors) *neighbors = list_;
+        return c;
+    }
+
+  private:
+    int list_[27];
+};
+
+/* UtilBuildGridForDomain (src/core/util.cpp:269-287) */
+inline Grid3 *UtilBuildGridForDomain(Bounds3f domain, Float spacing, Float spacingScale = 2.0) {
+    Grid3 *g = new Grid3();
+    bbx_throw_if(bbx_default_params(&g->prm), "UtilBuildGridForDomain");
+    vec3f c = domain.Center();
+    for (int k = 0; k < 3; k++) {
+        g->prm.tank_center[k] = c[k];
+        g->prm.tank_size[k] = domain.ExtentOn(k);
+    }
+    g->prm.grid_spacing = spacing;
+    g->prm.grid_scale = spacingScale;
+    bbx_throw_if(bbx_grid_describe(&g->prm, &g->desc), "UtilBuildGridForDomain");
+    return g;
+}
+
+struct Bucket {
+    std::vector<int> ids;
+    int Count() const { return (int)ids.size(); }
+    int Get(int where) const { return where < (int)ids.size() ? ids[where] : -1; }
+};
+
+class PciSphSolver3;
+class SphSolver3;
+
+/* ParticleSet3: host copy until a solver owns the particles, then lazy
+ * downloads from the device. */
+class ParticleSet3 {
+  public:
+    std::vector<double> pos, vel;
+    int count = 0;
+    Float mass = 1e-3, radius = 1e-3;
+    bbx_solver *dev = nullptr;
+
+    int GetParticleCount() const { return count; }
+    Float GetMass() const { return mass; }
+    Float GetRadius() const { return radius; }
+    vec3f GetParticlePosition(int i) { return vfield(BBX_FIELD_POSITION, i); }
+    vec3f GetParticleVelocity(int i) { return vfield(BBX_FIELD_VELOCITY, i); }
+    vec3f GetParticleForce(int i) { return vfield(BBX_FIELD_FORCE, i); }
+    Float GetParticleDensity(int i) { return sfield(BBX_FIELD_DENSITY, i); }
+    Float GetParticlePressure(int i) { return sfield(BBX_FIELD_PRESSURE, i); }
+    void SetParticlePosition(int i, const vec3f &p) { setv(BBX_FIELD_POSITION, i, p); }
+    void SetParticleVelocity(int i, const vec3f &v) { setv(BBX_FIELD_VELOCITY, i, v); }
+    Bucket *GetParticleBucket(int i) {
+        if (!dev) throw bubbles_error("GetParticleBucket: particles are not in a solver yet");
+        if (buckets_.empty()) {
+            std::vector<int> counts(count), ids((size_t)count * BBX_BUCKET_SIZE);
+            bbx_throw_if(bbx_neighbors_get(dev, counts.data(), ids.data()), "GetParticleBucket");
+            buckets_.resize(count);
+            for (int p = 0; p < count; p++)
+                buckets_[p].ids.assign(ids.begin() + (size_t)p * BBX_BUCKET_SIZE,
+                                       ids.begin() + (size_t)p * BBX_BUCKET_SIZE + counts[p]);
+        }
+        return &buckets_[i];
+    }
+    /* drop cached downloads (called after every device step) */
+    void Invalidate() {
+        for (auto &c : cache_) c.clear();
+        buckets_.clear();
+    }
+
+  private:
+    std::vector<double> cache_[10];
+    std::vector<Bucket> buckets_;
+    const std::vector<double> &fetch(int field, int width) {
+        auto &c = cache_[field];
+        if (c.empty()) {
+            c.resize((size_t)count * width);
+            bbx_throw_if(bbx_solver_download(dev, field, c.data()), "download");
+        }
+        return c;
+    }
+    vec3f vfield(int field, int i) {
+        if (i < 0 || i >= count) throw bubbles_error("particle id out of range");
+        if (!dev) {
+            if (field == BBX_FIELD_POSITION) return vec3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+            if (field == BBX_FIELD_VELOCITY) return vec3f(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+            return vec3f(0);
+        }
+        const auto &c = fetch(field, 3);
+        return vec3f(c[3 * (size_t)i], c[3 * (size_t)i + 1], c[3 * (size_t)i + 2]);
+    }
+    Float sfield(int field, int i) {
+        if (i < 0 || i >= count) throw bubbles_error("particle id out of range");
+        if (!dev) return 0;
+        return fetch(field, 1)[i];
+    }
+    void setv(int field, int i, const vec3f &v) {
+        if (i < 0 || i >= count) throw bubbles_error("particle id out of range");
+        std::vector<double> &host = field == BBX_FIELD_POSITION ? pos : vel;
+        if (dev) {
+            host = fetch(field, 3);
+            host[3 * (size_t)i] = v.x;
+            host[3 * (size_t)i + 1] = v.y;
+            host[3 * (size_t)i + 2] = v.z;
+            bbx_throw_if(bbx_solver_upload(dev, field, host.data()), "upload");
+            Invalidate();
+        } else {
+            host[3 * (size_t)i] = v.x;
+            host[3 * (size_t)i + 1] = v.y;
+            host[3 * (size_t)i + 2] = v.z;
+        }
+    }
+};
+
+class SphParticleSet3 {
+  public:
+    ParticleSet3 *particleSet = nullptr;
+    Float targetDensity = 1000.0, targetSpacing = 0.1, kernelRadiusOverTargetSpacing = 2.0;
+    Float kernelRadius = 0.2;
+    ParticleSet3 *GetParticleSet() { return particleSet; }
+    Float GetKernelRadius() const { return kernelRadius; }
+    Float GetTargetDensity() const { return targetDensity; }
+    Float GetTargetSpacing() const { return targetSpacing; }
+    void SetTargetDensity(Float d) { targetDensity = d; }
+    void SetTargetSpacing(Float s) {
+        targetSpacing = s;
+        kernelRadius = kernelRadiusOverTargetSpacing * s;
+        if (particleSet) particleSet->radius = s;
+    }
+    void SetRelativeKernelRadius(Float r) {
+        kernelRadiusOverTargetSpacing = r;
+        kernelRadius = r * targetSpacing;
+    }
+};
+
+class ParticleSetBuilder3 {
+  public:
+    std::vector<vec3f> positions, velocities;
+    int AddParticle(const vec3f &pos, const vec3f &vel = vec3f(0), const vec3f & = vec3f(0)) {
+        positions.push_back(pos);
+        velocities.push_back(vel);
+        return 1;
+    }
+    int GetParticleCount() const { return (int)positions.size(); }
+    void SetVelocityForAll(const vec3f &v) {
+        for (auto &x : velocities) x = v;
+    }
+    ParticleSet3 *MakeParticleSet() {
+        if (positions.empty()) throw bubbles_error("No particles in builder");
+        ParticleSet3 *p = new ParticleSet3();
+        p->count = (int)positions.size();
+        p->pos.resize(3 * positions.size());
+        p->vel.resize(3 * positions.size());
+        for (size_t i = 0; i < positions.size(); i++)
+            for (int k = 0; k < 3; k++) {
+                p->pos[3 * i + k] = positions[i][k];
+                p->vel[3 * i + k] = velocities[i][k];
+            }
+        positions.clear();
+        velocities.clear();
+        return p;
+    }
+};
+
+inline SphParticleSet3 *SphParticleSet3FromBuilder(ParticleSetBuilder3 *builder) {
+    if (!builder) throw bubbles_error("Invalid builder pointer for SphParticleSet3FromBuilder");
+    SphParticleSet3 *s = new SphParticleSet3();
+    s->particleSet = builder->MakeParticleSet();
+    return s;
+}
+
+struct SphSolverData3 {
+    SphParticleSet3 *sphpSet = nullptr;
+    Grid3 *domain = nullptr;
+    ColliderSet3 *collider = nullptr;
+    Float dragCoefficient = 0.0001, eosExponent = 7.0, negativePressureScale = 0.0;
+    Float viscosity = 0.04, pseudoViscosity = 10.0, soundSpeed = 100.0, timestepScale = 1.0;
+    vec3f gravity = vec3f(0.0, (double)(-9.8f), 0.0);
+};
+
+/* DefaultSphSolverData3 (src/solvers/sph_solver3.cpp:124-151) */
+inline SphSolverData3 *DefaultSphSolverData3(bool with_gravity = true) {
+    SphSolverData3 *d = new SphSolverData3();
+    if (!with_gravity) d->gravity = vec3f(0);
+    return d;
+}
+
+namespace bubbles_detail {
+/* Create the device solver from the SphSolverData3 + Setup arguments. */
+inline bbx_solver *make_device(int kind, SphSolverData3 *d, Float targetDensity, Float targetSpacing,
+                               Float relativeRadius, Grid3 *domain, SphParticleSet3 *pSet) {
+    if (!d || !domain || !pSet || !pSet->particleSet)
+        throw bubbles_error("Invalid solver initialization");
+    bbx_params p = domain->prm;
+    p.target_density = targetDensity;
+    p.target_spacing = targetSpacing;
+    p.relative_radius = relativeRadius;
+    p.eos_exponent = d->eosExponent;
+    p.negative_pressure_scale = d->negativePressureScale;
+    p.viscosity = d->viscosity;
+    p.pseudo_viscosity = d->pseudoViscosity;
+    p.sound_speed = d->soundSpeed;
+    p.drag_coefficient = d->dragCoefficient;
+    for (int k = 0; k < 3; k++) p.gravity[k] = d->gravity[k];
+    ParticleSet3 *ps = pSet->particleSet;
+    bbx_solver *s = nullptr;
+    bbx_throw_if(bbx_solver_create(kind, &p, ps->pos.data(), ps->vel.data(), ps->count, &s), "Setup");
+    bbx_throw_if(bbx_solver_setup(s), "Setup");
+    bbx_info info;
+    bbx_throw_if(bbx_solver_info(s, &info), "Setup");
+    ps->dev = s;
+    ps->mass = info.mass;
+    ps->radius = info.radius;
+    pSet->targetDensity = targetDensity;
+    pSet->targetSpacing = targetSpacing;
+    pSet->kernelRadiusOverTargetSpacing = relativeRadius;
+    pSet->kernelRadius = info.kernel_radius;
+    d->domain = domain;
+    d->sphpSet = pSet;
+    return s;
+}
+
+/* the container must be one reversed box whose bounds define the grid domain */
+inline void check_colliders(ColliderSet3 *c, Grid3 *g) {
+    if (!c) return;
+    if (c->shapes.size() != 1 || !c->shapes[0]->reverseOrientation)
+        throw bubbles_error("SetColliders: one reversed box container is supported");
+    const Shape *s = c->shapes[0];
+    for (int k = 0; k < 3; k++)
+        if (g && (s->center[k] != g->prm.tank_center[k] || s->size[k] != g->prm.tank_size[k]))
+            throw bubbles_error("SetColliders: container must be the grid's domain box");
+}
+}  // namespace bubbles_detail
+
+/* PciSphSolver3 (src/core/pcisph_solver.h:56-83) */
+class PciSphSolver3 {
+  public:
+    /* loop settings; the reference's shipped loop is REFERENCE_EXACT, maxIt 5, tol 0.01 */
+    int maxIterations = 5;
+    Float maxErrorDensity = 0.01;
+    int loopMode = BBX_MODE_REFERENCE_EXACT;
+    int warmStart = BBX_WARM_NONE;
+
+    PciSphSolver3() {}
+    ~PciSphSolver3() {
+        if (dev_) bbx_solver_destroy(dev_);
+    }
+    void Initialize(SphSolverData3 *data) { data_ = data; }
+    void Setup(Float targetDensity, Float targetSpacing, Float relativeRadius, Grid3 *domain,
+               SphParticleSet3 *pSet) {
+        if (!data_) throw bubbles_error("Invalid call to {PciSphSolver3::Setup}");
+        pending_collider_check();
+        dev_ = bubbles_detail::make_device(BBX_PCISPH, data_, targetDensity, targetSpacing, relativeRadius,
+                                           domain, pSet);
+        bbx_info info;
+        bbx_throw_if(bbx_solver_info(dev_, &info), "Setup");
+        deltaDenom = info.delta_denom;
+        Float m = info.mass / targetDensity;
+        massOverTargetDensitySquared = m * m;
+    }
+    void SetColliders(ColliderSet3 *colliders) {
+        if (!data_) throw bubbles_error("Invalid solverData for {SetColliders}");
+        data_->collider = colliders;
+        pending_collider_check();
+    }
+    ColliderSet3 *GetColliders() { return data_ ? data_->collider : nullptr; }
+    void SetViscosityCoefficient(Float v) {
+        if (!data_) throw bubbles_error("Invalid solverData for {SetViscosityCoefficient}");
+        if (dev_) throw bubbles_error("SetViscosityCoefficient must precede Setup");
+        data_->viscosity = v > 0 ? v : 0;
+    }
+    SphSolverData3 *GetSphSolverData() { return data_; }
+    SphParticleSet3 *GetSphParticleSet() { return data_ ? data_->sphpSet : nullptr; }
+    int GetParticleCount() { return GetSphParticleSet()->GetParticleSet()->GetParticleCount(); }
+    /* PciSphSolver3::Advance (src/solvers/pcisph_solver3.cpp:67-89) */
+    void Advance(Float timeIntervalInSeconds) {
+        require();
+        int n = 0;
+        bbx_throw_if(bbx_pcisph_advance(dev_, timeIntervalInSeconds, maxIterations, maxErrorDensity,
+                                        loopMode, warmStart, &n),
+                     "Advance");
+        lastSubsteps = n;
+        GetSphParticleSet()->GetParticleSet()->Invalidate();
+    }
+    /* one fixed-dt substep (AdvanceTimeStep); returns its statistics */
+    bbx_stats AdvanceSubstep(Float dt) {
+        require();
+        bbx_stats st;
+        bbx_throw_if(bbx_pcisph_substep(dev_, dt, maxIterations, maxErrorDensity, loopMode, warmStart, &st),
+                     "AdvanceSubstep");
+        stepInterval = st.gpu_ms;
+        GetSphParticleSet()->GetParticleSet()->Invalidate();
+        return st;
+    }
+    Float GetAdvanceTime() const { return stepInterval; }
+    Float ComputeBeta(Float dt) const { return 2.0 * massOverTargetDensitySquared * dt * dt; }
+    Float ComputeDelta(Float dt) const {
+        return std::fabs(deltaDenom) > 0 ? -1 / (ComputeBeta(dt) * deltaDenom) : 0;
+    }
+    Float ComputeDeltaDenom() const { return deltaDenom; }
+    bbx_solver *Device() { return dev_; }
+
+    Float massOverTargetDensitySquared = 0, deltaDenom = 0, stepInterval = 0;
+    int lastSubsteps = 0;
+
+  private:
+    SphSolverData3 *data_ = nullptr;
+    bbx_solver *dev_ = nullptr;
+    void require() {
+        if (!dev_) throw bubbles_error("PciSphSolver3: call Setup first");
+    }
+    void pending_collider_check() {
+        if (data_ && data_->collider) bubbles_detail::check_colliders(data_->collider, data_->domain);
+    }
+};
+
+/* SphSolver3 (src/core/sph_solver.h:76-96), race-free two-phase integration */
+class SphSolver3 {
+  public:
+    SphSolver3() {}
+    ~SphSolver3() {
+        if (dev_) bbx_solver_destroy(dev_);
+    }
+    void Initialize(SphSolverData3 *data) { data_ = data; }
+    void Setup(Float targetDensity, Float targetSpacing, Float relativeRadius, Grid3 *domain,
+               SphParticleSet3 *pSet) {
+        if (!data_) throw bubbles_error("Invalid SphSolver3 initialization");
+        if (data_->collider) bubbles_detail::check_colliders(data_->collider, domain);
+        dev_ = bubbles_detail::make_device(BBX_SPH, data_, targetDensity, targetSpacing, relativeRadius,
+                                           domain, pSet);
+    }
+    void SetColliders(ColliderSet3 *c) {
+        if (!data_) throw bubbles_error("Invalid solverData for {SetColliders}");
+        data_->collider = c;
+        if (data_->domain) bubbles_detail::check_colliders(c, data_->domain);
+    }
+    ColliderSet3 *GetColliders() { return data_ ? data_->collider : nullptr; }
+    SphSolverData3 *GetSphSolverData() { return data_; }
+    void SetViscosityCoefficient(Float v) {
+        if (dev_) throw bubbles_error("SetViscosityCoefficient must precede Setup");
+        data_->viscosity = v > 0 ? v : 0;
+    }
+    void SetPseudoViscosityCoefficient(Float v) {
+        if (dev_) throw bubbles_error("SetPseudoViscosityCoefficient must precede Setup");
+        data_->pseudoViscosity = v > 0 ? v : 0;
+    }
+    Float GetKernelRadius() { return data_->sphpSet->GetKernelRadius(); }
+    SphParticleSet3 *GetSphParticleSet() { return data_ ? data_->sphpSet : nullptr; }
+    void Advance(Float timeIntervalInSeconds) {
+        if (!dev_) throw bubbles_error("SphSolver3: call Setup first");
+        int n = 0;
+        bbx_throw_if(bbx_sph_advance(dev_, timeIntervalInSeconds, &n), "Advance");
+        lastSubsteps = n;
+        data_->sphpSet->GetParticleSet()->Invalidate();
+    }
+    bbx_solver *Device() { return dev_; }
+    int lastSubsteps = 0;
+
+  private:
+    SphSolverData3 *data_ = nullptr;
+    bbx_solver *dev_ = nullptr;
+};
+
+#endif

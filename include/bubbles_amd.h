@@ -159,6 +159,14 @@ int bbx_grid_set_chains(bbx_solver *s, const int *cell_start, const int *members
 int bbx_grid_hash(bbx_solver *s, const double *points, int n, int *cell_ids);
 /* Grid3::GetNeighborsOf (src/core/grid.h:502-510): <=27 ids, (y,x,z) order */
 int bbx_grid_cell_neighbors(bbx_solver *s, int cell, int *out27, int *count);
+/* UtilBuildGridForDomain + Grid3::Build without a solver (src/core/util.cpp:269-287,
+ * src/core/grid.h:626-666): the grid the params describe. */
+int bbx_grid_describe(const bbx_params *params, bbx_grid_desc *out);
+/* Grid3::GetHashedPosition / GetLinearHashedPosition on a described grid
+ * (src/core/grid.h:258-321); u may be NULL (else 3 ints per point); ids are
+ * -1 for points outside the grid. */
+int bbx_grid_hash_desc(const bbx_grid_desc *grid, const double *points, int n, int *cell_ids,
+                       int *u);
 /* ParticleSet3::GetParticleBucket for all particles (src/core/particle.h:276-279,
  * Bucket::Get :52-59): counts[n], ids[n*100] (row i = particle i). */
 int bbx_neighbors_get(bbx_solver *s, int *counts, int *ids);
