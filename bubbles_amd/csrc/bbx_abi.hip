@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -254,6 +255,7 @@ void fill_sim_const(bbx_solver *s) {
     c.spiky_grad = (float)(45.0 / (M_PI * h * h * h * h));
     c.spiky_lap = (float)(90.0 / (M_PI * h * h * h * h * h));
     c.h2d = h * h;
+    c.rho0d = s->prm.target_density;
     // FP32 prefilter band: outside [T(1-4e-6), T(1+4e-6)] with T = h^2 - 1e-8 the
     // FP32 squared distance (relative error < 5 * 2^-24) decides; inside it the
     // FP64 rule decides.
@@ -285,6 +287,12 @@ void fill_sim_const(bbx_solver *s) {
     c.eos_scale = (float)(s->prm.target_density * s->prm.sound_speed * s->prm.sound_speed);
     c.eos_exp = (float)s->prm.eos_exponent;
     c.pseudo_visc = (float)s->prm.pseudo_viscosity;
+    // LDS tile capacity; BBX_TILE_CAP (0 .. kTileCap) overrides it for A/B timing
+    c.tile_cap = kTileCapHost;
+    if (const char *e = std::getenv("BBX_TILE_CAP")) {
+        int v = std::atoi(e);
+        c.tile_cap = v < 0 ? 0 : (v > kTileCapHost ? kTileCapHost : v);
+    }
 }
 
 // host-side hash: Grid::GetLinearHashedPosition (src/core/grid.h:258-321)
@@ -337,11 +345,11 @@ static void free_all(bbx_solver *s) {
     dfree(s->nbr16); dfree(s->nbr32); dfree(s->nbr_cnt); dfree(s->tile_tab);
     dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->q);
     dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
-    dfree(s->max_err); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
-    if (s->h_max_err) (void)hipHostFree(s->h_max_err);
+    dfree(s->loop); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
+    if (s->h_loop) (void)hipHostFree(s->h_loop);
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     if (s->h_max_force) (void)hipHostFree(s->h_max_force);
-    s->h_max_err = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
+    s->h_loop = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     s->ev0 = s->ev1 = nullptr;
@@ -475,11 +483,11 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->xstar, N);
     if (!rc) rc = dalloc(&s->feat, N * BBX_NFEAT);
     if (!rc) rc = dalloc(&s->pwarm, N);
-    if (!rc) rc = dalloc(&s->max_err, 1);
+    if (!rc) rc = dalloc(&s->loop, 1);
     if (!rc) rc = dalloc(&s->max_force, 1);
     if (!rc) rc = dalloc(&s->flag, 1);
     if (!rc) rc = dalloc(&s->counters, CNT_N);
-    if (!rc && hipHostMalloc((void **)&s->h_max_err, sizeof(unsigned long long)) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_loop, sizeof(LoopState)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * CNT_N) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
@@ -514,7 +522,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (e == hipSuccess) e = hipMemset(s->tile_tab, 0, sizeof(int) * (size_t)s->ntiles * kTabIntsHost);
     if (e == hipSuccess) e = hipMemset(s->feat, 0, sizeof(float) * N * BBX_NFEAT);
     if (e == hipSuccess) e = hipMemset(s->max_force, 0, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(s->max_err, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->loop, 0, sizeof(LoopState));
     if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(int));
     if (e == hipSuccess) e = hipMemset(s->counters, 0, sizeof(int) * CNT_N);
     if (e != hipSuccess) {
@@ -670,11 +678,14 @@ int bbx_solver_set_flag(bbx_solver *s, int flag) {
     return BBX_OK;
 }
 
-static int read_stats(bbx_solver *s, bbx_stats *st) {
+// One readback per call: counters, max |f| and (PCISPH) the loop state.
+// iterations = index of the first iteration whose done flag is set, + 1;
+// the reported error is that iteration's FP64 max error / rho0.
+static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
     BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->counters, sizeof(int) * CNT_N, hipMemcpyDeviceToHost,
                                  s->stream));
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_err, s->max_err, sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, s->stream));
+    if (loop_len > 0)
+        BBX_HIP_CHECK(hipMemcpyAsync(s->h_loop, s->loop, sizeof(LoopState), hipMemcpyDeviceToHost, s->stream));
     BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned), hipMemcpyDeviceToHost,
                                  s->stream));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -682,9 +693,20 @@ static int read_stats(bbx_solver *s, bbx_stats *st) {
         st->full_redistribution = s->h_counters[CNT_FULL];
         st->bucket_overflows = s->h_counters[CNT_OVERFLOW];
         st->lost_particles = s->h_counters[CNT_LOST];
-        double e;
-        std::memcpy(&e, s->h_max_err, sizeof(double));
-        st->max_density_error = e / s->prm.target_density;
+        st->iterations = 0;
+        st->max_density_error = 0;
+        if (loop_len > 0) {
+            int it = loop_len;
+            for (int k = 0; k < loop_len; k++)
+                if (s->h_loop->done[k]) {
+                    it = k + 1;
+                    break;
+                }
+            double e;
+            std::memcpy(&e, &s->h_loop->err[it - 1], sizeof(double));
+            st->iterations = it;
+            st->max_density_error = e / s->prm.target_density;
+        }
     }
     return BBX_OK;
 }
@@ -947,12 +969,14 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
                        bbx_stats *stats) {
     BBX_CHECK(check_setup(s));
     if (s->kind != BBX_PCISPH) return fail(BBX_E_INVALID, "not a PCISPH solver");
-    if (max_it < 1) return fail(BBX_E_INVALID, "max_it must be >= 1");
+    if (max_it < 1 || max_it > kMaxIt)
+        return fail(BBX_E_INVALID, "max_it must be in [1, " + std::to_string(kMaxIt) + "]");
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
     if (warm == BBX_WARM_MLP && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
     BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
     BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, 1));
+    BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
     // UpdateGridDistribution + ResetHigherLevel + interaction + density
     BBX_CHECK(distribute(s, false, true, false));
     const double delta = compute_delta(s, dt);
@@ -967,37 +991,30 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
                                    (float)(delta * s->prm.target_density), s->pwarm, s->stream));
         BBX_CHECK(run_warm_init(s));
         // f_p(p_warm, rho) and the k = 0 prediction
-        BBX_CHECK(run_pressure_force(s, st, 0));
+        BBX_CHECK(run_pressure_force(s, st, -1, max_it, 0, tol));
     } else {
         BBX_CHECK(run_nonpressure(s, st, 1, 0));
     }
-    int it = 0;
-    double ratio = 0;
-    for (int k = 0; k < max_it; k++) {
+    // The whole predict-correct loop is queued without host round trips:
+    // the pressure-force kernel of iteration k reads that iteration's FP64
+    // error on the device and either integrates or predicts iteration k + 1;
+    // kernels of later iterations exit at entry.  REFERENCE_EXACT stops after
+    // the first iteration (its error is never recorded, so it never converges
+    // earlier and never runs longer).
+    const int exact = mode == BBX_MODE_REFERENCE_EXACT ? 1 : 0;
+    const int launches = exact ? 1 : max_it;
+    for (int k = 0; k < launches; k++) {
         st.first = (k == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
-        BBX_CHECK(run_predict_pressure(s, st));
-        it = k + 1;
-        bool last = (mode == BBX_MODE_REFERENCE_EXACT) || (k == max_it - 1);
-        if (!last) {
-            BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_err, s->max_err, sizeof(unsigned long long),
-                                         hipMemcpyDeviceToHost, s->stream));
-            BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
-            double e;
-            std::memcpy(&e, s->h_max_err, sizeof(double));
-            ratio = e / s->prm.target_density;
-            if (absd(ratio) < tol) last = true;
-        }
-        BBX_CHECK(run_pressure_force(s, st, last ? 1 : 0));
-        if (last) break;
+        BBX_CHECK(run_predict_pressure(s, st, k));
+        BBX_CHECK(run_pressure_force(s, st, k, max_it, exact, tol));
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
     BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
     bbx_stats local{};
-    BBX_CHECK(read_stats(s, &local));
+    BBX_CHECK(read_stats(s, &local, launches));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
-    local.iterations = it;
     local.gpu_ms = ms;
     if (stats) *stats = local;
     if (s->prof.enabled) s->prof.resolve();

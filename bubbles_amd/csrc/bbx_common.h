@@ -40,6 +40,7 @@ constexpr int kBucket = BBX_BUCKET_SIZE;   // reference bucket cap
 constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
 constexpr int kScanItems = 8;              // items per thread in the cell scan
 constexpr int kTabIntsHost = 32;           // ints per tile table (see bbx_tile.h)
+constexpr int kTileCapHost = 3072;         // LDS tile capacity in particles (bbx_tile.h)
 
 // Grid geometry in FP64 (the hash is computed in the reference's precision).
 struct GridGeom {
@@ -62,6 +63,7 @@ struct SimConst {
     float spiky_grad;  // 45 / (pi h^4)
     float spiky_lap;   // 90 / (pi h^5)
     double h2d;        // FP64 kernel radius squared for the bucket rule
+    double rho0d;      // FP64 rest density (convergence ratio)
     float lo2, hi2;    // FP32 prefilter band around the FP64 cutoff
     float mass, mass2, rho0, visc, drag, nps;
     float gx, gy, gz;
@@ -71,6 +73,7 @@ struct SimConst {
     float min_len;     // min cell length (0.9-cell flag)
     float eos_scale, eos_exp;
     float pseudo_visc;
+    int tile_cap;      // tiles with more staged candidates read global memory
 };
 
 // Per-call step constants.
@@ -78,6 +81,15 @@ struct StepConst {
     float dt, dt_over_m;
     float delta;       // PCISPH stiffness (FP64 on host, rounded for the update)
     int first;         // k == 0 without warm start: pressure starts at 0
+};
+
+// predict-correct loop state on the device: per-iteration FP64 max error
+// bits and done flags (iteration k's pressure-force kernel integrates and
+// sets done[k] once the error of iteration k is below the tolerance)
+constexpr int kMaxIt = 64;
+struct LoopState {
+    unsigned long long err[kMaxIt];
+    int done[kMaxIt];
 };
 
 // device-side counters
@@ -153,6 +165,8 @@ struct bbx_solver {
     float *feat = nullptr;       // [16][stride]
     float *pwarm = nullptr;      // MLP output
     // reductions / flags
+    bbx::LoopState *loop = nullptr;          // predict-correct loop state (device)
+    bbx::LoopState *h_loop = nullptr;        // pinned readback
     unsigned long long *max_err = nullptr;   // FP64 bits of max |error|
     unsigned *max_force = nullptr;           // FP32 bits of max |f|
     int *flag = nullptr;                     // requiresHigherLevelUpdate
@@ -180,8 +194,10 @@ int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
 int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure);
 int run_density_only(bbx_solver *s);
 int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features);
-int run_predict_pressure(bbx_solver *s, const StepConst &st);
-int run_pressure_force(bbx_solver *s, const StepConst &st, int integrate);
+int run_predict_pressure(bbx_solver *s, const StepConst &st, int k);
+// k < 0: predict only (warm start); otherwise integrate when iteration k is
+// converged (or k == max_it - 1, or exact), else predict iteration k + 1
+int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, int exact, double tol);
 int run_sph_force(bbx_solver *s);
 int run_integrate(bbx_solver *s, const StepConst &st);
 int run_xsph(bbx_solver *s, const StepConst &st);

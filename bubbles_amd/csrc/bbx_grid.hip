@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        tab_build(c.g, smin, smax, start, count, tt, rdelta);
+        tab_build(c.g, smin, smax, start, count, c.tile_cap, tt, rdelta);
         tab_store(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
     }
     __syncthreads();
@@ -281,29 +281,46 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     float sum = 0.f, comp = 0.f;
     const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2];
     const int ylo = hc >= 0 ? u[1] - 1 : 1, yhi = hc >= 0 ? u[1] + 1 : 0;
+    // neighbour cells in (y,x,z) order; their ranges are loaded up front so
+    // the 27 start/count loads are in flight together
+    int cell[27], shift[27];
+    int ncell = 0;
     for (int y = ylo; y <= yhi; y++) {
         if (y < 0 || y >= ny) continue;
         for (int x = u[0] - 1; x <= u[0] + 1; x++) {
             if (x < 0 || x >= nx) continue;
             for (int z = u[2] - 1; z <= u[2] + 1; z++) {
                 if (z < 0 || z >= nz) continue;
-                const int nc = x + y * nx + z * nx * ny;
-                const int b = start[nc], e = b + count[nc];
-                const int shift = staged ? rdelta[(z - u[2] + 1) * 3 + (y - u[1] + 1)] : 0;
-                for (int j = b; j < e; j++) {
-                    const int jl = j + shift;
-                    const float4 pj = staged ? sx[jl] : pos[j];
-                    if (!in_bucket(c, pi, pj)) continue;
-                    if (cnt < kBucket) {
-                        if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)jl;
-                        else nbr32[(size_t)cnt * c.stride + i] = j;
-                        cnt++;
-                        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-                    } else {
-                        dropped++;
-                    }
-                }
+                cell[ncell] = x + y * nx + z * nx * ny;
+                shift[ncell] = staged ? rdelta[(z - u[2] + 1) * 3 + (y - u[1] + 1)] : 0;
+                ncell++;
+            }
+        }
+    }
+    int cb[27], ce[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (q < ncell) {
+            cb[q] = start[cell[q]];
+            ce[q] = cb[q] + count[cell[q]];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (q >= ncell) break;
+        const int b = cb[q], e = ce[q], sh = shift[q];
+        for (int j = b; j < e; j++) {
+            const int jl = j + sh;
+            const float4 pj = staged ? sx[jl] : pos[j];
+            if (!in_bucket(c, pi, pj)) continue;
+            if (cnt < kBucket) {
+                if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)jl;
+                else nbr32[(size_t)cnt * c.stride + i] = j;
+                cnt++;
+                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+            } else {
+                dropped++;
             }
         }
     }
@@ -338,12 +355,11 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
     const float4 pi = pos[i];
     float sum = 0.f, comp = 0.f;
     const int cnt = nbr_cnt[i];
-    for (int k = 0; k < cnt; k++) {
-        const size_t at = (size_t)k * c.stride + i;
-        const float4 pj = staged ? sx[nbr16[at]] : pos[nbr32[at]];
+    for_each_neighbour<8>(nbr16, nbr32, staged, i, cnt, c.stride, [&](int j) {
+        const float4 pj = staged ? sx[j] : pos[j];
         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-    }
+    });
     rho[i] = sum * c.mass;
 }
 

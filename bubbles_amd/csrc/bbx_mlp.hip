@@ -30,7 +30,10 @@ constexpr int kTrainable = OFF_B3 + 1;    // 5313
 constexpr int OFF_MEAN = kTrainable;      // feature mean [16]
 constexpr int OFF_ISTD = OFF_MEAN + kF;   // feature 1/std [16]
 constexpr int OFF_OSCALE = OFF_ISTD + kF; // target scale [1]
-constexpr int kParams = OFF_OSCALE + 1;   // 5346
+constexpr int OFF_ALPHA = OFF_OSCALE + 1; // warm start = alpha * clamp(prediction, 0, pcap)
+constexpr int OFF_PCAP = OFF_ALPHA + 1;
+constexpr int OFF_RESID = OFF_PCAP + 1;   // prediction = net * out_scale + resid * x[PPREV]
+constexpr int kParams = OFF_RESID + 1;    // 5349
 
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kTilesPerWave = 4;  // training tiles per wave per step
@@ -139,19 +142,21 @@ __device__ __forceinline__ void forward_tile(const Weights &w, const float x4[4]
 }
 
 // ---------------------------------------------------------------------------
-// inference: out[i] = scale * max(0, y_i * out_scale)
-// features of sample i, feature f at x[f * fstride + i * sstride]
+// inference: prediction_i = y_i * out_scale + resid * x_i[PPREV] (target
+// units).  Solver path (clamp = 1): out_i = scale * alpha * clamp(prediction,
+// 0, pcap); raw path (clamp = 0): out_i = scale * prediction.
+// Features of sample i, feature f at x[f * fstride + i * sstride].
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restrict__ P,
                                                            const float *__restrict__ x, int n,
                                                            int fstride, int sstride, float scale,
-                                                           int clamp0, float *__restrict__ out) {
+                                                           int clamp, float *__restrict__ out) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * kWaves;
     Weights w;
     load_weights(P, w, lane);
-    const float oscale = P[OFF_OSCALE];
+    const float oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA], pcap = P[OFF_PCAP], resid = P[OFF_RESID];
     const int q = lane >> 4, j = lane & 15;
     const int tiles = (n + 15) / 16;
     for (int t = wave; t < tiles; t += nwaves) {
@@ -163,7 +168,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restri
         forward_tile(w, x4, a, lane);
         if (lane < 16 && i < n) {
             float y = a.y * oscale;
-            if (clamp0) y = fmaxf(y, 0.f);
+            if (resid != 0.f) y += resid * x[(size_t)BBX_F_PPREV * fstride + (size_t)i * sstride];
+            if (clamp) y = alpha * fminf(fmaxf(y, 0.f), pcap);
             out[i] = y * scale;
         }
     }
@@ -460,6 +466,9 @@ int bbx_mlp_create(int seed, bbx_mlp **out) {
     fill(OFF_W3, kH, kH, 1);
     for (int f = 0; f < kF; f++) p[OFF_ISTD + f] = 1.f;
     p[OFF_OSCALE] = 1.f;
+    p[OFF_ALPHA] = 1.f;
+    p[OFF_PCAP] = 3.0e38f;
+    p[OFF_RESID] = 0.f;
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&m->params, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc((void **)&m->m1, sizeof(float) * kTrainable);

@@ -83,8 +83,7 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
                                                         float4 *__restrict__ force,
                                                         float4 *__restrict__ xstar,
                                                         float *__restrict__ rho_star,
-                                                        float *__restrict__ feat,
-                                                        unsigned long long *__restrict__ max_err) {
+                                                        float *__restrict__ feat) {
     __shared__ float4 sx[kTileCap];
     __shared__ float4 sv[kTileCap];
     __shared__ TileTab tt;
@@ -100,7 +99,6 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     }
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0 && max_err) *max_err = 0ull;
     if (i >= c.n) return;
     const int self = staged ? tab_local(tt, i) : i;
     const float4 xi = pos[i];
@@ -111,16 +109,12 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     float fz = c.mass * c.gz - c.drag * vi.z;
     const int cnt = nb.cnt[i];
     float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-    for (int k = 0; k < cnt; k++) {
-        const size_t at = (size_t)k * c.stride + i;
-        int j;
+    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
         float4 xj, vj;
         if (staged) {
-            j = nb.n16[at];
             xj = sx[j];
             vj = sv[j];
         } else {
-            j = nb.n32[at];
             xj = pos[j];
             float4 v = vel[j];
             vj = make_float4(v.x, v.y, v.z, rho[j]);
@@ -152,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
                 divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
             }
         }
-    }
+    });
     force[i] = make_float4(fx, fy, fz, 0.f);
     if (kMode == 1) {
         xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
@@ -194,8 +188,9 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
 
 // ---------------------------------------------------------------------------
 // predicted density and pressure update (PredictPressureFor,
-// src/equations/pcisph_equations3.cpp:60-90) + FP64 max-error reduction.
-// LDS: predicted positions.
+// src/equations/pcisph_equations3.cpp:60-90) + FP64 max-error reduction
+// into the error slot of iteration k.  Does nothing once an earlier
+// iteration integrated.  LDS: predicted positions.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepConst st, double delta,
                                                              NbrTab nb,
@@ -203,10 +198,11 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
                                                              float *__restrict__ pres,
                                                              float *__restrict__ rho_star,
                                                              float *__restrict__ q,
-                                                             unsigned long long *__restrict__ max_err) {
+                                                             LoopState *__restrict__ loop, int k) {
     __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kBlock / 64];
+    if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
@@ -218,12 +214,11 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         const float4 xi = xstar[i];
         float sum = 0.f, comp = 0.f;
         const int cnt = nb.cnt[i];
-        for (int k = 0; k < cnt; k++) {
-            const size_t at = (size_t)k * c.stride + i;
-            const float4 xj = staged ? sx[nb.n16[at]] : xstar[nb.n32[at]];
+        for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+            const float4 xj = staged ? sx[j] : xstar[j];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-        }
+        });
         float density = sum * c.mass;
         // FP64 error and pressure increment (delta is FP64 on the host)
         double err = (double)density - (double)c.rho0;
@@ -237,7 +232,9 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         rho_star[i] = density;
         float d2 = density * density;
         q[i] = (fabsf(d2) < 1e-8f) ? 0.f : p / d2;
-        double ae = err > 0 ? err : -err;
+        // |err| with a cleared sign bit: -0.0 must not win the unsigned max
+        // (the reference's AbsfMax keeps the first of equal squares)
+        const double ae = fabs(err);
         ebits = (unsigned long long)__double_as_longlong(ae);
     }
     ebits = wave_max_u64(ebits);
@@ -246,18 +243,23 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     if (threadIdx.x == 0) {
         unsigned long long m = wmax[0];
         for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
-        if (m) atomicMax(max_err, m);
+        if (m) atomicMax(&loop->err[k], m);
     }
 }
 
 // ---------------------------------------------------------------------------
 // pressure force (PredictPressureForceFor, src/equations/pcisph_equations3.cpp:114-155)
 // at the current positions with predicted densities (q = p / rho*^2); then
-// either predict the next iteration's x* (kIntegrate = 0) or accumulate and
-// integrate (AccumulateForcesFor + TimeIntegrationFor).
+// either predict the next iteration's x* or accumulate and integrate
+// (AccumulateForcesFor + TimeIntegrationFor).
 // LDS: (position, q).
 // ---------------------------------------------------------------------------
-template <int kIntegrate>
+// Loop control (ComputePressureForceAndIntegrate, :218-256) is decided here
+// on the device: for iteration k >= 0 the kernel integrates when the FP64
+// max error of iteration k is below tol (ratio |max err| / rho0 < tol, as the
+// reference computes it), when k is the last iteration, or in REFERENCE_EXACT
+// mode; otherwise it predicts x* for iteration k + 1.  It does nothing when
+// an earlier iteration already integrated.  k < 0 is the warm-start predict.
 __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst st, NbrTab nb,
                                                            const float4 *__restrict__ pos,
                                                            const float4 *__restrict__ vel,
@@ -271,9 +273,22 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            float4 *__restrict__ force_out,
                                                            int *__restrict__ flag,
                                                            unsigned *__restrict__ max_force,
-                                                           unsigned long long *__restrict__ max_err) {
+                                                           LoopState *__restrict__ loop, int k,
+                                                           int max_it, int exact, double tol) {
     __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
+    if (k > 0 && loop->done[k - 1]) {
+        // an earlier iteration integrated: pass the flag on and exit (uniform)
+        if (blockIdx.x == 0 && threadIdx.x == 0) loop->done[k] = 1;
+        return;
+    }
+    bool integrate = false;
+    if (k >= 0) {
+        const double e = __longlong_as_double((long long)loop->err[k]);
+        const double ratio = e / c.rho0d;
+        integrate = exact || k == max_it - 1 || fabs(ratio) < tol;
+    }
+    if (integrate && blockIdx.x == 0 && threadIdx.x == 0) loop->done[k] = 1;
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
@@ -284,7 +299,6 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         });
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (!kIntegrate && i == 0) *max_err = 0ull;
     unsigned fbits = 0u;
     if (i < c.n) {
         const int self = staged ? tab_local(tt, i) : i;
@@ -292,33 +306,29 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         const float qi = q[i];
         float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
         const int cnt = nb.cnt[i];
-        for (int k = 0; k < cnt; k++) {
-            const size_t at = (size_t)k * c.stride + i;
-            int j;
+        for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+            if (j == self) return;
             float4 xj;
             if (staged) {
-                j = nb.n16[at];
                 xj = sx[j];
             } else {
-                j = nb.n32[at];
                 float4 p = pos[j];
                 xj = make_float4(p.x, p.y, p.z, q[j]);
             }
-            if (j == self) continue;
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
             float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (!spiky_in(c, r) || !(r > 0.f) || fabsf(r) < 1e-8f) continue;
+            if (!spiky_in(c, r) || !(r > 0.f) || fabsf(r) < 1e-8f) return;
             float x = 1.f - r * c.inv_h;
             // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r
             float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
             kahan_add(tx, ex, g * dx);
             kahan_add(ty, ey, g * dy);
             kahan_add(tz, ez, g * dz);
-        }
+        });
         const float fpx = -tx, fpy = -ty, fpz = -tz;
         const float4 fi = force[i];
         const float4 vi = vel[i];
-        if (!kIntegrate) {
+        if (!integrate) {
             xstar[i] = predict_position(c, st, xi, vi, fi.x + fpx, fi.y + fpy, fi.z + fpz);
         } else {
             const float Fx = fi.x + fpx, Fy = fi.y + fpy, Fz = fi.z + fpz;
@@ -331,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             fbits = __float_as_uint(sqrtf(Fx * Fx + Fy * Fy + Fz * Fz));
         }
     }
-    if (kIntegrate) {
+    if (integrate) {
         fbits = wave_max_u32(fbits);
         if ((threadIdx.x & 63) == 0 && fbits) atomicMax(max_force, fbits);
     }
@@ -380,23 +390,19 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     float fz = c.mass * c.gz - c.drag * vi.z;
     float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
     const int cnt = nb.cnt[i];
-    for (int k = 0; k < cnt; k++) {
-        const size_t at = (size_t)k * c.stride + i;
-        int j;
+    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        if (j == self) return;
         float4 xj, vj;
         if (staged) {
-            j = nb.n16[at];
             xj = sx[j];
             vj = sv[j];
         } else {
-            j = nb.n32[at];
             xj = xq(j);
             vj = vr(j);
         }
-        if (j == self) continue;
         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (!spiky_in(c, r)) continue;
+        if (!spiky_in(c, r)) return;
         float x = 1.f - r * c.inv_h;
         if (r > 0.f && !(fabsf(r) < 1e-8f)) {
             float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
@@ -408,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
         fx += (vj.x - vi.x) * s;
         fy += (vj.y - vi.y) * s;
         fz += (vj.z - vi.z) * s;
-    }
+    });
     force[i] = make_float4(fx - tx, fy - ty, fz - tz, 0.f);
 }
 
@@ -463,15 +469,12 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     float sx_ = 0.f, sy_ = 0.f, sz_ = 0.f, ws = 0.f;
     const float spiky_w = c.spiky_lap * c.h * c.h / 6.f;  // 15 / (pi h^3)
     const int cnt = nb.cnt[i];
-    for (int k = 0; k < cnt; k++) {
-        const size_t at = (size_t)k * c.stride + i;
+    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
         float4 xj, vj;
         if (staged) {
-            const int j = nb.n16[at];
             xj = sx[j];
             vj = sv[j];
         } else {
-            const int j = nb.n32[at];
             xj = pos[j];
             vj = vr(j);
         }
@@ -487,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
         sx_ += wj * vj.x;
         sy_ += wj * vj.y;
         sz_ += wj * vj.z;
-    }
+    });
     if (ws > 0.f) {
         float inv = 1.f / ws;
         sx_ *= inv;
@@ -527,55 +530,46 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
     if (features) {
         PROF_BEGIN("nonpressure_features");
         k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, s->max_err);
+                                                       s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     } else if (predict) {
         PROF_BEGIN("nonpressure_predict");
         k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, s->max_err);
+                                                       s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     } else {
         PROF_BEGIN("nonpressure");
         k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, nullptr);
+                                                       s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
         PROF_END();
     }
     return BBX_OK;
 }
 
-int run_predict_pressure(bbx_solver *s, const StepConst &st) {
+int run_predict_pressure(bbx_solver *s, const StepConst &st, int k) {
     if (s->n == 0) return BBX_OK;
+    if (k < 0 || k >= kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
     // the pressure increment uses the host's FP64 delta (s->delta64)
     PROF_BEGIN("predict_pressure");
     k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
-                                                                s->pres, s->rho_star, s->q, s->max_err);
+                                                                s->pres, s->rho_star, s->q, s->loop, k);
     BBX_HIP_CHECK(hipGetLastError());
     PROF_END();
     return BBX_OK;
 }
 
-int run_pressure_force(bbx_solver *s, const StepConst &st, int integrate) {
+int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, int exact, double tol) {
     if (s->n == 0) return BBX_OK;
-    const unsigned nb = nblocks(s->n);
-    const NbrTab t = nbr_tab(s);
-    if (integrate) {
-        PROF_BEGIN("pressure_force_integrate");
-        k_pressure_force<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->force, s->q,
-                                                          s->pres, s->xstar, s->fp, s->posB, s->velB,
-                                                          s->force, s->flag, s->max_force, s->max_err);
-        BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
-    } else {
-        PROF_BEGIN("pressure_force_predict");
-        k_pressure_force<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->force, s->q,
-                                                          s->pres, s->xstar, s->fp, s->posB, s->velB,
-                                                          s->force, s->flag, s->max_force, s->max_err);
-        BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
-    }
+    if (k >= kMaxIt || max_it > kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
+    PROF_BEGIN("pressure_force");
+    k_pressure_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(
+        s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->q, s->pres, s->xstar, s->fp, s->posB,
+        s->velB, s->force, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol);
+    BBX_HIP_CHECK(hipGetLastError());
+    PROF_END();
     return BBX_OK;
 }
 

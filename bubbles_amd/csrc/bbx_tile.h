@@ -17,7 +17,7 @@
 
 namespace bbx {
 
-constexpr int kTileCap = 3072;   // staged particles per tile
+constexpr int kTileCap = kTileCapHost;  // staged particles per tile
 constexpr int kMaxIvl = 9;
 constexpr int kTabInts = kTabIntsHost;  // per-tile table stride in ints
 
@@ -54,7 +54,7 @@ __device__ __forceinline__ int tab_local(const TileTab &t, int j) {
 // row offset, LDS index - storage index of that row's interval.
 __device__ __forceinline__ void tab_build(const GridGeom &g, int cmin, int cmax,
                                           const int *__restrict__ start,
-                                          const int *__restrict__ count, TileTab &t,
+                                          const int *__restrict__ count, int cap, TileTab &t,
                                           int *__restrict__ rdelta) {
     t.n_ivl = 0;
     t.total = 0;
@@ -105,11 +105,39 @@ __device__ __forceinline__ void tab_build(const GridGeom &g, int cmin, int cmax,
         off += t.len[q];
     }
     t.total = off;
-    t.staged = off <= kTileCap ? 1 : 0;
+    t.staged = off <= cap ? 1 : 0;
     for (int r = 0; r < 9; r++)
         if (row_a[r] >= 0)
             for (int q = 0; q < t.n_ivl; q++)
                 if (row_a[r] >= t.start[q] && row_a[r] < t.start[q] + t.len[q]) rdelta[r] = t.off[q] - t.start[q];
+}
+
+// Walk bucket slots 0..cnt-1 of particle i, software-pipelined: the next U
+// bucket entries are loaded while the current U neighbours are processed, so
+// each lane keeps several loads in flight instead of one dependent chain per
+// neighbour.  body(j) gets an LDS-local index (staged tile) or a storage
+// index, in bucket order.
+template <int U, typename Body>
+__device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
+                                                   const int *__restrict__ n32, bool staged,
+                                                   int i, int cnt, int stride, Body body) {
+    auto entry = [&](int k) -> int {
+        const size_t at = (size_t)k * stride + i;
+        return staged ? (int)n16[at] : n32[at];
+    };
+    int cur[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) cur[u] = u < cnt ? entry(u) : 0;
+    for (int k = 0; k < cnt; k += U) {
+        int nxt[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) nxt[u] = (k + U + u) < cnt ? entry(k + U + u) : 0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < cnt) body(cur[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) cur[u] = nxt[u];
+    }
 }
 
 // Cooperative copy of the tile's intervals into LDS: dst[off + e] = fn(start + e).

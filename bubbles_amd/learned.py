@@ -73,20 +73,36 @@ def harvest(scene_list, steps=400, every=25, start=25, per_snapshot=8192, max_it
     return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
 
 
-def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0):
+F_PPREV = 10        # feature: previous substep's converged pressure / P
+N_TRAIN = 5313      # trainable parameters; then mean, inv_std, out_scale, alpha, pcap, resid
+OFF_MEAN = N_TRAIN
+OFF_ISTD = OFF_MEAN + 16
+OFF_OSCALE = OFF_ISTD + 16
+OFF_ALPHA, OFF_PCAP, OFF_RESID = OFF_OSCALE + 1, OFF_OSCALE + 2, OFF_OSCALE + 3
+
+
+def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0, alpha=0.8):
+    """Train the network on the residual t - p_prev (the previous substep's
+    converged pressure is the strongest single predictor); the solver then
+    starts from alpha * clamp(p_prev + net, 0, pcap).  Under-estimating is
+    safe -- the unchanged loop adds pressure -- while an over-estimate can
+    never be removed (negative increments are scaled by 0), hence alpha < 1."""
     net = abi.Mlp(seed)
     p = net.get_params()
-    from_ = abi.Mlp.param_count()
     mean = x.mean(axis=0)
     std = x.std(axis=0)
     std[std < 1e-12] = 1.0
-    nf = abi.NFEAT
-    off_mean = from_ - 2 * nf - 1
-    p[off_mean:off_mean + nf] = mean
-    p[off_mean + nf:off_mean + 2 * nf] = 1.0 / std
-    p[-1] = max(float(t.std()), 1e-6)
+    res = (t - x[:, F_PPREV]).astype(np.float32)
+    p[OFF_MEAN:OFF_MEAN + 16] = mean
+    p[OFF_ISTD:OFF_ISTD + 16] = 1.0 / std
+    p[OFF_OSCALE] = max(float(res.std()), 1e-6)
     net.set_params(p)
-    loss = net.train(x, t, steps=steps, batch=batch, lr=lr, seed=seed)
+    loss = net.train(x, res, steps=steps, batch=batch, lr=lr, seed=seed)
+    p = net.get_params()
+    p[OFF_ALPHA] = alpha
+    p[OFF_PCAP] = 1.5 * float(t.max())
+    p[OFF_RESID] = 1.0
+    net.set_params(p)
     return net, loss
 
 

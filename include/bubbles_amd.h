@@ -197,7 +197,10 @@ int bbx_number_of_time_steps(bbx_solver *s, double interval, double scale, unsig
 int bbx_mlp_create(int seed, bbx_mlp **out);
 int bbx_mlp_destroy(bbx_mlp *m);
 /* params layout: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1] in FP32
- * (master copy), then feature mean[16], feature inv_std[16], out_scale[1]. */
+ * (master copy, trained), then feature mean[16], feature inv_std[16],
+ * out_scale[1], alpha[1], pcap[1], resid[1] (not trained).  The prediction in
+ * target units is net * out_scale + resid * feature[BBX_F_PPREV]; the solver
+ * starts from p = delta*rho0 * alpha * clamp(prediction, 0, pcap). */
 int bbx_mlp_param_count(int *count);
 int bbx_mlp_get_params(bbx_mlp *m, float *out);
 int bbx_mlp_set_params(bbx_mlp *m, const float *in);

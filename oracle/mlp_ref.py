@@ -28,7 +28,11 @@ N_TRAIN = OFF_B3 + 1
 OFF_MEAN = N_TRAIN
 OFF_ISTD = OFF_MEAN + F
 OFF_OSCALE = OFF_ISTD + F
-N_PARAMS = OFF_OSCALE + 1
+OFF_ALPHA = OFF_OSCALE + 1
+OFF_PCAP = OFF_ALPHA + 1
+OFF_RESID = OFF_PCAP + 1
+N_PARAMS = OFF_RESID + 1
+F_PPREV = 10  # feature index of the previous substep's pressure
 
 K0 = 0.7978845608028654
 K1 = 0.044715
@@ -58,6 +62,7 @@ def unpack(p):
         "W2": p[OFF_W2:OFF_B2].reshape(H, H), "b2": p[OFF_B2:OFF_W3],
         "W3": p[OFF_W3:OFF_B3], "b3": p[OFF_B3],
         "mean": p[OFF_MEAN:OFF_ISTD], "istd": p[OFF_ISTD:OFF_OSCALE], "oscale": p[OFF_OSCALE],
+        "alpha": p[OFF_ALPHA], "pcap": p[OFF_PCAP], "resid": p[OFF_RESID],
     }
 
 
@@ -83,9 +88,16 @@ def forward(params, x, emulate=True):
 
 
 def predict(params, x, emulate=True):
-    """Network output in target units (y * out_scale)."""
+    """Prediction in target units: y * out_scale + resid * x[PPREV]."""
     y, c = forward(params, x, emulate)
-    return y * c["P"]["oscale"]
+    p = np.asarray(params, dtype=np.float64)
+    return y * c["P"]["oscale"] + p[OFF_RESID] * np.asarray(x, dtype=np.float64)[:, F_PPREV]
+
+
+def warm_pressure(params, x, emulate=True):
+    """What the solver starts from, in target units: alpha * clamp(pred, 0, pcap)."""
+    p = np.asarray(params, dtype=np.float64)
+    return p[OFF_ALPHA] * np.clip(predict(params, x, emulate), 0.0, p[OFF_PCAP])
 
 
 def loss_and_grad(params, x, t, emulate=True):

@@ -1115,6 +1115,9 @@ static void sph_int_range(void *vctx, int i0, int i1) {
     for (int i = i0; i < i1; i++) integrate_one(s, i, s->arg_dt);
 }
 
+/* force phase of ComputeAllForcesFor alone (two-phase form), for tests */
+void bbo_stage_sph_force(bbo_sim *s) { parallel_for(s->threads, 0, s->n, sph_force_range, s); }
+
 /* AdvanceTimeStep(SphSolver3*) (src/solvers/sph_solver3.cpp:48-67) */
 void bbo_sph_substep(bbo_sim *s, double dt, int sph_mode, bbo_stats *st) {
     bbo_stats local;

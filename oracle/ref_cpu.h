@@ -148,6 +148,7 @@ void bbo_stage_predict_pressure(bbo_sim *s, double delta, int record_error);
 void bbo_stage_pressure_force(bbo_sim *s);
 void bbo_stage_accumulate_integrate(bbo_sim *s, double dt);
 void bbo_stage_xsph(bbo_sim *s, double dt);
+void bbo_stage_sph_force(bbo_sim *s);                       /* ComputeAllForcesFor, force phase */
 double bbo_compute_delta(const bbo_sim *s, double dt);
 unsigned bbo_number_of_time_steps(const bbo_sim *s, double interval, double scale);
 

@@ -105,6 +105,7 @@ def lib():
         L.bbo_stage_pressure_force.argtypes = [vp]
         L.bbo_stage_accumulate_integrate.argtypes = [vp, d]
         L.bbo_stage_xsph.argtypes = [vp, d]
+        L.bbo_stage_sph_force.argtypes = [vp]
         L.bbo_compute_delta.restype = d
         L.bbo_compute_delta.argtypes = [vp, d]
         L.bbo_number_of_time_steps.restype = u

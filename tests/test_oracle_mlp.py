@@ -19,6 +19,9 @@ def random_params(seed=0):
     p[M.OFF_MEAN:M.OFF_ISTD] = rng.normal(scale=0.5, size=M.F)
     p[M.OFF_ISTD:M.OFF_OSCALE] = rng.uniform(0.5, 2.0, size=M.F)
     p[M.OFF_OSCALE] = 1.7
+    p[M.OFF_ALPHA] = 0.9
+    p[M.OFF_PCAP] = 2.5
+    p[M.OFF_RESID] = 1.0
     return p
 
 

@@ -325,3 +325,158 @@ def test_threads_do_not_change_results(oracle):
         b.pcisph_substep(sc.dt, 6, 0.01, oracle.MODE_TEXTBOOK)
     assert np.array_equal(a.get("position"), b.get("position"))
     assert np.array_equal(a.chains()[1], b.chains()[1])
+
+
+def _spiky_d2w(h, r):
+    return 90.0 / (math.pi * h ** 5) * (1.0 - r / h)
+
+
+def _spiky_grad(h, r):
+    return 45.0 / (math.pi * h ** 4) * (1.0 - r / h) ** 2
+
+
+def test_nonpressure_viscosity_pair_closed_form(oracle):
+    """f_i = m g - c_d v_i + mu m^2 (v_j - v_i) d2W(r) / rho_j for two particles."""
+    r = 0.013
+    sc = with_points(base_scene(), [[0.0, 0.0, 0.0], [r, 0.0, 0.0]], [[0.0, 0.0, 0.0], [0.0, 0.0, 0.5]])
+    s = oracle.Sim(sc, gravity=(0.0, -2.0, 0.0), drag_coefficient=1e-4, viscosity=0.05)
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    s.stage("nonpressure")
+    inf = s.info()
+    h, m = inf["kernel_radius"], inf["mass"]
+    rho = s.get("density")
+    f = s.get("force")
+    visc = 0.05 * m * m * 0.5 * _spiky_d2w(h, r)
+    assert f[0, 1] == pytest.approx(-2.0 * m, rel=1e-14)
+    assert f[0, 2] == pytest.approx(visc / rho[1], rel=1e-12)
+    assert f[1, 2] == pytest.approx(-visc / rho[0] - 1e-4 * 0.5, rel=1e-12)
+    assert f[0, 0] == 0.0 and f[1, 0] == 0.0
+
+
+def test_sph_force_momentum_at_rest(oracle):
+    """WCSPH force phase with v = 0: the pressure terms cancel pairwise, so
+    sum_i f_i = N m g (buckets are symmetric below the 100 cap)."""
+    sc = scenes.sph_dam_break()
+    sc = dataclasses.replace(sc, positions=sc.positions[:4096].copy(),
+                             velocities=np.zeros((4096, 3)))
+    s = oracle.Sim(sc, negative_pressure_scale=1.0)  # keep the (negative) Tait pressures
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    assert np.abs(s.get("pressure")).max() > 0
+    s.stage("sph_force")
+    f = s.get("force")
+    m = s.info()["mass"]
+    total = f.sum(axis=0)
+    scale = np.abs(f).sum()
+    assert abs(total[0]) <= 1e-12 * scale and abs(total[2]) <= 1e-12 * scale
+    assert total[1] == pytest.approx(sc.n * m * GY, rel=1e-9)
+
+
+def test_sph_force_pair_closed_form(oracle):
+    """Two particles: f_i = m g - m^2 (p_i/rho_i^2 + p_j/rho_j^2) |dW(r)| (x_j - x_i)/r."""
+    r = 0.021
+    sc = with_points(scenes.sph_dam_break(), [[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    s = oracle.Sim(sc, gravity=(0.0, 0.0, 0.0))
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 1)
+    s.set("pressure", np.array([300.0, 120.0]))
+    s.stage("sph_force")
+    inf = s.info()
+    h, m = inf["kernel_radius"], inf["mass"]
+    rho = s.get("density")
+    f = s.get("force")
+    mag = m * m * (300.0 / rho[0] ** 2 + 120.0 / rho[1] ** 2) * _spiky_grad(h, r)
+    assert f[0, 0] == pytest.approx(-mag, rel=1e-12)
+    assert f[1, 0] == pytest.approx(mag, rel=1e-12)
+
+
+class F:
+    """Feature indices, read from include/bbx_features.h."""
+
+
+def _load_feature_indices():
+    import os
+    import re
+
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "include", "bbx_features.h")
+    with open(path) as fh:
+        for name, val in re.findall(r"BBX_F_([A-Z0-9_]+)\s*=\s*(\d+)", fh.read()):
+            setattr(F, name, int(val))
+
+
+_load_feature_indices()
+
+
+def _features(s, dt):
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 0)
+    s.stage("nonpressure")
+    return s.features(dt)
+
+
+def test_features_isolated_particle(oracle):
+    """One particle: its bucket holds only itself, so moments, centroid and
+    divergence vanish; the kinematic features follow from v, f and the walls."""
+    sc = base_scene()
+    p = [0.05, -0.1, 0.02]
+    v = [0.1, -0.2, 0.3]
+    s = oracle.Sim(with_points(sc, [p], [v]), drag_coefficient=1e-4)
+    s.set("pressure", np.array([7.0]))
+    dt = 5e-4
+    x = _features(s, dt)[0]
+    inf = s.info()
+    h, m = inf["kernel_radius"], inf["mass"]
+    rho = m * 315.0 / (64.0 * math.pi * h ** 3)
+    assert x[F.COUNT] == 1.0
+    for k in (F.M1, F.M2, F.M3, F.CX, F.CY, F.CZ, F.CNORM, F.DIVV):
+        assert x[k] == 0.0
+    assert x[F.DENSITY_ERR] == pytest.approx(rho / 1000.0 - 1.0, rel=1e-12)
+    assert x[F.PPREV] == pytest.approx(7.0 / (s.delta(dt) * 1000.0), rel=1e-12)
+    assert x[F.VY] == pytest.approx(dt * -0.2 / h, rel=1e-14)
+    assert x[F.VNORM] == pytest.approx(dt * math.sqrt(0.14) / h, rel=1e-14)
+    fy = m * GY - 1e-4 * -0.2
+    fn = math.sqrt((1e-4 * 0.1) ** 2 + fy ** 2 + (1e-4 * 0.3) ** 2)
+    assert x[F.AY] == pytest.approx(dt * dt * fy / (m * h), rel=1e-12)
+    assert x[F.ANORM] == pytest.approx(dt * dt * fn / (m * h), rel=1e-12)
+    half = np.asarray(sc.tank_size) / 2
+    wall = min(min(pi + hi, hi - pi) for pi, hi in zip(p, half))
+    assert x[F.WALL] == pytest.approx(min(wall / h, 4.0), rel=1e-12)
+
+
+def test_features_perfect_lattice_interior(oracle):
+    """Interior particle of a perfect simple-cubic lattice with h = 2 s: the
+    bucket holds the 27 offsets with i^2 + j^2 + k^2 <= 3 (distance 2s is on
+    the cutoff and excluded), the centroid vanishes by symmetry, and the
+    distance moments are lattice sums."""
+    sc = scenes.dam_break(12, 12, 12, spacing=0.01, tank=(0.4, 0.4, 0.4), seed=0)
+    pos = scenes.lattice_block((12, 12, 12), 0.01, (-0.06, -0.06, -0.06), 0, jitter=0.0)
+    s = oracle.Sim(with_points(sc, pos))
+    x = _features(s, 5e-4)
+    i = 6 + 12 * (6 + 12 * 6)
+    u = np.array([1.0] * 6 + [math.sqrt(2)] * 12 + [math.sqrt(3)] * 8) / 2.0  # r / h
+    assert x[i, F.COUNT] == 27.0
+    for k, pw in ((F.M1, 1), (F.M2, 2), (F.M3, 3)):
+        assert x[i, k] == pytest.approx((u ** pw).sum() / 26.0, rel=1e-12)
+    assert abs(x[i, F.CX]) < 1e-12 and abs(x[i, F.CY]) < 1e-12 and abs(x[i, F.CZ]) < 1e-12
+    assert x[i, F.DIVV] == 0.0
+
+
+def test_features_divergence_sign_and_linearity(oracle):
+    """An expanding field v = a x has positive SPH divergence, a compressing
+    one negative, and the estimate is linear in a."""
+    sc = scenes.dam_break(12, 12, 12, spacing=0.01, tank=(0.4, 0.4, 0.4), seed=0)
+    pos = scenes.lattice_block((12, 12, 12), 0.01, (-0.06, -0.06, -0.06), 0, jitter=0.0)
+    i = 6 + 12 * (6 + 12 * 6)
+    out = {}
+    for a in (0.5, 1.0, -1.0):
+        s = oracle.Sim(with_points(sc, pos, a * pos), gravity=(0.0, 0.0, 0.0))
+        out[a] = _features(s, 5e-4)[i, F.DIVV]
+    assert out[1.0] > 0 > out[-1.0]
+    assert out[0.5] == pytest.approx(0.5 * out[1.0], rel=1e-12)
+    assert out[-1.0] == pytest.approx(-out[1.0], rel=1e-12)
