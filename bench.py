@@ -1,17 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: PCISPH particle-substeps/s on one MI355X.
+"""Benchmark: PCISPH particle-substeps/s on one MI355X, plain vs learned warm start.
 
 Workload: the 1,048,576-particle PCISPH dam break (128x64x128 simple-cubic
 block at spacing 0.01, kernel radius 0.02, grid cell = kernel radius, 2x1x2
 tank, dt 5e-4 fixed, TEXTBOOK predict-correct loop to <= 1% max density error,
 at most 8 iterations).  One step = one substep of every particle, including
-all correction iterations; value = particles * steps / wall time, inputs
-already resident in HBM.
+all correction iterations; inputs are resident in HBM before timing starts.
 
-Prints one JSON line (see the keys below).  The dominant kernel's roofline is
-measured live with HIP events on the solver's stream over the timed region.
-The CPU baseline times the FP64 oracle (a restatement of the reference CPU
-path) on a bounded sample of the same workload.
+The same scene runs twice: plain PCISPH (pressure starts at 0) and PCISPH
+started from the learned warm-start network (bubbles_amd/data).  `value` is
+the learned run's particle-steps/s; the plain run's figures and the speed-up
+sit beside it in `config`.  A second pair with an iteration cap of 64
+("to_tolerance") shows the same comparison when both runs iterate until the
+tolerance is met.
+
+Timing: W warm-up substeps (the last ones profiled per kernel with HIP events
+to pick the dominant kernel), then exactly K timed substeps bracketed by
+device synchronisations.  Inside the timed region only one launch per substep
+of the dominant kernel is bracketed by events (on the solver's stream), so
+the wall-clock value is not inflated by per-kernel events.  The CPU baseline
+times the FP64 oracle (a restatement of the reference CPU path) on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -27,20 +36,23 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s @1M PCISPH, plain vs learned warm-start, ≤1% density err"
 PEAK_HBM_GBS = 8000.0
 
-# Compulsory FP32 bytes per particle of each kernel (xyz = 12 B, scalar = 4 B),
-# excluding neighbour re-reads and the neighbour table itself.
+# Compulsory FP32 bytes per particle of each kernel (xyz = 12 B, scalar = 4 B):
+# the particle's own fields read and written once, excluding neighbour
+# re-reads and the neighbour table.
 KERNEL_BYTES = {
-    "grid_hash": 36,                  # x, old cell -> new cell, key, rank, hash
-    "grid_scatter": 16,               # new cell, rank, start -> slot index
-    "grid_rank_move": 76,             # slot, x+id, v+p -> x+id, v+p, cell, home
-    "buckets_density": 16,            # x -> rho
-    "nonpressure_predict": 52,        # x, v, rho -> f, x*
-    "nonpressure_features": 104,      # x, v, rho -> f, 16 features
-    "predict_pressure": 28,           # x*, p -> p, rho*, q
-    "pressure_force_predict": 52,     # x, q, v, f -> x*
-    "pressure_force_integrate": 92,   # x, q, v, f, p -> x, v, f, f_p
-    "mlp_infer": 68,                  # 16 features -> p
+    "grid_hash": 36,                   # x, old cell -> new cell, key, rank, hash
+    "grid_scatter": 16,                # new cell, rank, start -> slot index
+    "grid_rank_move": 76,              # slot, x+id, v+p -> x+id, v+p, cell, home
+    "buckets_density": 16,             # x -> rho
+    "nonpressure_predict": 56,         # x, v, rho -> f, x*, rho*
+    "nonpressure_features": 108,       # x, v, rho -> f, rho*, 16 features
+    "mlp_infer": 68,                   # 16 features -> p
+    "predict_pressure": 28,            # x*, p -> p, rho*, q
+    "pressure_force_warm": 52,         # x, q, v, f -> x*
+    "pressure_force_predict": 52,      # x, q, v, f -> x*
+    "pressure_force_integrate": 96,    # x, q, v, f, p -> x, v+p, f, f_p
 }
+LOOP_KERNELS = ("predict_pressure", "pressure_force")
 
 
 def substep_bytes(k: float) -> float:
@@ -49,61 +61,110 @@ def substep_bytes(k: float) -> float:
     return 208.0 + 116.0 * k
 
 
-def run_gpu(args):
+def base_kernel(name: str):
+    """(base kernel, iteration filter) that selects launches of `name`."""
+    for b in LOOP_KERNELS:
+        if name.startswith(b):
+            role = name[len(b):]
+            if role == "_warm":
+                return b, -1
+            return b, 0 if role in ("", "_predict") else None
+    return name, None
+
+
+def load_traffic(kernel: str, n: int):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    kernel (FETCH_SIZE + WRITE_SIZE, scaled by the stream-copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if not k or rec.get("n") != n:
+        return None
+    return k.get("bytes_per_launch")
+
+
+def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
     import numpy as np
 
-    from bubbles_amd import abi, scenes
+    from bubbles_amd import abi
 
-    sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
     mode = abi.MODE_TEXTBOOK if args.mode == "textbook" else abi.MODE_REFERENCE_EXACT
     solver = abi.Solver(sc)
     warm = abi.WARM_NONE
-    if args.warm == "mlp":
+    if warm_mlp:
         from bubbles_amd import learned
 
-        mlp = learned.load_or_train(solver, sc)
-        solver.attach_mlp(mlp)
+        solver.attach_mlp(learned.load())
         warm = abi.WARM_MLP
-    for _ in range(args.warmup):
-        solver.pcisph_substep(sc.dt, args.max_it, args.tol, mode, warm)
-    solver.synchronize()
-    solver.profile(True)
-    solver.profile_reset()
+    n_prof = min(args.warmup, 20) if profile else 0
+    for _ in range(args.warmup - n_prof):
+        solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
+    table, dominant = {}, None
+    if n_prof:
+        solver.profile_select(None)
+        solver.profile(True)
+        solver.synchronize()
+        solver.profile_reset()
+        for _ in range(n_prof):
+            solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
+        solver.synchronize()
+        prof = solver.profile_get()
+        solver.profile(False)
+        table = {k: {"avg_ms": round(v["total_ms"] / v["launches"], 5),
+                     "launches_per_step": round(v["launches"] / n_prof, 3),
+                     "ms_per_step": round(v["total_ms"] / n_prof, 4)}
+                 for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
+        dominant = next(k for k in table if not k.endswith("_skipped"))
+        base, it = base_kernel(dominant)
+        solver.profile_select(base, it)
+        solver.profile(True)
+        solver.profile_reset()
     iters, errs = [], []
+    solver.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = solver.pcisph_substep(sc.dt, args.max_it, args.tol, mode, warm)
+        st = solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
         iters.append(st.iterations)
         errs.append(st.max_density_error)
     solver.synchronize()
     t1 = time.perf_counter()
-    prof = solver.profile_get()
+    live = solver.profile_get() if dominant else {}
     solver.profile(False)
-    elapsed = t1 - t0
-    n = sc.n
-    value = n * args.steps / elapsed
-    kbar = float(np.mean(iters))
-    # dominant kernel
-    name, rec = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
-    avg_ms = rec["total_ms"] / max(rec["launches"], 1)
-    per_unit = KERNEL_BYTES.get(name, 0)
-    achieved = per_unit * n / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": None, "avg_ms": round(avg_ms, 5), "bytes_per_particle": per_unit,
-                "units_per_launch": n}
-    step_bytes = substep_bytes(kbar)
-    whole = {"bytes_per_particle_substep": step_bytes,
-             "achieved_GBs": round(step_bytes * value / 1e9, 1),
-             "frac": round(step_bytes * value / 1e9 / PEAK_HBM_GBS, 4)}
-    kernels = {k: {"avg_ms": round(v["total_ms"] / max(v["launches"], 1), 5),
-                   "launches": v["launches"], "total_ms": round(v["total_ms"], 3)}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
     solver.close()
-    gbs, ms = abi.stream_copy(2.0 ** 30, 20)
-    roofline["measured_stream_copy_GBs"] = round(gbs, 1)
-    roofline["frac_of_measured"] = round(achieved / gbs, 4) if gbs > 0 else None
-    return sc, value, elapsed, kbar, float(np.max(errs)), roofline, whole, kernels
+    elapsed = t1 - t0
+    errs = np.asarray(errs)
+    out = {"value": sc.n * args.steps / elapsed, "elapsed": elapsed,
+           "mean_iterations": float(np.mean(iters)), "max_iterations": int(np.max(iters)),
+           "max_density_error": float(errs.max()),
+           "converged_fraction": float(np.mean(errs < args.tol)),
+           "kernels": table, "dominant": dominant}
+    if dominant:
+        rec = live.get(dominant)
+        out["dominant_live"] = (rec["total_ms"] / rec["launches"], rec["launches"]) if rec else None
+    return out
+
+
+def roofline_for(res, n, stream_gbs):
+    name = res["dominant"]
+    if name is None:
+        return None
+    avg_ms, launches = res["dominant_live"] or (res["kernels"][name]["avg_ms"], 0)
+    per_unit = KERNEL_BYTES.get(name)
+    achieved = per_unit * n / (avg_ms * 1e-3) / 1e9 if per_unit else None
+    traffic = load_traffic(name, n)
+    return {"bound": "hbm", "kernel": name,
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
+            "traffic": traffic,
+            "avg_ms": round(avg_ms, 5), "timed_launches": launches,
+            "bytes_per_particle": per_unit, "units_per_launch": n,
+            "measured_stream_copy_GBs": round(stream_gbs, 1),
+            "frac_of_measured_copy": round(achieved / stream_gbs, 4) if achieved and stream_gbs else None}
 
 
 def run_cpu_baseline(sc, args):
@@ -130,19 +191,32 @@ def run_cpu_baseline(sc, args):
             "cpu_model": cpu_model,
             "kind": "port",
             "sample": f"{done} timed substeps (after 1 warm-up) of the same {sc.n}-particle dam "
-                      f"break, FP64 oracle with linked-list cells and 100-slot buckets, "
-                      f"{args.mode} loop, {threads} threads, {t:.1f} s"}
+                      f"break, plain PCISPH, FP64 oracle with linked-list cells and 100-slot "
+                      f"buckets, {args.mode} loop, max {args.max_it} iterations, {threads} "
+                      f"threads, {t:.1f} s"}
+
+
+def summary(res):
+    return {"particle_steps_per_s": round(res["value"], 1),
+            "ms_per_step": round(res["elapsed"] / max(1, res.get("steps", 1)) * 1e3, 4),
+            "mean_iterations": round(res["mean_iterations"], 3),
+            "max_iterations": res["max_iterations"],
+            "max_density_error": round(res["max_density_error"], 6),
+            "converged_fraction": round(res["converged_fraction"], 4)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=480)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--mode", choices=["textbook", "exact"], default="textbook")
-    ap.add_argument("--warm", choices=["none", "mlp"], default="none")
+    ap.add_argument("--warm", choices=["both", "none", "mlp"], default="both",
+                    help="both: learned run is `value`, plain run beside it")
     ap.add_argument("--max-it", type=int, default=8)
     ap.add_argument("--tol", type=float, default=0.01)
+    ap.add_argument("--tol-max-it", type=int, default=64,
+                    help="iteration cap of the to-tolerance pair (0: skip)")
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=128)
@@ -154,30 +228,63 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the PCISPH path "
               f"here has no multi-GPU mode", file=sys.stderr)
         sys.exit(2)
-    sc, value, elapsed, kbar, maxerr, roofline, whole, kernels = run_gpu(args)
+
+    from bubbles_amd import abi, scenes
+
+    sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+    runs = {}
+    if args.warm in ("both", "none"):
+        runs["plain"] = run_variant(args, sc, False, args.max_it, profile=args.warm == "none")
+    if args.warm in ("both", "mlp"):
+        runs["learned"] = run_variant(args, sc, True, args.max_it, profile=True)
+    for r in runs.values():
+        r["steps"] = args.steps
+    head = runs["learned"] if "learned" in runs else runs["plain"]
+    tol_pair = None
+    if args.tol_max_it and args.warm == "both" and args.mode == "textbook":
+        tp = run_variant(args, sc, False, args.tol_max_it, profile=False)
+        tl = run_variant(args, sc, True, args.tol_max_it, profile=False)
+        tp["steps"] = tl["steps"] = args.steps
+        tol_pair = {"max_it": args.tol_max_it, "plain": summary(tp), "learned": summary(tl),
+                    "speedup": round(tl["value"] / tp["value"], 4)}
+    gbs, _ = abi.stream_copy(2.0 ** 30, 20)
+    roof = roofline_for(head, sc.n, gbs)
+    kbar = head["mean_iterations"]
+    step_bytes = substep_bytes(kbar)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(sc, args)
+    cfg = {"workload": f"PCISPH dam break {sc.n} particles ({args.nx}x{args.ny}x{args.nz} "
+                       f"at 0.01, h 0.02, 2x1x2 tank), dt 5e-4",
+           "n": sc.n, "mode": args.mode, "warm_start": "learned" if "learned" in runs else "none",
+           "max_it": args.max_it, "tol": args.tol,
+           "mean_iterations": round(kbar, 3),
+           "max_density_error": round(head["max_density_error"], 6),
+           "converged_fraction": round(head["converged_fraction"], 4)}
+    if "plain" in runs and "learned" in runs:
+        cfg["plain"] = summary(runs["plain"])
+        cfg["learned"] = summary(runs["learned"])
+        cfg["speedup_learned_vs_plain"] = round(runs["learned"]["value"] / runs["plain"]["value"], 4)
+    if tol_pair:
+        cfg["to_tolerance"] = tol_pair
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(head["value"], 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": round(head["elapsed"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"PCISPH dam break {sc.n} particles ({args.nx}x{args.ny}x{args.nz} "
-                               f"at 0.01, h 0.02, 2x1x2 tank), dt 5e-4",
-                   "n": sc.n, "mode": args.mode, "warm_start": args.warm, "max_it": args.max_it,
-                   "tol": args.tol, "mean_iterations": round(kbar, 3),
-                   "max_density_error": round(maxerr, 6)},
-        "roofline": roofline,
-        "substep_roofline": whole,
+        "config": cfg,
+        "roofline": roof,
+        "substep_roofline": {"bytes_per_particle_substep": round(step_bytes, 1),
+                             "achieved_GBs": round(step_bytes * head["value"] / 1e9, 1),
+                             "frac": round(step_bytes * head["value"] / 1e9 / PEAK_HBM_GBS, 4)},
         "cpu_baseline": cpu,
-        "kernels": kernels,
+        "kernels": head["kernels"],
     }
     print(json.dumps(out))
 

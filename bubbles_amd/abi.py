@@ -21,6 +21,7 @@ OK = 0
 PCISPH, SPH = 0, 1
 MODE_REFERENCE_EXACT, MODE_TEXTBOOK = 0, 1
 WARM_NONE, WARM_MLP = 0, 1
+PROFILE_ALL_LAUNCHES = -1000
 BUCKET = 100
 NFEAT = 16
 
@@ -125,6 +126,7 @@ def load():
         "bbx_pcisph_harvest": (i, [vp, d, i, d, fp, fp, C.POINTER(Stats)]),
         "bbx_profile_enable": (i, [vp, i]),
         "bbx_profile_reset": (i, [vp]),
+        "bbx_profile_select": (i, [vp, C.c_char_p, i]),
         "bbx_profile_get": (i, [vp, C.c_char_p, i, dp, ip, i, ip]),
         "bbx_solver_synchronize": (i, [vp]),
         "bbx_stream_copy": (i, [d, i, dp, dp]),
@@ -324,6 +326,7 @@ class Solver:
 
     def attach_mlp(self, mlp):
         check(load().bbx_solver_attach_mlp(self.h, mlp.h if mlp else None), "bbx_solver_attach_mlp")
+        self._mlp = mlp  # the network object must outlive its use by this solver
 
     def harvest(self, dt, max_it=8, tol=0.01):
         feats = np.zeros((self.n, NFEAT), dtype=np.float32)
@@ -335,6 +338,13 @@ class Solver:
 
     def profile(self, enable=True):
         check(load().bbx_profile_enable(self.h, 1 if enable else 0), "bbx_profile_enable")
+
+    def profile_select(self, kernel=None, iteration=None):
+        """Time only `kernel` (None: all) and only its launch of `iteration`
+        (None: every launch; -1: the warm-start launch)."""
+        it = PROFILE_ALL_LAUNCHES if iteration is None else int(iteration)
+        check(load().bbx_profile_select(self.h, kernel.encode() if kernel else None, it),
+              "bbx_profile_select")
 
     def profile_reset(self):
         check(load().bbx_profile_reset(self.h), "bbx_profile_reset")
@@ -348,7 +358,8 @@ class Solver:
         check(load().bbx_profile_get(self.h, names, 8192, _dp(ms), _ip(cnt), cap, C.byref(k)),
               "bbx_profile_get")
         nm = names.value.decode().split("\n")[: k.value]
-        return {n: {"total_ms": float(ms[i]), "launches": int(cnt[i])} for i, n in enumerate(nm)}
+        return {n: {"total_ms": float(ms[i]), "launches": int(cnt[i])}
+                for i, n in enumerate(nm) if cnt[i] > 0}
 
     def synchronize(self):
         check(load().bbx_solver_synchronize(self.h), "bbx_solver_synchronize")

@@ -24,13 +24,19 @@ size_t scan_scratch_elems(size_t n);
 // ---------------------------------------------------------------------------
 // profiler
 // ---------------------------------------------------------------------------
-int Profiler::name_id(const char *name) {
+int Profiler::name_id(const std::string &name) {
     for (size_t i = 0; i < names.size(); i++)
         if (names[i] == name) return (int)i;
     names.emplace_back(name);
     total_ms.push_back(0.0);
     launches.push_back(0);
     return (int)names.size() - 1;
+}
+
+bool Profiler::wants(const char *name, int iter) const {
+    if (!only.empty() && only != name) return false;
+    if (only_iter != kNoIter && iter != only_iter) return false;
+    return true;
 }
 
 static hipEvent_t pool_get(std::vector<hipEvent_t> &pool) {
@@ -44,24 +50,37 @@ static hipEvent_t pool_get(std::vector<hipEvent_t> &pool) {
     return e;
 }
 
-void Profiler::begin(hipStream_t s, const char *, hipEvent_t *a) {
+void Profiler::begin(hipStream_t s, hipEvent_t *a) {
     *a = pool_get(pool);
     if (*a) (void)hipEventRecord(*a, s);
 }
 
-void Profiler::end(hipStream_t s, int id, hipEvent_t a) {
+void Profiler::end(hipStream_t s, int id, int iter, hipEvent_t a) {
     hipEvent_t b = pool_get(pool);
     if (!a || !b) return;
     (void)hipEventRecord(b, s);
-    pending.push_back({id, a, b});
+    pending.push_back({id, iter, a, b});
 }
 
-int Profiler::resolve() {
+static std::string role_name(const std::string &base, int iter, int iterations) {
+    if (iter == kNoIter || iterations < 0) return base;
+    if (base == "predict_pressure") return iter < iterations ? base : base + "_skipped";
+    if (base == "pressure_force") {
+        if (iter < 0) return base + "_warm";
+        if (iter < iterations - 1) return base + "_predict";
+        if (iter == iterations - 1) return base + "_integrate";
+        return base + "_skipped";
+    }
+    return base;
+}
+
+int Profiler::resolve(int iterations) {
     for (auto &p : pending) {
         float ms = 0.f;
         if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-            total_ms[p.id] += ms;
-            launches[p.id] += 1;
+            const int id = name_id(role_name(names[p.id], p.iter, iterations));
+            total_ms[id] += ms;
+            launches[id] += 1;
         }
         pool.push_back(p.a);
         pool.push_back(p.b);
@@ -547,6 +566,8 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
 int bbx_solver_destroy(bbx_solver *s) {
     if (!s) return BBX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    mlp_release(s->mlp);
+    s->mlp = nullptr;
     free_all(s);
     delete s;
     return BBX_OK;
@@ -987,8 +1008,12 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     st.delta = (float)delta;
     if (warm == BBX_WARM_MLP) {
         BBX_CHECK(run_nonpressure(s, st, 0, 1));
-        BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride,
-                                   (float)(delta * s->prm.target_density), s->pwarm, s->stream));
+        {
+            BBX_PROF_BEGIN("mlp_infer");
+            BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride,
+                                       (float)(delta * s->prm.target_density), s->pwarm, s->stream));
+            BBX_PROF_END();
+        }
         BBX_CHECK(run_warm_init(s));
         // f_p(p_warm, rho) and the k = 0 prediction
         BBX_CHECK(run_pressure_force(s, st, -1, max_it, 0, tol));
@@ -1017,7 +1042,7 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
     local.gpu_ms = ms;
     if (stats) *stats = local;
-    if (s->prof.enabled) s->prof.resolve();
+    if (s->prof.enabled) s->prof.resolve(local.iterations);
     return BBX_OK;
 }
 
@@ -1085,7 +1110,12 @@ int bbx_sph_advance(bbx_solver *s, double interval, int *substeps) {
 
 int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m) {
     BBX_CHECK(check_solver(s));
+    if (s->mlp == m) return BBX_OK;
+    // the solver's queued work may still read the old network
+    if (s->mlp) BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    mlp_release(s->mlp);
     s->mlp = m;
+    mlp_retain(m);
     return BBX_OK;
 }
 
@@ -1129,6 +1159,14 @@ int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *
 int bbx_profile_enable(bbx_solver *s, int enable) {
     BBX_CHECK(check_solver(s));
     s->prof.enabled = enable != 0;
+    return BBX_OK;
+}
+
+int bbx_profile_select(bbx_solver *s, const char *kernel, int iteration) {
+    BBX_CHECK(check_solver(s));
+    static_assert(BBX_PROFILE_ALL_LAUNCHES == kNoIter, "profile sentinel");
+    s->prof.only = kernel ? kernel : "";
+    s->prof.only_iter = iteration;
     return BBX_OK;
 }
 

@@ -104,23 +104,44 @@ enum {
 // ---------------------------------------------------------------------------
 // kernel profiler (HIP events on the solver stream)
 // ---------------------------------------------------------------------------
+constexpr int kNoIter = -1000;  // launch outside the predict-correct loop
+
 struct Profiler {
     bool enabled = false;
-    struct Pending { int id; hipEvent_t a, b; };
+    std::string only;        // time only this kernel (empty: every kernel)
+    int only_iter = kNoIter; // and only its launch of this iteration (kNoIter: all)
+    struct Pending { int id, iter; hipEvent_t a, b; };
     std::vector<std::string> names;
     std::vector<double> total_ms;
     std::vector<int> launches;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
-    int name_id(const char *name);
-    void begin(hipStream_t s, const char *name, hipEvent_t *a);
-    void end(hipStream_t s, int id, hipEvent_t a);
-    int resolve();
+    int name_id(const std::string &name);
+    bool wants(const char *name, int iter) const;
+    void begin(hipStream_t s, hipEvent_t *a);
+    void end(hipStream_t s, int id, int iter, hipEvent_t a);
+    // iterations >= 0 names the loop kernels by role: predict_pressure or
+    // predict_pressure_skipped; pressure_force_{warm,predict,integrate,skipped}
+    int resolve(int iterations = -1);
     void reset();
     void destroy();
 };
 
 }  // namespace bbx
+
+// Bracket one kernel launch with profiler events (needs `s`, a bbx_solver *).
+#define BBX_PROF_BEGIN_IT(name, it)                                          \
+    hipEvent_t ev_a_ = nullptr;                                              \
+    int ev_id_ = -1;                                                         \
+    const int ev_it_ = (it);                                                 \
+    const bool ev_on_ = s->prof.enabled && s->prof.wants(name, ev_it_);      \
+    if (ev_on_) {                                                            \
+        ev_id_ = s->prof.name_id(name);                                      \
+        s->prof.begin(s->stream, &ev_a_);                                    \
+    }
+#define BBX_PROF_BEGIN(name) BBX_PROF_BEGIN_IT(name, ::bbx::kNoIter)
+#define BBX_PROF_END() \
+    if (ev_on_) s->prof.end(s->stream, ev_id_, ev_it_, ev_a_);
 
 // ---------------------------------------------------------------------------
 // solver
@@ -205,4 +226,6 @@ int run_warm_init(bbx_solver *s);
 int swap_after_integrate(bbx_solver *s);
 int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float scale, float *out,
                      hipStream_t stream);
+void mlp_retain(bbx_mlp *m);
+void mlp_release(bbx_mlp *m);
 }  // namespace bbx

@@ -109,60 +109,75 @@ __device__ __forceinline__ bool inside_box(float px, float py, float pz, const f
     return rv;
 }
 
+// Written with scalars and selects only: a run-time index into a local
+// array (the chosen face) would put the array in scratch memory.
 __device__ __forceinline__ void collide(const SimConst &c, float e, float3 &p, float3 &v) {
-    const float lo[3] = {-c.tank_h[0], -c.tank_h[1], -c.tank_h[2]};
-    const float hi[3] = {c.tank_h[0], c.tank_h[1], c.tank_h[2]};
-    float l[3] = {p.x - c.tank_c[0], p.y - c.tank_c[1], p.z - c.tank_c[2]};
-    float cl[3];
+    const float hx = c.tank_h[0], hy = c.tank_h[1], hz = c.tank_h[2];
+    const float lo[3] = {-hx, -hy, -hz};
+    const float hi[3] = {hx, hy, hz};
+    const float lx = p.x - c.tank_c[0], ly = p.y - c.tank_c[1], lz = p.z - c.tank_c[2];
+    float clx, cly, clz;
     int axis = 0;
     float sgn = 1.f;  // outward normal of the chosen face: sgn * e_axis
     float dist;
-    bool in = inside_box(l[0], l[1], l[2], lo, hi);
-    if (in) {
+    if (inside_box(lx, ly, lz, lo, hi)) {
         // nearest of the six planes, order +x,+y,+z,-x,-y,-z, strict <
-        float best = (hi[0] - l[0]) * (hi[0] - l[0]);
+        const float d[6] = {hx - lx, hy - ly, hz - lz, -hx - lx, -hy - ly, -hz - lz};
+        float best = d[0] * d[0];
+#pragma unroll
         for (int f = 1; f < 6; f++) {
-            int a = f % 3;
-            float d = (f < 3) ? (hi[a] - l[a]) : (lo[a] - l[a]);
-            float d2 = d * d;
-            if (d2 < best) { best = d2; axis = a; sgn = (f < 3) ? 1.f : -1.f; }
+            const float d2 = d[f] * d[f];
+            if (d2 < best) {
+                best = d2;
+                axis = f % 3;
+                sgn = (f < 3) ? 1.f : -1.f;
+            }
         }
-        cl[0] = l[0]; cl[1] = l[1]; cl[2] = l[2];
-        cl[axis] = (sgn > 0) ? hi[axis] : lo[axis];
+        const float face = sgn > 0.f ? (axis == 0 ? hx : (axis == 1 ? hy : hz))
+                                     : (axis == 0 ? -hx : (axis == 1 ? -hy : -hz));
+        clx = axis == 0 ? face : lx;
+        cly = axis == 1 ? face : ly;
+        clz = axis == 2 ? face : lz;
         dist = -sqrtf(best);
     } else {
-        cl[0] = fminf(fmaxf(l[0], lo[0]), hi[0]);
-        cl[1] = fminf(fmaxf(l[1], lo[1]), hi[1]);
-        cl[2] = fminf(fmaxf(l[2], lo[2]), hi[2]);
-        float cov[3] = {l[0] - cl[0], l[1] - cl[1], l[2] - cl[2]};
+        clx = fminf(fmaxf(lx, -hx), hx);
+        cly = fminf(fmaxf(ly, -hy), hy);
+        clz = fminf(fmaxf(lz, -hz), hz);
+        const float cov[3] = {lx - clx, ly - cly, lz - clz};
         float maxc = cov[0];
+#pragma unroll
         for (int f = 1; f < 6; f++) {
-            int a = f % 3;
-            float cs = (f < 3) ? cov[a] : -cov[a];
-            if (cs > maxc) { maxc = cs; axis = a; sgn = (f < 3) ? 1.f : -1.f; }
+            const float cs = (f < 3) ? cov[f % 3] : -cov[f % 3];
+            if (cs > maxc) {
+                maxc = cs;
+                axis = f % 3;
+                sgn = (f < 3) ? 1.f : -1.f;
+            }
         }
-        float dx = cl[0] - l[0], dy = cl[1] - l[1], dz = cl[2] - l[2];
+        const float dx = clx - lx, dy = cly - ly, dz = clz - lz;
         dist = sqrtf(dx * dx + dy * dy + dz * dz);
     }
     // reversed orientation: the normal points into the tank
-    float n[3] = {0.f, 0.f, 0.f};
-    n[axis] = -sgn;
+    const float nx = axis == 0 ? -sgn : 0.f;
+    const float ny = axis == 1 ? -sgn : 0.f;
+    const float nz = axis == 2 ? -sgn : 0.f;
     // IsInside(reversed) = !(ClosestDistance < 0); penetrating if that or |d| < r
-    bool is_inside = !(dist < 0.f);
+    const bool is_inside = !(dist < 0.f);
     if (!(is_inside || fabsf(dist) < c.radius)) return;
-    float tp[3] = {cl[0] + c.tank_c[0] + c.radius * n[0], cl[1] + c.tank_c[1] + c.radius * n[1],
-                   cl[2] + c.tank_c[2] + c.radius * n[2]};
-    float ndr = n[0] * v.x + n[1] * v.y + n[2] * v.z;
+    const float tx = clx + c.tank_c[0] + c.radius * nx;
+    const float ty = cly + c.tank_c[1] + c.radius * ny;
+    const float tz = clz + c.tank_c[2] + c.radius * nz;
+    const float ndr = nx * v.x + ny * v.y + nz * v.z;
     if (ndr < 0.f) {
         // relN = ndr * n -> -e * relN; relT unchanged (friction 0)
-        float rn = ndr * (-e) - ndr;   // change along n
-        v.x += rn * n[0];
-        v.y += rn * n[1];
-        v.z += rn * n[2];
+        const float rn = ndr * (-e) - ndr;  // change along n
+        v.x += rn * nx;
+        v.y += rn * ny;
+        v.z += rn * nz;
     }
-    p.x = tp[0];
-    p.y = tp[1];
-    p.z = tp[2];
+    p.x = tx;
+    p.y = ty;
+    p.z = tz;
 }
 
 // wave64 reductions

@@ -309,18 +309,27 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     for (int q = 0; q < 27; q++) {
         if (q >= ncell) break;
         const int b = cb[q], e = ce[q], sh = shift[q];
-        for (int j = b; j < e; j++) {
-            const int jl = j + sh;
-            const float4 pj = staged ? sx[jl] : pos[j];
-            if (!in_bucket(c, pi, pj)) continue;
-            if (cnt < kBucket) {
-                if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)jl;
-                else nbr32[(size_t)cnt * c.stride + i] = j;
-                cnt++;
-                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-            } else {
-                dropped++;
+        // candidates in chain order, four loads issued before the tests
+        for (int j0 = b; j0 < e; j0 += 4) {
+            float4 pc[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (j0 + u < e) pc[u] = staged ? sx[j0 + u + sh] : pos[j0 + u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = j0 + u;
+                if (j >= e) break;
+                const float4 pj = pc[u];
+                if (!in_bucket(c, pi, pj)) continue;
+                if (cnt < kBucket) {
+                    if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)(j + sh);
+                    else nbr32[(size_t)cnt * c.stride + i] = j;
+                    cnt++;
+                    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+                } else {
+                    dropped++;
+                }
             }
         }
     }
@@ -365,16 +374,6 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
 
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-#define PROF_BEGIN(name)                                                \
-    hipEvent_t ev_a_ = nullptr;                                         \
-    int ev_id_ = -1;                                                    \
-    if (s->prof.enabled) {                                              \
-        ev_id_ = s->prof.name_id(name);                                 \
-        s->prof.begin(s->stream, name, &ev_a_);                         \
-    }
-#define PROF_END()                                                      \
-    if (s->prof.enabled) s->prof.end(s->stream, ev_id_, ev_a_);
-
 // UpdateGridDistribution: full or incremental chain update, then buckets.
 int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure) {
     SimConst &c = s->sc;
@@ -383,44 +382,44 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
     BBX_CHECK(launch_fill_int(s, s->cell_count, 0, cells));
     if (n > 0) {
         {
-            PROF_BEGIN("grid_hash");
+            BBX_PROF_BEGIN("grid_hash");
             k_hash<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->posA, s->cellA, s->flag,
                                                          force_full ? 1 : 0, s->newc,
                                                          s->hash_tmp, s->key, s->cell_count,
                                                          s->rank_tmp, s->counters);
             BBX_HIP_CHECK(hipGetLastError());
-            PROF_END();
+            BBX_PROF_END();
         }
         {
-            PROF_BEGIN("grid_scan");
+            BBX_PROF_BEGIN("grid_scan");
             BBX_CHECK(launch_scan(s, s->cell_count, s->cell_start, cells));
-            PROF_END();
+            BBX_PROF_END();
         }
         {
-            PROF_BEGIN("grid_scatter");
+            BBX_PROF_BEGIN("grid_scatter");
             k_scatter<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->newc, s->rank_tmp,
                                                             s->cell_start, s->cell_count,
                                                             s->tmp_idx, s->flag,
                                                             force_full ? 1 : 0, s->counters);
             BBX_HIP_CHECK(hipGetLastError());
-            PROF_END();
+            BBX_PROF_END();
         }
         {
-            PROF_BEGIN("grid_rank_move");
+            BBX_PROF_BEGIN("grid_rank_move");
             k_rank_move<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->tmp_idx, s->newc,
                                                               s->hash_tmp, s->key,
                                                               s->cell_start, s->cell_count,
                                                               s->posA, s->velA, s->posB,
                                                               s->velB, s->cellB, s->home);
             BBX_HIP_CHECK(hipGetLastError());
-            PROF_END();
+            BBX_PROF_END();
         }
         std::swap(s->posA, s->posB);
         std::swap(s->velA, s->velB);
         std::swap(s->cellA, s->cellB);
     }
     if (build_buckets && n > 0) {
-        PROF_BEGIN("buckets_density");
+        BBX_PROF_BEGIN("buckets_density");
         if (with_pressure)
             k_buckets_density<true><<<nblocks(n), kBlock, 0, s->stream>>>(
                 c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
@@ -430,18 +429,18 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
                 c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
                 s->rho, s->pres, s->tile_tab, s->counters);
         BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
+        BBX_PROF_END();
     }
     return BBX_OK;
 }
 
 int run_density_only(bbx_solver *s) {
     if (s->n == 0) return BBX_OK;
-    PROF_BEGIN("density");
+    BBX_PROF_BEGIN("density");
     k_density<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->posA, s->nbr16, s->nbr32, s->nbr_cnt,
                                                         s->tile_tab, s->rho);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     return BBX_OK;
 }
 

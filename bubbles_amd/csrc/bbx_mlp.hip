@@ -357,6 +357,8 @@ struct bbx_mlp {
     int tmp_n = 0;
     int adam_t = 0;
     hipStream_t stream = nullptr;
+    int refs = 0;          // solvers this network is attached to
+    bool dead = false;     // destroyed while attached: freed at the last release
 };
 
 namespace bbx {
@@ -486,8 +488,7 @@ int bbx_mlp_create(int seed, bbx_mlp **out) {
     return BBX_OK;
 }
 
-int bbx_mlp_destroy(bbx_mlp *m) {
-    if (!m) return BBX_OK;
+static void mlp_free(bbx_mlp *m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     float *bufs[] = {m->params, m->m1, m->m2, m->grad, m->loss, m->gpart, m->lpart,
                      m->data_x, m->data_t, m->tmp_x, m->tmp_y};
@@ -495,6 +496,17 @@ int bbx_mlp_destroy(bbx_mlp *m) {
         if (b) (void)hipFree(b);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
+}
+
+// A network attached to a solver stays alive until the last solver releases
+// it, whatever order the caller destroys them in.
+int bbx_mlp_destroy(bbx_mlp *m) {
+    if (!m) return BBX_OK;
+    if (m->refs > 0) {
+        m->dead = true;
+        return BBX_OK;
+    }
+    mlp_free(m);
     return BBX_OK;
 }
 
@@ -583,3 +595,13 @@ int bbx_mlp_train(bbx_mlp *m, const float *x, const float *t, int n, int steps, 
 }
 
 }  // extern "C"
+
+namespace bbx {
+void mlp_retain(bbx_mlp *m) {
+    if (m) m->refs++;
+}
+void mlp_release(bbx_mlp *m) {
+    if (!m) return;
+    if (--m->refs <= 0 && m->dead) mlp_free(m);
+}
+}  // namespace bbx

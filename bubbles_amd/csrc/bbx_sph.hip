@@ -19,16 +19,6 @@
 
 namespace bbx {
 
-#define PROF_BEGIN(name)                                                \
-    hipEvent_t ev_a_ = nullptr;                                         \
-    int ev_id_ = -1;                                                    \
-    if (s->prof.enabled) {                                              \
-        ev_id_ = s->prof.name_id(name);                                 \
-        s->prof.begin(s->stream, name, &ev_a_);                         \
-    }
-#define PROF_END()                                                      \
-    if (s->prof.enabled) s->prof.end(s->stream, ev_id_, ev_a_);
-
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // neighbour tables of the substep
@@ -528,23 +518,23 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
     const unsigned nb = nblocks(s->n);
     const NbrTab t = nbr_tab(s);
     if (features) {
-        PROF_BEGIN("nonpressure_features");
+        BBX_PROF_BEGIN("nonpressure_features");
         k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
+        BBX_PROF_END();
     } else if (predict) {
-        PROF_BEGIN("nonpressure_predict");
+        BBX_PROF_BEGIN("nonpressure_predict");
         k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
+        BBX_PROF_END();
     } else {
-        PROF_BEGIN("nonpressure");
+        BBX_PROF_BEGIN("nonpressure");
         k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat);
         BBX_HIP_CHECK(hipGetLastError());
-        PROF_END();
+        BBX_PROF_END();
     }
     return BBX_OK;
 }
@@ -553,43 +543,43 @@ int run_predict_pressure(bbx_solver *s, const StepConst &st, int k) {
     if (s->n == 0) return BBX_OK;
     if (k < 0 || k >= kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
     // the pressure increment uses the host's FP64 delta (s->delta64)
-    PROF_BEGIN("predict_pressure");
+    BBX_PROF_BEGIN_IT("predict_pressure", k);
     k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
                                                                 s->pres, s->rho_star, s->q, s->loop, k);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     return BBX_OK;
 }
 
 int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, int exact, double tol) {
     if (s->n == 0) return BBX_OK;
     if (k >= kMaxIt || max_it > kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
-    PROF_BEGIN("pressure_force");
+    BBX_PROF_BEGIN_IT("pressure_force", k < 0 ? -1 : k);
     k_pressure_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(
         s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->q, s->pres, s->xstar, s->fp, s->posB,
         s->velB, s->force, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     return BBX_OK;
 }
 
 int run_sph_force(bbx_solver *s) {
     if (s->n == 0) return BBX_OK;
-    PROF_BEGIN("sph_force");
+    BBX_PROF_BEGIN("sph_force");
     k_sph_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, nbr_tab(s), s->posA, s->velA, s->rho,
                                                          s->pres, s->force);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     return BBX_OK;
 }
 
 int run_integrate(bbx_solver *s, const StepConst &st) {
     if (s->n == 0) return BBX_OK;
-    PROF_BEGIN("integrate");
+    BBX_PROF_BEGIN("integrate");
     k_integrate<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->posA, s->velA, s->force,
                                                          s->posB, s->velB, s->flag, s->max_force);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     return BBX_OK;
 }
 
@@ -602,11 +592,11 @@ int swap_after_integrate(bbx_solver *s) {
 int run_xsph(bbx_solver *s, const StepConst &st) {
     if (s->n == 0) return BBX_OK;
     if (!(s->prm.pseudo_viscosity * (double)st.dt > 0.1)) return BBX_OK;
-    PROF_BEGIN("xsph");
+    BBX_PROF_BEGIN("xsph");
     k_xsph<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, nbr_tab(s), s->posA, s->velA, s->rho,
                                                     s->velB);
     BBX_HIP_CHECK(hipGetLastError());
-    PROF_END();
+    BBX_PROF_END();
     std::swap(s->velA, s->velB);
     return BBX_OK;
 }

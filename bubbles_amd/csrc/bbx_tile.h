@@ -141,11 +141,33 @@ __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restr
 }
 
 // Cooperative copy of the tile's intervals into LDS: dst[off + e] = fn(start + e).
+// The LDS image is the concatenation of the intervals, so thread t copies
+// LDS slots t, t + blockDim, ...; each thread issues kBatch global loads
+// before its first LDS store, keeping that many loads in flight instead of
+// waiting out one memory latency per staged particle.
 template <typename T, typename Fn>
 __device__ __forceinline__ void tab_stage(const TileTab &t, T *__restrict__ dst, Fn fn) {
-    for (int k = 0; k < t.n_ivl; k++) {
-        const int s = t.start[k], o = t.off[k], l = t.len[k];
-        for (int e = threadIdx.x; e < l; e += blockDim.x) dst[o + e] = fn(s + e);
+    constexpr int kBatch = 8;
+    const int total = t.total;
+    int k = 0;  // interval holding the current slot (slots only increase)
+    for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+        int src[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            const int e = base + b * (int)blockDim.x;
+            src[b] = -1;
+            if (e < total) {
+                while (e >= t.off[k] + t.len[k]) k++;
+                src[b] = t.start[k] + (e - t.off[k]);
+            }
+        }
+        T v[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; b++)
+            if (src[b] >= 0) v[b] = fn(src[b]);
+#pragma unroll
+        for (int b = 0; b < kBatch; b++)
+            if (src[b] >= 0) dst[base + b * (int)blockDim.x] = v[b];
     }
 }
 

@@ -5,9 +5,13 @@ harvested from plain TEXTBOOK PCISPH runs on seeded dam-break and
 block-drop scenes (P = delta * rho0, the pressure one correction assigns to a
 100% density error).  The network (16 -> 64 -> 64 -> 1, GELU, BF16 MFMA,
 FP32 accumulation) is trained on the device with Adam by the library's own
-forward/backward kernels.  At run time the solver starts its predict-correct
-loop from p = P * max(0, MLP(features)) and the unchanged loop still iterates
-to the same tolerance.
+forward/backward kernels on the residual t - p_prev (p_prev / P is feature
+10, the particle's pressure at the end of the previous substep).  At run time
+the solver starts its predict-correct loop from
+    p0 = P * alpha * clamp(net(features) * out_scale + p_prev / P, 0, pcap)
+(alpha < 1: an over-estimate cannot be corrected because negative pressure
+increments are scaled by 0) and the unchanged loop still iterates to the same
+tolerance.
 
 CLI:
   python -m bubbles_amd.learned train  [--out PATH] [--quick]

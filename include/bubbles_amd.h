@@ -195,6 +195,8 @@ int bbx_number_of_time_steps(bbx_solver *s, double interval, double scale, unsig
 /* ---- learned pressure warm start (no reference counterpart) ----
  * 16 -> 64 -> 64 -> 1 MLP, GELU, BF16 weights, FP32 accumulate, on MFMA. */
 int bbx_mlp_create(int seed, bbx_mlp **out);
+/* Destroying a network that is attached to a solver is deferred: it is freed
+ * when the last solver it is attached to detaches it or is destroyed. */
 int bbx_mlp_destroy(bbx_mlp *m);
 /* params layout: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1] in FP32
  * (master copy, trained), then feature mean[16], feature inv_std[16],
@@ -216,6 +218,8 @@ int bbx_mlp_train_step(bbx_mlp *m, const float *x, const float *t, int n, float 
  * given host arrays (copied once); batch sampled by a counter-based RNG */
 int bbx_mlp_train(bbx_mlp *m, const float *x, const float *t, int n, int steps, int batch,
                   float lr, int seed, float *final_loss);
+/* Attach (or, with m == NULL, detach) the warm-start network; the solver
+ * holds a reference until it detaches the network or is destroyed. */
 int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m);
 /* Harvest (features, converged pressure / P) pairs of one TEXTBOOK substep:
  * runs the substep and writes n*16 features and n targets (by particle id). */
@@ -227,6 +231,14 @@ int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *
  * since the last reset, measured with HIP events on the solver's stream. */
 int bbx_profile_enable(bbx_solver *s, int enable);
 int bbx_profile_reset(bbx_solver *s);
+/* Time only the launches of `kernel` (NULL or "": every kernel) and, when
+ * iteration != BBX_PROFILE_ALL_LAUNCHES, only its launch for predict-correct
+ * iteration `iteration` (-1: the warm-start launch).  Fewer events keep the
+ * timed region closer to an unprofiled run.  Loop kernels are reported by
+ * role: predict_pressure[_skipped], pressure_force_{warm,predict,integrate,
+ * skipped} (a launch is skipped once an earlier iteration integrated). */
+#define BBX_PROFILE_ALL_LAUNCHES (-1000)
+int bbx_profile_select(bbx_solver *s, const char *kernel, int iteration);
 /* names: '\n'-separated kernel names; ms/launches arrays of up to cap entries */
 int bbx_profile_get(bbx_solver *s, char *names, int names_cap, double *total_ms,
                     int *launches, int cap, int *count);

@@ -197,3 +197,41 @@ def test_learned_warm_start_reduces_iterations(bbx):
     assert res["mlp"][0] <= 0.9 * res["plain"][0]
     assert res["mlp"][1] <= 1.25 * res["plain"][1] + 1e-3
     assert res["mlp"][2] <= res["plain"][2] + 0.01
+
+
+def test_attached_network_outlives_its_wrapper(bbx):
+    """A network attached to a solver stays valid when the caller drops or
+    destroys its handle: the solver holds a reference until it is closed."""
+    import gc
+
+    sc = scenes.dam_break(16, 8, 16, spacing=0.01, tank=(0.4, 0.3, 0.4), seed=4)
+    p = random_params(3)
+    p[M.OFF_PCAP] = 0.05
+    held = bbx.Mlp(0)
+    held.set_params(p)
+    a = bbx.Solver(sc)
+    a.attach_mlp(held)
+    b = bbx.Solver(sc)
+    tmp = bbx.Mlp(0)
+    tmp.set_params(p)
+    b.attach_mlp(tmp)
+    tmp.close()          # destroy deferred: b still holds the network
+    del tmp
+    c = bbx.Solver(sc)
+
+    def make():
+        n = bbx.Mlp(0)
+        n.set_params(p)
+        return n
+
+    c.attach_mlp(make())  # only the solver refers to the network now
+    gc.collect()
+    for _ in range(3):
+        ra = a.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK, bbx.WARM_MLP)
+        rb = b.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK, bbx.WARM_MLP)
+        rc = c.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK, bbx.WARM_MLP)
+        assert ra.iterations == rb.iterations == rc.iterations
+    pa = a.get("position")
+    assert np.array_equal(pa, b.get("position")) and np.array_equal(pa, c.get("position"))
+    for s in (a, b, c):
+        s.close()

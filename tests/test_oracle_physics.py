@@ -480,3 +480,47 @@ def test_features_divergence_sign_and_linearity(oracle):
     assert out[1.0] > 0 > out[-1.0]
     assert out[0.5] == pytest.approx(0.5 * out[1.0], rel=1e-12)
     assert out[-1.0] == pytest.approx(-out[1.0], rel=1e-12)
+
+
+@pytest.mark.parametrize("spacing,nps", [(0.0097, 0.0), (0.0103, 0.0), (0.0103, 0.5)])
+def test_predicted_density_at_rest_equals_density(oracle, spacing, nps):
+    """With no force and no velocity the predicted positions are the current
+    ones, so rho* equals the density stage's rho, the pressure increment is
+    delta (rho - rho0) (times nps when negative) and the recorded error is
+    the same scaled difference."""
+    sc = scenes.dam_break(10, 10, 10, spacing=spacing, tank=(0.4, 0.4, 0.4), seed=5)
+    sc = dataclasses.replace(sc, target_spacing=0.01 * 2 ** (1 / 3),
+                             relative_radius=0.02 / (0.01 * 2 ** (1 / 3)))
+    # block in the middle of the tank: no particle within a collider radius of a wall
+    pos = scenes.lattice_block((10, 10, 10), spacing, (-0.05, -0.05, -0.05), 5)
+    s = oracle.Sim(with_points(sc, pos), gravity=(0.0, 0.0, 0.0), drag_coefficient=0.0,
+                   negative_pressure_scale=nps)
+    s.distribute()
+    s.stage("interaction")
+    s.stage("density", 0)
+    s.stage("nonpressure")
+    rho = s.get("density")
+    s.stage("predict", 5e-4, 1)
+    np.testing.assert_array_equal(s.get("temp_position"), s.get("position"))
+    delta = s.delta(5e-4)
+    s.stage("predict_pressure", delta, 1)
+    np.testing.assert_array_equal(s.get("density_predicted"), rho)
+    err = rho - 1000.0
+    scale = np.where(delta * err < 0, nps, 1.0)
+    np.testing.assert_allclose(s.get("pressure"), delta * err * scale, rtol=1e-13, atol=1e-12)
+    np.testing.assert_allclose(s.get("density_error"), err * scale, rtol=1e-13, atol=1e-12)
+    assert (err > 0).any() if spacing < 0.01 else (err < 0).any()
+
+
+def test_fast_particle_sets_full_redistribution_flag(oracle):
+    """TimeIntegrationFor flags a higher-level update when a particle moves at
+    least 0.9 of the smallest cell length in one substep."""
+    sc = base_scene()
+    cell = min(oracle.Sim(sc).info()["grid"].cells_len)
+    dt = 5e-4
+    for speed, flagged in ((0.85 * cell / dt, 0), (0.95 * cell / dt, 1)):
+        s = oracle.Sim(with_points(sc, [[0.0, 0.0, 0.0]], [[speed, 0.0, 0.0]]),
+                       gravity=(0.0, 0.0, 0.0), drag_coefficient=0.0)
+        s.flag = 0
+        s.pcisph_substep(dt, 1, 0.01, oracle.MODE_TEXTBOOK)
+        assert s.flag == flagged
