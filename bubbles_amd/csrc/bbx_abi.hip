@@ -2,6 +2,7 @@
 // Setup scalars (FP64 on the host), id-ordered transfers, substep control.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -306,11 +307,24 @@ void fill_sim_const(bbx_solver *s) {
     c.eos_scale = (float)(s->prm.target_density * s->prm.sound_speed * s->prm.sound_speed);
     c.eos_exp = (float)s->prm.eos_exponent;
     c.pseudo_visc = (float)s->prm.pseudo_viscosity;
-    // LDS tile capacity; BBX_TILE_CAP (0 .. kTileCap) overrides it for A/B timing
+    // LDS tile capacities; BBX_TILE_CAP (0 .. kTileCap) and BBX_TILE_CAP2
+    // (0 .. kTileCap2) lower them to time and test the global-memory paths
     c.tile_cap = kTileCapHost;
+    c.tile_cap2 = kTileCap2Host;
     if (const char *e = std::getenv("BBX_TILE_CAP")) {
         int v = std::atoi(e);
         c.tile_cap = v < 0 ? 0 : (v > kTileCapHost ? kTileCapHost : v);
+    }
+    // rows of up to 64 candidates take the two-pass mask path of the bucket
+    // build; BBX_BUCKET_ROW_MAX (0 .. 64) lowers the limit to test the other path
+    c.row_max = 64;
+    if (const char *e = std::getenv("BBX_BUCKET_ROW_MAX")) {
+        int v = std::atoi(e);
+        c.row_max = v < 0 ? 0 : (v > 64 ? 64 : v);
+    }
+    if (const char *e = std::getenv("BBX_TILE_CAP2")) {
+        int v = std::atoi(e);
+        c.tile_cap2 = v < 0 ? 0 : (v > kTileCap2Host ? kTileCap2Host : v);
     }
 }
 
@@ -503,14 +517,14 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->feat, N * BBX_NFEAT);
     if (!rc) rc = dalloc(&s->pwarm, N);
     if (!rc) rc = dalloc(&s->loop, 1);
-    if (!rc) rc = dalloc(&s->max_force, 1);
+    if (!rc) rc = dalloc(&s->max_force, kSlots);
     if (!rc) rc = dalloc(&s->flag, 1);
     if (!rc) rc = dalloc(&s->counters, CNT_N);
     if (!rc && hipHostMalloc((void **)&s->h_loop, sizeof(LoopState)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * CNT_N) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned)) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned) * kSlots) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess))
         rc = fail(BBX_E_HIP, "event create");
@@ -540,7 +554,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (e == hipSuccess) e = hipMemset(s->nbr_cnt, 0, sizeof(int) * N);
     if (e == hipSuccess) e = hipMemset(s->tile_tab, 0, sizeof(int) * (size_t)s->ntiles * kTabIntsHost);
     if (e == hipSuccess) e = hipMemset(s->feat, 0, sizeof(float) * N * BBX_NFEAT);
-    if (e == hipSuccess) e = hipMemset(s->max_force, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(s->max_force, 0, sizeof(unsigned) * kSlots);
     if (e == hipSuccess) e = hipMemset(s->loop, 0, sizeof(LoopState));
     if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(int));
     if (e == hipSuccess) e = hipMemset(s->counters, 0, sizeof(int) * CNT_N);
@@ -706,9 +720,10 @@ static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
     BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->counters, sizeof(int) * CNT_N, hipMemcpyDeviceToHost,
                                  s->stream));
     if (loop_len > 0)
-        BBX_HIP_CHECK(hipMemcpyAsync(s->h_loop, s->loop, sizeof(LoopState), hipMemcpyDeviceToHost, s->stream));
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned), hipMemcpyDeviceToHost,
-                                 s->stream));
+        BBX_HIP_CHECK(hipMemcpyAsync(s->h_loop, s->loop, offsetof(LoopState, slot),
+                                     hipMemcpyDeviceToHost, s->stream));
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
+                                 hipMemcpyDeviceToHost, s->stream));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
     if (st) {
         st->full_redistribution = s->h_counters[CNT_FULL];
@@ -903,26 +918,30 @@ int bbx_neighbors_get(bbx_solver *s, int *counts, int *out_ids) {
         const int ntiles = (int)((n + kBlock - 1) / kBlock);
         std::vector<int> tab((size_t)ntiles * kTabIntsHost);
         BBX_HIP_CHECK(hipMemcpy(tab.data(), s->tile_tab, sizeof(int) * tab.size(), hipMemcpyDeviceToHost));
-        std::vector<int> row32(n);
-        std::vector<unsigned short> row16(n);
+        // one quad row (entries 4q..4q+3 of every particle) at a time
+        std::vector<int> quad32(4 * n);
+        std::vector<unsigned short> quad16(4 * n);
         for (int k = 0; k < kBucket; k++) {
-            BBX_HIP_CHECK(hipMemcpy(row32.data(), s->nbr32 + (size_t)k * s->stride, sizeof(int) * n,
-                                    hipMemcpyDeviceToHost));
-            BBX_HIP_CHECK(hipMemcpy(row16.data(), s->nbr16 + (size_t)k * s->stride,
-                                    sizeof(unsigned short) * n, hipMemcpyDeviceToHost));
+            if ((k & 3) == 0) {
+                const size_t base = nbr_index(k, 0, s->stride);
+                BBX_HIP_CHECK(hipMemcpy(quad32.data(), s->nbr32 + base, sizeof(int) * 4 * n,
+                                        hipMemcpyDeviceToHost));
+                BBX_HIP_CHECK(hipMemcpy(quad16.data(), s->nbr16 + base,
+                                        sizeof(unsigned short) * 4 * n, hipMemcpyDeviceToHost));
+            }
             for (size_t i = 0; i < n; i++) {
                 int v = -1;
                 if (k < cnt[i]) {
                     const int *t = &tab[(i / kBlock) * kTabIntsHost];
                     int j = -1;
                     if (t[1]) {  // staged tile: local index -> interval -> slot
-                        const int jl = row16[i];
+                        const int jl = quad16[4 * i + (k & 3)];
                         for (int q = 0; q < t[0]; q++) {
                             const int st = t[3 + q], len = t[3 + 9 + q], off = t[3 + 18 + q];
                             if (jl >= off && jl < off + len) j = st + (jl - off);
                         }
                     } else {
-                        j = row32[i];
+                        j = quad32[4 * i + (k & 3)];
                     }
                     v = (j >= 0 && (size_t)j < n) ? ids[j] : -2;
                 }
@@ -956,11 +975,13 @@ int bbx_pcisph_delta(bbx_solver *s, double dt, double *delta) {
 }
 
 static int max_force_host(bbx_solver *s, double *mf) {
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned), hipMemcpyDeviceToHost,
-                                 s->stream));
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
+                                 hipMemcpyDeviceToHost, s->stream));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    unsigned m = 0;  // non-negative floats order like their bit patterns
+    for (int k = 0; k < kSlots; k++) m = s->h_max_force[k] > m ? s->h_max_force[k] : m;
     float f;
-    std::memcpy(&f, s->h_max_force, sizeof(float));
+    std::memcpy(&f, &m, sizeof(float));
     *mf = (double)f;
     return BBX_OK;
 }
@@ -996,7 +1017,7 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     if (warm == BBX_WARM_MLP && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
     BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, 1));
+    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
     BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
     // UpdateGridDistribution + ResetHigherLevel + interaction + density
     BBX_CHECK(distribute(s, false, true, false));
@@ -1070,7 +1091,7 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
     BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, 1));
+    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
     BBX_CHECK(distribute(s, false, true, true));
     StepConst st{};
     st.dt = (float)dt;

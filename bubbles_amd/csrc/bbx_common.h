@@ -41,6 +41,15 @@ constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
 constexpr int kScanItems = 8;              // items per thread in the cell scan
 constexpr int kTabIntsHost = 32;           // ints per tile table (see bbx_tile.h)
 constexpr int kTileCapHost = 3072;         // LDS tile capacity in particles (bbx_tile.h)
+constexpr int kTileCap2Host = 2528;        // the same for two-array kernels
+
+// Neighbour tables are quad-major: bucket entries 4q..4q+3 of particle i are
+// adjacent at [(q * stride + i) * 4 + (k & 3)], so one 8-byte (u16) or
+// 16-byte (int) load per lane fetches four entries and a wave's load of
+// quad q is one contiguous 512 B / 1 KB run.
+__host__ __device__ inline size_t nbr_index(int k, int i, int stride) {
+    return ((size_t)(k >> 2) * (size_t)stride + (size_t)i) * 4 + (size_t)(k & 3);
+}
 
 // Grid geometry in FP64 (the hash is computed in the reference's precision).
 struct GridGeom {
@@ -74,6 +83,8 @@ struct SimConst {
     float eos_scale, eos_exp;
     float pseudo_visc;
     int tile_cap;      // tiles with more staged candidates read global memory
+    int tile_cap2;     // the same for kernels that stage two arrays
+    int row_max;       // bucket build: rows up to this many candidates use the mask path (<= 64)
 };
 
 // Per-call step constants.
@@ -86,10 +97,15 @@ struct StepConst {
 // predict-correct loop state on the device: per-iteration FP64 max error
 // bits and done flags (iteration k's pressure-force kernel integrates and
 // sets done[k] once the error of iteration k is below the tolerance)
+// Global maxima are reduced per workgroup and then with atomics into one of
+// kSlots slots (workgroup index mod kSlots), so no single address takes an
+// atomic from every workgroup; readers take the max over the slots.
 constexpr int kMaxIt = 64;
+constexpr int kSlots = 32;
 struct LoopState {
-    unsigned long long err[kMaxIt];
+    unsigned long long err[kMaxIt];              // max over the slots (host readback)
     int done[kMaxIt];
+    unsigned long long slot[kMaxIt][kSlots];     // per-slot FP64 bits of max |err|
 };
 
 // device-side counters
@@ -174,8 +190,9 @@ struct bbx_solver {
     // sort scratch (indexed by old slot / new position)
     unsigned long long *key = nullptr;
     int *newc = nullptr, *rank_tmp = nullptr, *tmp_idx = nullptr, *hash_tmp = nullptr;
-    // neighbour lists (bucket slot k of particle i at [k * stride + i], k < 100):
-    // 16-bit LDS-local indices for staged tiles, 32-bit storage indices otherwise
+    // neighbour lists (bucket slot k of particle i at nbr_index(k, i, stride),
+    // k < 100): 16-bit LDS-local indices for staged tiles, 32-bit storage
+    // indices otherwise
     unsigned short *nbr16 = nullptr;
     int *nbr32 = nullptr, *nbr_cnt = nullptr;
     int *tile_tab = nullptr;   // per-tile interval tables (kTabInts ints each)
@@ -189,7 +206,7 @@ struct bbx_solver {
     bbx::LoopState *loop = nullptr;          // predict-correct loop state (device)
     bbx::LoopState *h_loop = nullptr;        // pinned readback
     unsigned long long *max_err = nullptr;   // FP64 bits of max |error|
-    unsigned *max_force = nullptr;           // FP32 bits of max |f|
+    unsigned *max_force = nullptr;           // kSlots FP32 bits of max |f|
     int *flag = nullptr;                     // requiresHigherLevelUpdate
     int *counters = nullptr;                 // CNT_*
     // pinned readback

@@ -195,6 +195,14 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
     }
     return v;
 }
+__device__ __forceinline__ int wave_min_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
 __device__ __forceinline__ int wave_sum_i32(int v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;

@@ -252,6 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     __shared__ TileTab tt;
     __shared__ int rdelta[9];
     __shared__ int smin, smax;
+    __shared__ int row_a[9], row_b[9];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < c.n;
     const int hc = valid ? home[i] : -1;
@@ -260,16 +261,21 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         smax = -1;
     }
     __syncthreads();
-    if (hc >= 0) {
-        atomicMin(&smin, hc);
-        atomicMax(&smax, hc);
+    {
+        const int wmin = wave_min_i32(hc >= 0 ? hc : 0x7fffffff);
+        const int wmax = wave_max_i32(hc);
+        if ((threadIdx.x & 63) == 0 && wmax >= 0) {
+            atomicMin(&smin, wmin);
+            atomicMax(&smax, wmax);
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        tab_build(c.g, smin, smax, start, count, c.tile_cap, tt, rdelta);
-        tab_store(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
-    }
+    // the nine row ranges in parallel, then one thread merges them
+    if (threadIdx.x < 9) tab_row(c.g, smin, smax, start, threadIdx.x, row_a[threadIdx.x], row_b[threadIdx.x]);
     __syncthreads();
+    if (threadIdx.x == 0) tab_merge(row_a, row_b, c.tile_cap, tt, rdelta);
+    __syncthreads();
+    tab_store(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
     const bool staged = tt.staged != 0;
     if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
     __syncthreads();
@@ -279,60 +285,116 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     if (hc >= 0) unlinear(c.g, hc, u);
     int cnt = 0, dropped = 0;
     float sum = 0.f, comp = 0.f;
-    const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2];
-    const int ylo = hc >= 0 ? u[1] - 1 : 1, yhi = hc >= 0 ? u[1] + 1 : 0;
-    // neighbour cells in (y,x,z) order; their ranges are loaded up front so
-    // the 27 start/count loads are in flight together
-    int cell[27], shift[27];
-    int ncell = 0;
-    for (int y = ylo; y <= yhi; y++) {
-        if (y < 0 || y >= ny) continue;
-        for (int x = u[0] - 1; x <= u[0] + 1; x++) {
-            if (x < 0 || x >= nx) continue;
-            for (int z = u[2] - 1; z <= u[2] + 1; z++) {
-                if (z < 0 || z >= nz) continue;
-                cell[ncell] = x + y * nx + z * nx * ny;
-                shift[ncell] = staged ? rdelta[(z - u[2] + 1) * 3 + (y - u[1] + 1)] : 0;
-                ncell++;
+    // the current quad of bucket entries, stored once complete
+    unsigned e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+    auto flush_quad = [&](int k) {  // k = index of the quad's last entry + 1
+        const size_t at = (size_t)((k - 1) >> 2) * c.stride + i;
+        if (staged)
+            reinterpret_cast<ushort4 *>(nbr16)[at] =
+                make_ushort4((unsigned short)e0, (unsigned short)e1, (unsigned short)e2,
+                             (unsigned short)e3);
+        else
+            reinterpret_cast<int4 *>(nbr32)[at] = make_int4((int)e0, (int)e1, (int)e2, (int)e3);
+    };
+    const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
+    // append accepted candidate j (storage index; LDS index j + sh) to the
+    // bucket, up to the cap, and to the density sum
+    auto emit = [&](int j, int sh) {
+        if (cnt >= kBucket) {
+            dropped++;
+            return;
+        }
+        const int jl = j + sh;
+        const float4 pj = staged ? sx[jl] : pos[j];
+        const unsigned v = staged ? (unsigned)jl : (unsigned)j;
+        const int slot = cnt & 3;
+        e0 = slot == 0 ? v : e0;
+        e1 = slot == 1 ? v : e1;
+        e2 = slot == 2 ? v : e2;
+        e3 = slot == 3 ? v : e3;
+        cnt++;
+        if (slot == 3) flush_quad(cnt);
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+    };
+    if (hc >= 0) {
+        // Row r = (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are
+        // contiguous in storage.  B[r] = start of x-1, x, x+1 and end of x+1;
+        // cells and rows outside the grid are empty ranges.
+        int B[9][4], sh[9];
+        bool narrow = true;
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
+            sh[r] = staged ? rdelta[r] : 0;
+            B[r][0] = B[r][1] = B[r][2] = B[r][3] = 0;
+            if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                const int cx = u[0] + yy * nx + zz * nxy;
+                const int b1 = start[cx], b2 = start[cx + 1];
+                B[r][0] = u[0] > 0 ? start[cx - 1] : b1;
+                B[r][1] = b1;
+                B[r][2] = b2;
+                B[r][3] = u[0] + 1 < nx ? start[cx + 2] : b2;
+                narrow = narrow && (B[r][3] - B[r][0] <= c.row_max);
             }
         }
-    }
-    int cb[27], ce[27];
+        if (narrow) {
+            // pass 1: each row's candidates tested in storage order into a mask
+            unsigned long long M[9];
 #pragma unroll
-    for (int q = 0; q < 27; q++) {
-        if (q < ncell) {
-            cb[q] = start[cell[q]];
-            ce[q] = cb[q] + count[cell[q]];
-        }
-    }
+            for (int r = 0; r < 9; r++) {
+                unsigned long long m = 0ull;
+                const int a = B[r][0], len = B[r][3] - B[r][0], s = sh[r];
+                for (int t0 = 0; t0 < len; t0 += 4) {
+                    float4 pc[4];
 #pragma unroll
-    for (int q = 0; q < 27; q++) {
-        if (q >= ncell) break;
-        const int b = cb[q], e = ce[q], sh = shift[q];
-        // candidates in chain order, four loads issued before the tests
-        for (int j0 = b; j0 < e; j0 += 4) {
-            float4 pc[4];
+                    for (int q = 0; q < 4; q++)
+                        if (t0 + q < len) pc[q] = staged ? sx[a + t0 + q + s] : pos[a + t0 + q];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (j0 + u < e) pc[u] = staged ? sx[j0 + u + sh] : pos[j0 + u];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int j = j0 + u;
-                if (j >= e) break;
-                const float4 pj = pc[u];
-                if (!in_bucket(c, pi, pj)) continue;
-                if (cnt < kBucket) {
-                    if (staged) nbr16[(size_t)cnt * c.stride + i] = (unsigned short)(j + sh);
-                    else nbr32[(size_t)cnt * c.stride + i] = j;
-                    cnt++;
-                    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-                } else {
-                    dropped++;
+                    for (int q = 0; q < 4; q++)
+                        if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
                 }
+                M[r] = m;
             }
+            // pass 2: emit in the reference's order, cells (y, x, z), chain order inside
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                    for (int dz = 0; dz < 3; dz++) {
+                        const int r = dz * 3 + dy;
+                        const int w = B[r][dx + 1] - B[r][dx];
+                        if (w <= 0) continue;
+                        unsigned long long m = M[r] >> (B[r][dx] - B[r][0]);
+                        if (w < 64) m &= (1ull << w) - 1ull;
+                        while (m) {
+                            const int b = __builtin_ctzll(m);
+                            m &= m - 1ull;
+                            emit(B[r][dx] + b, sh[r]);
+                        }
+                    }
+        } else {
+            // a row with more than 64 candidates: cell by cell
+            for (int dy = 0; dy < 3; dy++)
+                for (int dx = 0; dx < 3; dx++) {
+                    const int x = u[0] + dx - 1;
+                    if (x < 0 || x >= nx) continue;
+                    for (int dz = 0; dz < 3; dz++) {
+                        const int yy = u[1] + dy - 1, zz = u[2] + dz - 1;
+                        if (yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+                        const int cell = x + yy * nx + zz * nxy;
+                        const int sr = staged ? rdelta[dz * 3 + dy] : 0;
+                        const int b = start[cell], e = b + count[cell];
+                        for (int j = b; j < e; j++) {
+                            const float4 pj = staged ? sx[j + sr] : pos[j];
+                            if (in_bucket(c, pi, pj)) emit(j, sr);
+                        }
+                    }
+                }
         }
     }
+    if (cnt & 3) flush_quad(cnt);
     nbr_cnt[i] = cnt;
     float di = sum * c.mass;
     rho[i] = di;
@@ -364,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
     const float4 pi = pos[i];
     float sum = 0.f, comp = 0.f;
     const int cnt = nbr_cnt[i];
-    for_each_neighbour<8>(nbr16, nbr32, staged, i, cnt, c.stride, [&](int j) {
+    for_each_neighbour<2>(nbr16, nbr32, staged, i, cnt, c.stride, [&](int j) {
         const float4 pj = staged ? sx[j] : pos[j];
         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));

@@ -41,30 +41,33 @@ constexpr int kTilesPerWave = 4;  // training tiles per wave per step
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// BF16 round-to-nearest-even conversions (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ unsigned short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return (unsigned short)(u >> 16);
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
-    bf16x4 r;
-    r[0] = (short)f2bf(a);
-    r[1] = (short)f2bf(b);
-    r[2] = (short)f2bf(c);
-    r[3] = (short)f2bf(d);
-    return r;
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    const bf4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(bf16x4, v);
 }
 
-// GELU, tanh form: 0.5 z (1 + tanh(u)), u = sqrt(2/pi) (z + 0.044715 z^3)
+// GELU, tanh form: 0.5 z (1 + tanh(u)) = z s with s = 1 / (1 + exp(-2u)) and
+// u = sqrt(2/pi) (z + 0.044715 z^3); exp and the reciprocal are the hardware
+// approximations (about 1 ulp), far below the BF16 rounding that follows.
+__device__ __forceinline__ float gelu_sig(float z) {
+    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+    const float u = k0 * (z + k1 * (z * z) * z);
+    return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
+}
+
+// value and derivative: gelu'(z) = s + 2 z s (1 - s) u'(z)
 __device__ __forceinline__ void gelu_fwd(float z, float &g, float &dg) {
     const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    float z2 = z * z;
-    float u = k0 * (z + k1 * z2 * z);
-    float t = 1.f - 2.f / (1.f + __expf(2.f * u));
-    g = 0.5f * z * (1.f + t);
-    dg = 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * k0 * (1.f + 3.f * k1 * z2);
+    const float sg = gelu_sig(z);
+    g = z * sg;
+    dg = sg + 2.f * z * sg * (1.f - sg) * k0 * (1.f + 3.f * k1 * (z * z));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -141,6 +144,40 @@ __device__ __forceinline__ void forward_tile(const Weights &w, const float x4[4]
     a.y = ys + w.b3;
 }
 
+// forward of one tile for inference only: no derivatives, nothing kept
+__device__ __forceinline__ float forward_infer(const Weights &w, const float x4[4]) {
+    const bf16x4 xb = pack4((x4[0] - w.mean[0]) * w.istd[0], (x4[1] - w.mean[1]) * w.istd[1],
+                            (x4[2] - w.mean[2]) * w.istd[2], (x4[3] - w.mean[3]) * w.istd[3]);
+    bf16x4 h1b[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(w.w1[m], xb, acc);
+        float g[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float z = acc[v] + w.b1[m][v];
+            g[v] = z * gelu_sig(z);
+        }
+        h1b[m] = pack4(g[0], g[1], g[2], g[3]);
+    }
+    float ys = 0.f;
+#pragma unroll
+    for (int m2 = 0; m2 < 4; m2++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) acc = mfma(w.w2[m2][kc], h1b[kc], acc);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float z = acc[v] + w.b2[m2][v];
+            ys += w.w3[m2][v] * (z * gelu_sig(z));
+        }
+    }
+    ys += __shfl_xor(ys, 16, 64);
+    ys += __shfl_xor(ys, 32, 64);
+    return ys + w.b3;
+}
+
 // ---------------------------------------------------------------------------
 // inference: prediction_i = y_i * out_scale + resid * x_i[PPREV] (target
 // units).  Solver path (clamp = 1): out_i = scale * alpha * clamp(prediction,
@@ -164,10 +201,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restri
         float x4[4] = {0.f, 0.f, 0.f, 0.f};
         if (i < n)
             for (int v = 0; v < 4; v++) x4[v] = x[(size_t)(4 * q + v) * fstride + (size_t)i * sstride];
-        Acts a;
-        forward_tile(w, x4, a, lane);
+        const float yn = forward_infer(w, x4);
         if (lane < 16 && i < n) {
-            float y = a.y * oscale;
+            float y = yn * oscale;
             if (resid != 0.f) y += resid * x[(size_t)BBX_F_PPREV * fstride + (size_t)i * sstride];
             if (clamp) y = alpha * fminf(fmaxf(y, 0.f), pcap);
             out[i] = y * scale;

@@ -3,8 +3,8 @@
 // Every kernel is one thread per particle in cell-sorted storage order.  A
 // workgroup's 256 particles form a tile: the tile's candidate intervals
 // (recorded by the bucket kernel) are first copied into LDS with coalesced
-// loads, then each thread walks its bucket -- LDS-local 16-bit indices,
-// slot-major table nbr16[k * stride + i] -- reading neighbours from LDS.
+// loads, then each thread walks its bucket -- LDS-local 16-bit indices in
+// the quad-major table (four entries per load) -- reading neighbours from LDS.
 // Tiles too large to stage walk 32-bit storage indices and read global
 // memory instead.  Fusions relative to the reference's launch sequence
 // (src/equations/pcisph_equations3.cpp:218-256):
@@ -20,6 +20,19 @@
 namespace bbx {
 
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// block-wide max of v, one atomic per workgroup into its slot (whole block calls)
+__device__ __forceinline__ void block_max_to_slot(unsigned v, unsigned *__restrict__ slots) {
+    __shared__ unsigned wmax[kBlock / 64];
+    v = wave_max_u32(v);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = wmax[0];
+        for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
+        if (m) atomicMax(&slots[blockIdx.x & (kSlots - 1)], m);
+    }
+}
 
 // neighbour tables of the substep
 struct NbrTab {
@@ -74,13 +87,14 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
                                                         float4 *__restrict__ xstar,
                                                         float *__restrict__ rho_star,
                                                         float *__restrict__ feat) {
-    __shared__ float4 sx[kTileCap];
-    __shared__ float4 sv[kTileCap];
+    __shared__ float4 sx[kTileCap2];
+    __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
-    const bool staged = tt.staged != 0;
-    if (staged) {
+    const bool staged = tt.staged != 0;                  // buckets hold LDS-local indices
+    const bool in_lds = staged && tt.total <= c.tile_cap2;  // and they fit this kernel's LDS
+    if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return pos[j]; });
         tab_stage(tt, sv, [&](int j) {
             float4 v = vel[j];
@@ -99,15 +113,16 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     float fz = c.mass * c.gz - c.drag * vi.z;
     const int cnt = nb.cnt[i];
     float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
         float4 xj, vj;
-        if (staged) {
+        if (in_lds) {
             xj = sx[j];
             vj = sv[j];
         } else {
-            xj = pos[j];
-            float4 v = vel[j];
-            vj = make_float4(v.x, v.y, v.z, rho[j]);
+            const int g = staged ? tab_global(tt, j) : j;
+            xj = pos[g];
+            float4 v = vel[g];
+            vj = make_float4(v.x, v.y, v.z, rho[g]);
         }
         const float rj = vj.w;
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -204,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         const float4 xi = xstar[i];
         float sum = 0.f, comp = 0.f;
         const int cnt = nb.cnt[i];
-        for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
             const float4 xj = staged ? sx[j] : xstar[j];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
@@ -233,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     if (threadIdx.x == 0) {
         unsigned long long m = wmax[0];
         for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
-        if (m) atomicMax(&loop->err[k], m);
+        if (m) atomicMax(&loop->slot[k][blockIdx.x & (kSlots - 1)], m);
     }
 }
 
@@ -272,13 +287,24 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         if (blockIdx.x == 0 && threadIdx.x == 0) loop->done[k] = 1;
         return;
     }
+    __shared__ unsigned long long emax;
     bool integrate = false;
     if (k >= 0) {
-        const double e = __longlong_as_double((long long)loop->err[k]);
+        // the iteration's FP64 max error: max over the slots
+        if (threadIdx.x < 64) {
+            unsigned long long v = threadIdx.x < kSlots ? loop->slot[k][threadIdx.x] : 0ull;
+            v = wave_max_u64(v);
+            if (threadIdx.x == 0) emax = v;
+        }
+        __syncthreads();
+        const double e = __longlong_as_double((long long)emax);
         const double ratio = e / c.rho0d;
         integrate = exact || k == max_it - 1 || fabs(ratio) < tol;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            loop->err[k] = emax;
+            if (integrate) loop->done[k] = 1;
+        }
     }
-    if (integrate && blockIdx.x == 0 && threadIdx.x == 0) loop->done[k] = 1;
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
@@ -296,7 +322,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         const float qi = q[i];
         float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
         const int cnt = nb.cnt[i];
-        for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
             if (j == self) return;
             float4 xj;
             if (staged) {
@@ -331,10 +357,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             fbits = __float_as_uint(sqrtf(Fx * Fx + Fy * Fy + Fz * Fz));
         }
     }
-    if (integrate) {
-        fbits = wave_max_u32(fbits);
-        if ((threadIdx.x & 63) == 0 && fbits) atomicMax(max_force, fbits);
-    }
+    if (integrate) block_max_to_slot(fbits, max_force);
 }
 
 // ---------------------------------------------------------------------------
@@ -348,12 +371,13 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
                                                       const float *__restrict__ rho,
                                                       const float *__restrict__ pres,
                                                       float4 *__restrict__ force) {
-    __shared__ float4 sx[kTileCap];
-    __shared__ float4 sv[kTileCap];
+    __shared__ float4 sx[kTileCap2];
+    __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
+    const bool in_lds = staged && tt.total <= c.tile_cap2;
     auto xq = [&](int j) {
         float4 p = pos[j];
         float d = rho[j];
@@ -363,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
         float4 v = vel[j];
         return make_float4(v.x, v.y, v.z, rho[j]);
     };
-    if (staged) {
+    if (in_lds) {
         tab_stage(tt, sx, xq);
         tab_stage(tt, sv, vr);
     }
@@ -380,15 +404,16 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     float fz = c.mass * c.gz - c.drag * vi.z;
     float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
     const int cnt = nb.cnt[i];
-    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
         if (j == self) return;
         float4 xj, vj;
-        if (staged) {
+        if (in_lds) {
             xj = sx[j];
             vj = sv[j];
         } else {
-            xj = xq(j);
-            vj = vr(j);
+            const int g = staged ? tab_global(tt, j) : j;
+            xj = xq(g);
+            vj = vr(g);
         }
         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);
@@ -427,8 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(SimConst c, StepConst st,
         vel_out[i] = make_float4(v.x, v.y, v.z, vi.w);
         fbits = __float_as_uint(sqrtf(fi.x * fi.x + fi.y * fi.y + fi.z * fi.z));
     }
-    fbits = wave_max_u32(fbits);
-    if ((threadIdx.x & 63) == 0 && fbits) atomicMax(max_force, fbits);
+    block_max_to_slot(fbits, max_force);
 }
 
 // XSPH velocity smoothing (src/equations/sph_equations3.cpp:341-382), active
@@ -438,17 +462,18 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
                                                  const float4 *__restrict__ vel,
                                                  const float *__restrict__ rho,
                                                  float4 *__restrict__ vel_out) {
-    __shared__ float4 sx[kTileCap];
-    __shared__ float4 sv[kTileCap];
+    __shared__ float4 sx[kTileCap2];
+    __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
     tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
+    const bool in_lds = staged && tt.total <= c.tile_cap2;
     auto vr = [&](int j) {
         float4 v = vel[j];
         return make_float4(v.x, v.y, v.z, rho[j]);
     };
-    if (staged) {
+    if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return pos[j]; });
         tab_stage(tt, sv, vr);
     }
@@ -459,14 +484,15 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     float sx_ = 0.f, sy_ = 0.f, sz_ = 0.f, ws = 0.f;
     const float spiky_w = c.spiky_lap * c.h * c.h / 6.f;  // 15 / (pi h^3)
     const int cnt = nb.cnt[i];
-    for_each_neighbour<8>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
         float4 xj, vj;
-        if (staged) {
+        if (in_lds) {
             xj = sx[j];
             vj = sv[j];
         } else {
-            xj = pos[j];
-            vj = vr(j);
+            const int g = staged ? tab_global(tt, j) : j;
+            xj = pos[g];
+            vj = vr(g);
         }
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         float r = sqrtf(dx * dx + dy * dy + dz * dz);

@@ -18,6 +18,10 @@
 namespace bbx {
 
 constexpr int kTileCap = kTileCapHost;  // staged particles per tile
+// Kernels that stage two float4 arrays hold at most kTileCap2 particles so
+// that two workgroups fit the 160 KB LDS of a CU; a staged tile above that
+// reads its neighbours from global memory through tab_global.
+constexpr int kTileCap2 = kTileCap2Host;
 constexpr int kMaxIvl = 9;
 constexpr int kTabInts = kTabIntsHost;  // per-tile table stride in ints
 
@@ -31,9 +35,10 @@ struct TileTab {
 constexpr int kTabWords = 3 + 3 * kMaxIvl;
 static_assert(sizeof(TileTab) == kTabWords * sizeof(int), "TileTab layout");
 
+// the block's threads copy a __shared__ TileTab to global memory
 __device__ __forceinline__ void tab_store(int *__restrict__ g, const TileTab &t) {
     const int *src = reinterpret_cast<const int *>(&t);
-    for (int k = 0; k < kTabWords; k++) g[k] = src[k];
+    if (threadIdx.x < kTabWords) g[threadIdx.x] = src[threadIdx.x];
 }
 
 // the block's threads copy the table into a __shared__ TileTab
@@ -49,43 +54,55 @@ __device__ __forceinline__ int tab_local(const TileTab &t, int j) {
     return -1;
 }
 
-// Build the interval table of a tile from the min/max home cell of its
-// particles (single thread).  rdelta[(dz+1)*3 + (dy+1)] receives, for each
-// row offset, LDS index - storage index of that row's interval.
-__device__ __forceinline__ void tab_build(const GridGeom &g, int cmin, int cmax,
-                                          const int *__restrict__ start,
-                                          const int *__restrict__ count, int cap, TileTab &t,
-                                          int *__restrict__ rdelta) {
-    t.n_ivl = 0;
-    t.total = 0;
-    t.staged = 1;
-    for (int r = 0; r < 9; r++) rdelta[r] = 0;
+// storage index of LDS-local index jl (jl < t.total)
+__device__ __forceinline__ int tab_global(const TileTab &t, int jl) {
+    int k = 0;
+    while (k + 1 < t.n_ivl && jl >= t.off[k] + t.len[k]) k++;
+    return t.start[k] + (jl - t.off[k]);
+}
+
+// Storage range [a, b) of row r = (dz+1)*3 + (dy+1) of a tile whose home
+// cells span [cmin, cmax]: cells [cmin + dy*nx + dz*nx*ny - 1, cmax + ... + 1]
+// clipped to the grid; a = -1 for an empty row.
+__device__ __forceinline__ void tab_row(const GridGeom &g, int cmin, int cmax,
+                                        const int *__restrict__ start, int r, int &a, int &b) {
+    a = -1;
+    b = -1;
     if (cmax < 0) return;
     const int nx = g.n[0], nxy = g.n[0] * g.n[1];
+    const int dz = r / 3 - 1, dy = r % 3 - 1;
     const int last = (int)g.total - 1;
+    long long lo = (long long)cmin + dy * nx + (long long)dz * nxy - 1;
+    long long hi = (long long)cmax + dy * nx + (long long)dz * nxy + 1;
+    if (lo < 0) lo = 0;
+    if (hi > last) hi = last;
+    if (lo > hi) return;
+    const int ra = start[lo], rb = start[hi + 1];  // start[] has total + 1 entries
+    if (rb <= ra) return;
+    a = ra;
+    b = rb;
+}
+
+// Merge the nine row ranges of a tile into its interval table (single
+// thread, no memory traffic).  rdelta[r] receives, for each row, LDS index -
+// storage index of that row's interval.
+__device__ __forceinline__ void tab_merge(const int *row_a, const int *row_b, int cap, TileTab &t,
+                                          int *__restrict__ rdelta) {
     int s0[kMaxIvl], s1[kMaxIvl], m = 0;
-    int row_a[9];
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            row_a[(dz + 1) * 3 + (dy + 1)] = -1;
-            long long lo = (long long)cmin + dy * nx + (long long)dz * nxy - 1;
-            long long hi = (long long)cmax + dy * nx + (long long)dz * nxy + 1;
-            if (lo < 0) lo = 0;
-            if (hi > last) hi = last;
-            if (lo > hi) continue;
-            int a = start[lo], b = start[hi] + count[hi];
-            if (b <= a) continue;
-            row_a[(dz + 1) * 3 + (dy + 1)] = a;
-            // insertion by start
-            int p = m++;
-            while (p > 0 && s0[p - 1] > a) {
-                s0[p] = s0[p - 1];
-                s1[p] = s1[p - 1];
-                p--;
-            }
-            s0[p] = a;
-            s1[p] = b;
+    for (int r = 0; r < 9; r++) {
+        rdelta[r] = 0;
+        const int a = row_a[r];
+        if (a < 0) continue;
+        // insertion by start
+        int p = m++;
+        while (p > 0 && s0[p - 1] > a) {
+            s0[p] = s0[p - 1];
+            s1[p] = s1[p - 1];
+            p--;
         }
+        s0[p] = a;
+        s1[p] = row_b[r];
+    }
     // merge overlapping / touching ranges
     int k = -1;
     for (int q = 0; q < m; q++) {
@@ -112,31 +129,42 @@ __device__ __forceinline__ void tab_build(const GridGeom &g, int cmin, int cmax,
                 if (row_a[r] >= t.start[q] && row_a[r] < t.start[q] + t.len[q]) rdelta[r] = t.off[q] - t.start[q];
 }
 
-// Walk bucket slots 0..cnt-1 of particle i, software-pipelined: the next U
-// bucket entries are loaded while the current U neighbours are processed, so
-// each lane keeps several loads in flight instead of one dependent chain per
-// neighbour.  body(j) gets an LDS-local index (staged tile) or a storage
-// index, in bucket order.
-template <int U, typename Body>
+// Walk bucket slots 0..cnt-1 of particle i in bucket order.  Entries are
+// read a quad (4 entries, one 8 B or 16 B load) at a time and software-
+// pipelined: the next D quads are in flight while the current D are
+// processed.  body(j) gets an LDS-local index (staged tile) or a storage
+// index.
+template <int D, typename Body>
 __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
                                                    const int *__restrict__ n32, bool staged,
                                                    int i, int cnt, int stride, Body body) {
-    auto entry = [&](int k) -> int {
-        const size_t at = (size_t)k * stride + i;
-        return staged ? (int)n16[at] : n32[at];
+    const int nq = (cnt + 3) >> 2;
+    auto quad = [&](int q) -> int4 {
+        const size_t at = (size_t)q * stride + i;
+        if (staged) {
+            const ushort4 v = reinterpret_cast<const ushort4 *>(n16)[at];
+            return make_int4(v.x, v.y, v.z, v.w);
+        }
+        return reinterpret_cast<const int4 *>(n32)[at];
     };
-    int cur[U];
+    int4 cur[D];
 #pragma unroll
-    for (int u = 0; u < U; u++) cur[u] = u < cnt ? entry(u) : 0;
-    for (int k = 0; k < cnt; k += U) {
-        int nxt[U];
+    for (int d = 0; d < D; d++) cur[d] = d < nq ? quad(d) : make_int4(0, 0, 0, 0);
+    for (int q = 0; q < nq; q += D) {
+        int4 nxt[D];
 #pragma unroll
-        for (int u = 0; u < U; u++) nxt[u] = (k + U + u) < cnt ? entry(k + U + u) : 0;
+        for (int d = 0; d < D; d++)
+            nxt[d] = (q + D + d) < nq ? quad(q + D + d) : make_int4(0, 0, 0, 0);
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k + u < cnt) body(cur[u]);
+        for (int d = 0; d < D; d++) {
+            const int k = (q + d) * 4;
+            if (k < cnt) body(cur[d].x);
+            if (k + 1 < cnt) body(cur[d].y);
+            if (k + 2 < cnt) body(cur[d].z);
+            if (k + 3 < cnt) body(cur[d].w);
+        }
 #pragma unroll
-        for (int u = 0; u < U; u++) cur[u] = nxt[u];
+        for (int d = 0; d < D; d++) cur[d] = nxt[d];
     }
 }
 
