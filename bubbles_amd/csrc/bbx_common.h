@@ -40,7 +40,10 @@ constexpr int kBucket = BBX_BUCKET_SIZE;   // reference bucket cap
 constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
 constexpr int kScanItems = 8;              // items per thread in the cell scan
 constexpr int kTabIntsHost = 32;           // ints per tile table (see bbx_tile.h)
-constexpr int kTileCapHost = 3072;         // LDS tile capacity in particles (bbx_tile.h)
+// LDS tile capacity in particles (bbx_tile.h): 2528 float4 (+ the tile
+// table) lets four single-array or two two-array workgroups share a CU's
+// 160 KB of LDS
+constexpr int kTileCapHost = 2528;
 constexpr int kTileCap2Host = 2528;        // the same for two-array kernels
 
 // Neighbour tables are quad-major: bucket entries 4q..4q+3 of particle i are

@@ -135,10 +135,11 @@ __global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__res
                                                  int *__restrict__ count,
                                                  int *__restrict__ rank_tmp,
                                                  int *__restrict__ counters) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= c.n) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int none = (int)c.g.total;
-    float4 p = pos[s];
+    const bool valid = s < c.n;
+    // the wave stays whole for the aggregated count below
+    const float4 p = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     int u[3], bad = 0;
     int cid = hash_cell(c.g, p, u, &bad);
     bool full = force_full || (*flag != 0);
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__res
         nc = bad ? none : cid;
         k = (unsigned long long)(unsigned)pid_of(p);
     } else {
-        int co = cell_old[s];
+        const int co = valid ? cell_old[s] : none;
         nc = none;
         k = (unsigned long long)s;
         if (!bad && co != none) {
@@ -162,12 +163,28 @@ __global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__res
             }
         }
     }
+    // Storage is sorted by cell, so lanes with the same new cell mostly form
+    // runs: one atomic per run reserves its slots (the order inside a cell
+    // is set later by the key, not by arrival).
+    const int lane = threadIdx.x & 63;
+    const int cellv = valid ? nc : -1;
+    const int prev = __shfl_up(cellv, 1, 64);
+    const bool head = lane == 0 || prev != cellv;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int head_lane = 63 - __builtin_clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next_head = after ? __builtin_ctzll(after) : 64;
+    int base = 0;
+    if (head && valid) base = atomicAdd(&count[nc], next_head - lane);
+    base = __shfl(base, head_lane, 64);
+    if (!valid) return;
     if (bad) atomicAdd(&counters[CNT_BAD_HASH], 1);
     newc[s] = nc;
     // a point outside the grid has no home cell: its bucket stays empty
     hcell[s] = bad ? -1 : cid;
     key[s] = k;
-    rank_tmp[s] = atomicAdd(&count[nc], 1);
+    rank_tmp[s] = base + (lane - head_lane);
 }
 
 // ---------------------------------------------------------------------------

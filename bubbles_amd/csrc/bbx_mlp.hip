@@ -144,69 +144,126 @@ __device__ __forceinline__ void forward_tile(const Weights &w, const float x4[4]
     a.y = ys + w.b3;
 }
 
-// forward of one tile for inference only: no derivatives, nothing kept
-__device__ __forceinline__ float forward_infer(const Weights &w, const float x4[4]) {
-    const bf16x4 xb = pack4((x4[0] - w.mean[0]) * w.istd[0], (x4[1] - w.mean[1]) * w.istd[1],
-                            (x4[2] - w.mean[2]) * w.istd[2], (x4[3] - w.mean[3]) * w.istd[3]);
-    bf16x4 h1b[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma(w.w1[m], xb, acc);
-        float g[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const float z = acc[v] + w.b1[m][v];
-            g[v] = z * gelu_sig(z);
-        }
-        h1b[m] = pack4(g[0], g[1], g[2], g[3]);
-    }
-    float ys = 0.f;
-#pragma unroll
-    for (int m2 = 0; m2 < 4; m2++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kc = 0; kc < 4; kc++) acc = mfma(w.w2[m2][kc], h1b[kc], acc);
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const float z = acc[v] + w.b2[m2][v];
-            ys += w.w3[m2][v] * (z * gelu_sig(z));
-        }
-    }
-    ys += __shfl_xor(ys, 16, 64);
-    ys += __shfl_xor(ys, 32, 64);
-    return ys + w.b3;
-}
-
 // ---------------------------------------------------------------------------
 // inference: prediction_i = y_i * out_scale + resid * x_i[PPREV] (target
 // units).  Solver path (clamp = 1): out_i = scale * alpha * clamp(prediction,
 // 0, pcap); raw path (clamp = 0): out_i = scale * prediction.
 // Features of sample i, feature f at x[f * fstride + i * sstride].
+// Each wave keeps the MFMA weight operands in registers (biases and the
+// output layer in LDS) and runs two 16-sample tiles at a time, so the MFMA
+// and GELU chains of one tile overlap those of the other.
 // ---------------------------------------------------------------------------
+template <int T>
+__device__ __forceinline__ void forward_infer_tiles(const bf16x4 (&w1)[4], const bf16x4 (&w2)[4][4],
+                                                    const float *sb1, const float *sb2,
+                                                    const float *sw3, float b3, const float (&x)[T][4],
+                                                    const float (&mean)[4], const float (&istd)[4],
+                                                    int q, float (&y)[T]) {
+    bf16x4 xb[T];
+#pragma unroll
+    for (int t = 0; t < T; t++)
+        xb[t] = pack4((x[t][0] - mean[0]) * istd[0], (x[t][1] - mean[1]) * istd[1],
+                      (x[t][2] - mean[2]) * istd[2], (x[t][3] - mean[3]) * istd[3]);
+    bf16x4 h1b[T][4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const float4 b = *reinterpret_cast<const float4 *>(sb1 + 16 * m + 4 * q);
+        f32x4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) acc[t] = mfma(w1[m], xb[t], f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const float z0 = acc[t][0] + b.x, z1 = acc[t][1] + b.y, z2 = acc[t][2] + b.z,
+                        z3 = acc[t][3] + b.w;
+            h1b[t][m] = pack4(z0 * gelu_sig(z0), z1 * gelu_sig(z1), z2 * gelu_sig(z2), z3 * gelu_sig(z3));
+        }
+    }
+    float ys[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) ys[t] = 0.f;
+#pragma unroll
+    for (int m2 = 0; m2 < 4; m2++) {
+        const float4 b = *reinterpret_cast<const float4 *>(sb2 + 16 * m2 + 4 * q);
+        const float4 o = *reinterpret_cast<const float4 *>(sw3 + 16 * m2 + 4 * q);
+        f32x4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++)
+#pragma unroll
+            for (int t = 0; t < T; t++) acc[t] = mfma(w2[m2][kc], h1b[t][kc], acc[t]);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const float z0 = acc[t][0] + b.x, z1 = acc[t][1] + b.y, z2 = acc[t][2] + b.z,
+                        z3 = acc[t][3] + b.w;
+            ys[t] += o.x * (z0 * gelu_sig(z0)) + o.y * (z1 * gelu_sig(z1)) +
+                     o.z * (z2 * gelu_sig(z2)) + o.w * (z3 * gelu_sig(z3));
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        float v = ys[t];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        y[t] = v + b3;
+    }
+}
+
 __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restrict__ P,
                                                            const float *__restrict__ x, int n,
                                                            int fstride, int sstride, float scale,
                                                            int clamp, float *__restrict__ out) {
+    __shared__ __align__(16) float sb1[kH];
+    __shared__ __align__(16) float sb2[kH];
+    __shared__ __align__(16) float sw3[kH];
+    if (threadIdx.x < kH) {
+        sb1[threadIdx.x] = P[OFF_B1 + threadIdx.x];
+        sb2[threadIdx.x] = P[OFF_B2 + threadIdx.x];
+        sw3[threadIdx.x] = P[OFF_W3 + threadIdx.x];
+    }
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * kWaves;
-    Weights w;
-    load_weights(P, w, lane);
+    const int q = lane >> 4, j = lane & 15, r = lane & 15;
+    bf16x4 w1[4], w2[4][4];
+    float mean[4], istd[4];
+    for (int m = 0; m < 4; m++) {
+        const float *row = P + OFF_W1 + (16 * m + r) * kF + 4 * q;
+        w1[m] = pack4(row[0], row[1], row[2], row[3]);
+        for (int kc = 0; kc < 4; kc++) {
+            const float *r2 = P + OFF_W2 + (16 * m + r) * kH + 16 * kc + 4 * q;
+            w2[m][kc] = pack4(r2[0], r2[1], r2[2], r2[3]);
+        }
+    }
+    for (int v = 0; v < 4; v++) {
+        mean[v] = P[OFF_MEAN + 4 * q + v];
+        istd[v] = P[OFF_ISTD + 4 * q + v];
+    }
+    const float b3 = P[OFF_B3];
     const float oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA], pcap = P[OFF_PCAP], resid = P[OFF_RESID];
-    const int q = lane >> 4, j = lane & 15;
+    __syncthreads();
     const int tiles = (n + 15) / 16;
-    for (int t = wave; t < tiles; t += nwaves) {
-        const int i = t * 16 + j;
-        float x4[4] = {0.f, 0.f, 0.f, 0.f};
-        if (i < n)
-            for (int v = 0; v < 4; v++) x4[v] = x[(size_t)(4 * q + v) * fstride + (size_t)i * sstride];
-        const float yn = forward_infer(w, x4);
-        if (lane < 16 && i < n) {
-            float y = yn * oscale;
-            if (resid != 0.f) y += resid * x[(size_t)BBX_F_PPREV * fstride + (size_t)i * sstride];
-            if (clamp) y = alpha * fminf(fmaxf(y, 0.f), pcap);
-            out[i] = y * scale;
+    for (int t0 = 2 * wave; t0 < tiles; t0 += 2 * nwaves) {
+        float xv[2][4];
+        int idx[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            idx[t] = (t0 + t) * 16 + j;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                xv[t][v] = idx[t] < n ? x[(size_t)(4 * q + v) * fstride + (size_t)idx[t] * sstride] : 0.f;
+        }
+        float yn[2];
+        forward_infer_tiles<2>(w1, w2, sb1, sb2, sw3, b3, xv, mean, istd, q, yn);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int i = idx[t];
+            if (lane < 16 && i < n) {
+                float y = yn[t] * oscale;
+                if (resid != 0.f) y += resid * x[(size_t)BBX_F_PPREV * fstride + (size_t)i * sstride];
+                if (clamp) y = alpha * fminf(fmaxf(y, 0.f), pcap);
+                out[i] = y * scale;
+            }
         }
     }
 }
@@ -426,9 +483,11 @@ static int ensure_gpart(bbx_mlp *m, int waves) {
 static int infer_launch(bbx_mlp *m, const float *x, int n, int fstride, int sstride, float scale,
                         int clamp0, float *out, hipStream_t st) {
     if (n <= 0) return BBX_OK;
+    // two tiles per wave per pass; about four passes per wave at the largest
+    // sizes so the per-wave weight load is amortised
     const int tiles = (n + 15) / 16;
-    int blocks = (tiles + kWaves - 1) / kWaves;
-    if (blocks > 4096) blocks = 4096;
+    int blocks = (tiles + 2 * kWaves - 1) / (2 * kWaves);
+    if (blocks > 1024) blocks = 1024;
     k_mlp_infer<<<blocks, 64 * kWaves, 0, st>>>(m->params, x, n, fstride, sstride, scale, clamp0, out);
     BBX_HIP_CHECK(hipGetLastError());
     return BBX_OK;
