@@ -270,7 +270,8 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     __shared__ int rdelta[9];
     __shared__ int smin, smax;
     __shared__ int row_a[9], row_b[9];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int tile = tile_of_block();
+    const int i = tile * kBlock + threadIdx.x;
     const bool valid = i < c.n;
     const int hc = valid ? home[i] : -1;
     if (threadIdx.x == 0) {
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     __syncthreads();
     if (threadIdx.x == 0) tab_merge(row_a, row_b, c.tile_cap, tt, rdelta);
     __syncthreads();
-    tab_store(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
+    tab_store(tile_tab + (size_t)tile * kTabInts, tt);
     const bool staged = tt.staged != 0;
     if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
     __syncthreads();
@@ -433,12 +434,13 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
                                                     float *__restrict__ rho) {
     __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
-    tab_load_shared(tile_tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(tile_tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
     if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = pos[i];
     float sum = 0.f, comp = 0.f;

@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     __shared__ float4 sx[kTileCap2];
     __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
-    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;                  // buckets hold LDS-local indices
     const bool in_lds = staged && tt.total <= c.tile_cap2;  // and they fit this kernel's LDS
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
         });
     }
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const int self = staged ? tab_local(tt, i) : i;
     const float4 xi = pos[i];
@@ -208,12 +209,13 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kBlock / 64];
     if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
-    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
     if (staged) tab_stage(tt, sx, [&](int j) { return xstar[j]; });
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     unsigned long long ebits = 0ull;
     if (i < c.n) {
         const float4 xi = xstar[i];
@@ -305,7 +307,8 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             if (integrate) loop->done[k] = 1;
         }
     }
-    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
     if (staged)
@@ -314,7 +317,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             return make_float4(p.x, p.y, p.z, q[j]);
         });
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     unsigned fbits = 0u;
     if (i < c.n) {
         const int self = staged ? tab_local(tt, i) : i;
@@ -374,7 +377,8 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     __shared__ float4 sx[kTileCap2];
     __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
-    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
     const bool in_lds = staged && tt.total <= c.tile_cap2;
@@ -392,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
         tab_stage(tt, sv, vr);
     }
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const int self = staged ? tab_local(tt, i) : i;
     const float4 xi = pos[i], vi = vel[i];
@@ -465,7 +469,8 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     __shared__ float4 sx[kTileCap2];
     __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
-    tab_load_shared(nb.tab + (size_t)blockIdx.x * kTabInts, tt);
+    const int tile = tile_of_block();
+    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
     const bool in_lds = staged && tt.total <= c.tile_cap2;
@@ -478,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
         tab_stage(tt, sv, vr);
     }
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const float4 xi = pos[i], vi = vel[i];
     float sx_ = 0.f, sy_ = 0.f, sz_ = 0.f, ws = 0.f;

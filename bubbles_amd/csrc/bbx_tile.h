@@ -23,6 +23,19 @@ constexpr int kTileCap = kTileCapHost;  // staged particles per tile
 // reads its neighbours from global memory through tab_global.
 constexpr int kTileCap2 = kTileCap2Host;
 constexpr int kMaxIvl = 9;
+
+// XCD-aware tile order.  Workgroups go to the chip's eight XCDs in
+// round-robin order of blockIdx, and each XCD has its own L2.  Workgroup b
+// runs tile start(x) + b / 8 of XCD x = b % 8, so every XCD works through a
+// contiguous run of tiles and neighbouring tiles, which stage the same
+// neighbour rows, meet in the same L2.  A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int tile_of_block() {
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, k = b / kXcds;
+    return x * q + min(x, r) + k;
+}
 constexpr int kTabInts = kTabIntsHost;  // per-tile table stride in ints
 
 // per-tile table, stored in global memory with the same 30-int layout

@@ -85,7 +85,7 @@ OFF_OSCALE = OFF_ISTD + 16
 OFF_ALPHA, OFF_PCAP, OFF_RESID = OFF_OSCALE + 1, OFF_OSCALE + 2, OFF_OSCALE + 3
 
 
-def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0, alpha=0.8):
+def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0, alpha=0.7):
     """Train the network on the residual t - p_prev (the previous substep's
     converged pressure is the strongest single predictor); the solver then
     starts from alpha * clamp(p_prev + net, 0, pcap).  Under-estimating is
@@ -173,6 +173,9 @@ def main(argv=None):
     tr.add_argument("--steps", type=int, default=20000)
     tr.add_argument("--batch", type=int, default=4096)
     tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--harvest-max-it", type=int, default=32,
+                    help="iteration cap of the plain runs that produce the targets")
+    tr.add_argument("--alpha", type=float, default=0.7)
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--steps", type=int, default=480)
@@ -187,14 +190,16 @@ def main(argv=None):
         seeds = TRAIN_SEEDS[:4] if args.quick else TRAIN_SEEDS
         scale = 0.5 if args.quick else 1.0
         x, t = harvest(training_scenes(seeds, scale), steps=200 if args.quick else 400,
-                       per_snapshot=2048 if args.quick else 8192)
+                       per_snapshot=2048 if args.quick else 8192, max_it=args.harvest_max_it)
         t1 = time.perf_counter()
         print(f"harvested {x.shape[0]} samples in {t1 - t0:.1f} s; target mean {t.mean():.4f} std {t.std():.4f}")
-        net, loss = fit(x, t, steps=200 if args.quick else args.steps, batch=args.batch, lr=args.lr)
+        net, loss = fit(x, t, steps=200 if args.quick else args.steps, batch=args.batch, lr=args.lr,
+                        alpha=args.alpha)
         t2 = time.perf_counter()
         print(f"trained {args.steps} Adam steps in {t2 - t1:.1f} s, final batch loss {loss:.5f}")
         save(net, args.out, {"samples": int(x.shape[0]), "steps": args.steps, "batch": args.batch,
-                             "lr": args.lr, "seeds": seeds, "final_loss": loss})
+                             "lr": args.lr, "seeds": seeds, "final_loss": loss,
+                             "harvest_max_it": args.harvest_max_it, "alpha": args.alpha})
         print("saved", args.out)
     else:
         net = load(args.weights)
