@@ -46,6 +46,7 @@ KERNEL_BYTES = {
     "buckets_density": 16,             # x -> rho
     "nonpressure_predict": 56,         # x, v, rho -> f, x*, rho*
     "nonpressure_features": 108,       # x, v, rho -> f, rho*, 16 features
+    "nonpressure_features_warm": 120,  # x, v, rho -> f, rho*, 16 features, p_warm, p, q
     "mlp_infer": 68,                   # 16 features -> p
     "predict_pressure": 28,            # x*, p -> p, rho*, q
     "pressure_force_warm": 52,         # x, q, v, f -> x*
@@ -190,8 +191,8 @@ def run_cpu_baseline(sc, args):
     return {"value": round(sc.n * done / t, 1), "unit": "particle-steps/s", "cores": threads,
             "cpu_model": cpu_model,
             "kind": "port",
-            "sample": f"{done} timed substeps (after 1 warm-up) of the same {sc.n}-particle dam "
-                      f"break, plain PCISPH, FP64 oracle with linked-list cells and 100-slot "
+            "sample": f"{done} timed substeps (after 1 warm-up) of the same {sc.n}-particle "
+                      f"{sc.name}, plain PCISPH, FP64 oracle with linked-list cells and 100-slot "
                       f"buckets, {args.mode} loop, max {args.max_it} iterations, {threads} "
                       f"threads, {t:.1f} s"}
 
@@ -217,6 +218,8 @@ def main():
     ap.add_argument("--tol", type=float, default=0.01)
     ap.add_argument("--tol-max-it", type=int, default=64,
                     help="iteration cap of the to-tolerance pair (0: skip)")
+    ap.add_argument("--scene", choices=["dam_break", "block_drop"], default="dam_break",
+                    help="block_drop: the 8,388,608-particle cube-onto-layer scene (dt 2.5e-4)")
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=128)
@@ -231,7 +234,14 @@ def main():
 
     from bubbles_amd import abi, scenes
 
-    sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+    if args.scene == "block_drop":
+        sc = scenes.block_drop()
+        workload = (f"PCISPH block drop {sc.n} particles (64^3 cube onto a 512x32x496 layer at "
+                    f"0.005, h 0.01, 4x2x4 tank), dt 2.5e-4")
+    else:
+        sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+        workload = (f"PCISPH dam break {sc.n} particles ({args.nx}x{args.ny}x{args.nz} "
+                    f"at 0.01, h 0.02, 2x1x2 tank), dt 5e-4")
     runs = {}
     if args.warm in ("both", "none"):
         runs["plain"] = run_variant(args, sc, False, args.max_it, profile=args.warm == "none")
@@ -252,8 +262,7 @@ def main():
     kbar = head["mean_iterations"]
     step_bytes = substep_bytes(kbar)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(sc, args)
-    cfg = {"workload": f"PCISPH dam break {sc.n} particles ({args.nx}x{args.ny}x{args.nz} "
-                       f"at 0.01, h 0.02, 2x1x2 tank), dt 5e-4",
+    cfg = {"workload": workload,
            "n": sc.n, "mode": args.mode, "warm_start": "learned" if "learned" in runs else "none",
            "max_it": args.max_it, "tol": args.tol,
            "mean_iterations": round(kbar, 3),

@@ -522,7 +522,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->counters, CNT_N);
     if (!rc && hipHostMalloc((void **)&s->h_loop, sizeof(LoopState)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * CNT_N) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * (CNT_N + 1)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned) * kSlots) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
@@ -1028,37 +1028,48 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     st.dt_over_m = (float)(dt / s->mass);
     st.delta = (float)delta;
     if (warm == BBX_WARM_MLP) {
-        BBX_CHECK(run_nonpressure(s, st, 0, 1));
-        {
-            BBX_PROF_BEGIN("mlp_infer");
-            BBX_CHECK(mlp_infer_device(s->mlp, s->feat, s->n, s->stride,
-                                       (float)(delta * s->prm.target_density), s->pwarm, s->stream));
-            BBX_PROF_END();
-        }
-        BBX_CHECK(run_warm_init(s));
+        // features, network inference and p = p_warm, rho* = rho in one gather
+        st.warm_scale = (float)(delta * s->prm.target_density);
+        BBX_CHECK(run_nonpressure(s, st, 0, 2));
         // f_p(p_warm, rho) and the k = 0 prediction
         BBX_CHECK(run_pressure_force(s, st, -1, max_it, 0, tol));
     } else {
         BBX_CHECK(run_nonpressure(s, st, 1, 0));
     }
-    // The whole predict-correct loop is queued without host round trips:
-    // the pressure-force kernel of iteration k reads that iteration's FP64
-    // error on the device and either integrates or predicts iteration k + 1;
-    // kernels of later iterations exit at entry.  REFERENCE_EXACT stops after
+    // The predict-correct loop is queued without per-iteration host round
+    // trips: the pressure-force kernel of iteration k reads that iteration's
+    // FP64 error on the device and either integrates or predicts iteration
+    // k + 1; kernels of later iterations exit at entry.  The host queues as
+    // many iterations as the previous substep needed plus one; only when
+    // those have not integrated does it wait once and queue more, so a high
+    // cap costs nothing while it is not reached.  REFERENCE_EXACT stops after
     // the first iteration (its error is never recorded, so it never converges
     // earlier and never runs longer).
     const int exact = mode == BBX_MODE_REFERENCE_EXACT ? 1 : 0;
-    const int launches = exact ? 1 : max_it;
-    for (int k = 0; k < launches; k++) {
-        st.first = (k == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
-        BBX_CHECK(run_predict_pressure(s, st, k));
-        BBX_CHECK(run_pressure_force(s, st, k, max_it, exact, tol));
+    const int cap = exact ? 1 : max_it;
+    int launches = 0;
+    int goal = s->last_iterations > 0 ? s->last_iterations + 1 : 2;
+    for (;;) {
+        goal = goal < cap ? goal : cap;
+        for (int k = launches; k < goal; k++) {
+            st.first = (k == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
+            BBX_CHECK(run_predict_pressure(s, st, k));
+            BBX_CHECK(run_pressure_force(s, st, k, max_it, exact, tol));
+        }
+        launches = goal;
+        if (launches >= cap) break;
+        BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters + CNT_N, &s->loop->done[launches - 1], sizeof(int),
+                                     hipMemcpyDeviceToHost, s->stream));
+        BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (s->h_counters[CNT_N]) break;
+        goal = launches + (launches > 4 ? launches / 2 : 2);
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
     BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
     bbx_stats local{};
     BBX_CHECK(read_stats(s, &local, launches));
+    s->last_iterations = local.iterations;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
     local.gpu_ms = ms;

@@ -95,6 +95,7 @@ struct StepConst {
     float dt, dt_over_m;
     float delta;       // PCISPH stiffness (FP64 on host, rounded for the update)
     int first;         // k == 0 without warm start: pressure starts at 0
+    float warm_scale;  // delta * rho0: warm-start network output -> pressure
 };
 
 // predict-correct loop state on the device: per-iteration FP64 max error
@@ -174,6 +175,7 @@ struct bbx_solver {
     int stride = 0;
     double mass = 1e-3, kr = 0, radius = 1e-3, delta_denom = 0, mass_over_rho2 = 0;
     double delta64 = 0;   // PCISPH delta of the current substep (FP64)
+    int last_iterations = 0;  // iterations of the previous PCISPH substep
     bbx::GridGeom grid{};
     bbx::SimConst sc{};
     hipStream_t stream = nullptr;
@@ -214,7 +216,7 @@ struct bbx_solver {
     int *counters = nullptr;                 // CNT_*
     // pinned readback
     unsigned long long *h_max_err = nullptr;
-    int *h_counters = nullptr;
+    int *h_counters = nullptr;               // CNT_N counters + one loop flag
     unsigned *h_max_force = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -234,6 +236,8 @@ int launch_fill_int(bbx_solver *s, int *p, int value, size_t n);
 int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
 int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure);
 int run_density_only(bbx_solver *s);
+// features: 0 none, 1 warm-start features, 2 features + network inference +
+// the warm k = 0 initialisation in the same kernel
 int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features);
 int run_predict_pressure(bbx_solver *s, const StepConst &st, int k);
 // k < 0: predict only (warm start); otherwise integrate when iteration k is
@@ -242,10 +246,8 @@ int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, in
 int run_sph_force(bbx_solver *s);
 int run_integrate(bbx_solver *s, const StepConst &st);
 int run_xsph(bbx_solver *s, const StepConst &st);
-int run_warm_init(bbx_solver *s);
 int swap_after_integrate(bbx_solver *s);
-int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float scale, float *out,
-                     hipStream_t stream);
+const float *mlp_params_device(const bbx_mlp *m);
 void mlp_retain(bbx_mlp *m);
 void mlp_release(bbx_mlp *m);
 }  // namespace bbx

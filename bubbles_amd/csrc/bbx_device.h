@@ -48,7 +48,8 @@ __device__ __forceinline__ void unlinear(const GridGeom &g, int id, int u[3]) {
 // IsWithinStd(Distance(pi, pj), h) in FP64 (src/core/kernel.cpp:229-234,
 // src/core/geometry.h:632-633): d = sqrt(dx^2+dy^2+dz^2), d2 = d*d,
 // inside iff !(|d2 - h2| < 1e-8 || d2 - h2 > 0).
-__device__ __forceinline__ bool within_std_fp64(float4 a, float4 b, double h2) {
+// Not inlined: only pairs inside the FP32 band around the cutoff reach it.
+__device__ __attribute__((noinline)) bool within_std_fp64(float4 a, float4 b, double h2) {
 #pragma clang fp contract(off)
     double dx = (double)a.x - (double)b.x;
     double dy = (double)a.y - (double)b.y;

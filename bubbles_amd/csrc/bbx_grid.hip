@@ -298,131 +298,167 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
     __syncthreads();
     if (!valid) return;
-    const float4 pi = pos[i];
-    int u[3] = {0, 0, 0};
-    if (hc >= 0) unlinear(c.g, hc, u);
-    int cnt = 0, dropped = 0;
-    float sum = 0.f, comp = 0.f;
-    // the current quad of bucket entries, stored once complete
-    unsigned e0 = 0, e1 = 0, e2 = 0, e3 = 0;
-    auto flush_quad = [&](int k) {  // k = index of the quad's last entry + 1
-        const size_t at = (size_t)((k - 1) >> 2) * c.stride + i;
-        if (staged)
-            reinterpret_cast<ushort4 *>(nbr16)[at] =
-                make_ushort4((unsigned short)e0, (unsigned short)e1, (unsigned short)e2,
-                             (unsigned short)e3);
-        else
-            reinterpret_cast<int4 *>(nbr32)[at] = make_int4((int)e0, (int)e1, (int)e2, (int)e3);
-    };
-    const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
-    // append accepted candidate j (storage index; LDS index j + sh) to the
-    // bucket, up to the cap, and to the density sum
-    auto emit = [&](int j, int sh) {
-        if (cnt >= kBucket) {
-            dropped++;
-            return;
-        }
-        const int jl = j + sh;
-        const float4 pj = staged ? sx[jl] : pos[j];
-        const unsigned v = staged ? (unsigned)jl : (unsigned)j;
-        const int slot = cnt & 3;
-        e0 = slot == 0 ? v : e0;
-        e1 = slot == 1 ? v : e1;
-        e2 = slot == 2 ? v : e2;
-        e3 = slot == 3 ? v : e3;
-        cnt++;
-        if (slot == 3) flush_quad(cnt);
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
-    };
-    if (hc >= 0) {
-        // Row r = (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are
-        // contiguous in storage.  B[r] = start of x-1, x, x+1 and end of x+1;
-        // cells and rows outside the grid are empty ranges.
-        int B[9][4], sh[9];
-        bool narrow = true;
-#pragma unroll
-        for (int r = 0; r < 9; r++) {
-            const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
-            sh[r] = staged ? rdelta[r] : 0;
-            B[r][0] = B[r][1] = B[r][2] = B[r][3] = 0;
-            if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-                const int cx = u[0] + yy * nx + zz * nxy;
-                const int b1 = start[cx], b2 = start[cx + 1];
-                B[r][0] = u[0] > 0 ? start[cx - 1] : b1;
-                B[r][1] = b1;
-                B[r][2] = b2;
-                B[r][3] = u[0] + 1 < nx ? start[cx + 2] : b2;
-                narrow = narrow && (B[r][3] - B[r][0] <= c.row_max);
+    // The particle's work is instantiated once for LDS-staged and once for
+    // global-memory tiles, so no load selects between the two address spaces
+    // (that would compile to a slower generic load).
+    auto per_particle = [&](auto staged_tag) {
+        constexpr bool kStaged = decltype(staged_tag)::value;
+        const float4 pi = pos[i];
+        int u[3] = {0, 0, 0};
+        if (hc >= 0) unlinear(c.g, hc, u);
+        int cnt = 0, dropped = 0;
+        float sum = 0.f, comp = 0.f;
+        // the current quad of bucket entries, stored once complete
+        unsigned e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+        auto flush_quad = [&](int k) {  // k = index of the quad's last entry + 1
+            const size_t at = (size_t)((k - 1) >> 2) * c.stride + i;
+            if (kStaged)
+                reinterpret_cast<ushort4 *>(nbr16)[at] =
+                    make_ushort4((unsigned short)e0, (unsigned short)e1, (unsigned short)e2,
+                                 (unsigned short)e3);
+            else
+                reinterpret_cast<int4 *>(nbr32)[at] = make_int4((int)e0, (int)e1, (int)e2, (int)e3);
+        };
+        const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
+        // append accepted candidate j (storage index; LDS index j + sh) to the
+        // bucket, up to the cap, and to the density sum
+        auto emit = [&](int j, int sh) {
+            if (cnt >= kBucket) {
+                dropped++;
+                return;
             }
-        }
-        if (narrow) {
-            // pass 1: each row's candidates tested in storage order into a mask
-            unsigned long long M[9];
-#pragma unroll
+            const int jl = j + sh;
+            float4 pj;
+            if (kStaged)
+                pj = sx[jl];
+            else
+                pj = pos[j];
+            const unsigned v = kStaged ? (unsigned)jl : (unsigned)j;
+            const int slot = cnt & 3;
+            e0 = slot == 0 ? v : e0;
+            e1 = slot == 1 ? v : e1;
+            e2 = slot == 2 ? v : e2;
+            e3 = slot == 3 ? v : e3;
+            cnt++;
+            if (slot == 3) flush_quad(cnt);
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+        };
+        if (hc >= 0) {
+            // Row r = (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are
+            // contiguous in storage.  B[r] = start of x-1, x, x+1 and end of x+1;
+            // cells and rows outside the grid are empty ranges.
+            int B[9][4], sh[9];
+            bool narrow = true;
+    #pragma unroll
             for (int r = 0; r < 9; r++) {
-                unsigned long long m = 0ull;
-                const int a = B[r][0], len = B[r][3] - B[r][0], s = sh[r];
-                for (int t0 = 0; t0 < len; t0 += 4) {
-                    float4 pc[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if (t0 + q < len) pc[q] = staged ? sx[a + t0 + q + s] : pos[a + t0 + q];
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
+                const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
+                sh[r] = kStaged ? rdelta[r] : 0;
+                B[r][0] = B[r][1] = B[r][2] = B[r][3] = 0;
+                if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                    const int cx = u[0] + yy * nx + zz * nxy;
+                    const int b1 = start[cx], b2 = start[cx + 1];
+                    B[r][0] = u[0] > 0 ? start[cx - 1] : b1;
+                    B[r][1] = b1;
+                    B[r][2] = b2;
+                    B[r][3] = u[0] + 1 < nx ? start[cx + 2] : b2;
+                    narrow = narrow && (B[r][3] - B[r][0] <= c.row_max);
                 }
-                M[r] = m;
             }
-            // pass 2: emit in the reference's order, cells (y, x, z), chain order inside
-#pragma unroll
-            for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++)
-#pragma unroll
-                    for (int dz = 0; dz < 3; dz++) {
-                        const int r = dz * 3 + dy;
-                        const int w = B[r][dx + 1] - B[r][dx];
-                        if (w <= 0) continue;
-                        unsigned long long m = M[r] >> (B[r][dx] - B[r][0]);
-                        if (w < 64) m &= (1ull << w) - 1ull;
-                        while (m) {
-                            const int b = __builtin_ctzll(m);
-                            m &= m - 1ull;
-                            emit(B[r][dx] + b, sh[r]);
+            if (narrow) {
+                // pass 1: each row's candidates tested in storage order into a mask
+                unsigned long long M[9];
+    #pragma unroll
+                for (int r = 0; r < 9; r++) {
+                    unsigned long long m = 0ull;
+                    const int a = B[r][0], len = B[r][3] - B[r][0], s = sh[r];
+                    // separate LDS and global loops: a select between the two
+                    // addresses would become a generic (flat) load
+                    if (kStaged) {
+                        const float4 *row = sx + a + s;
+                        for (int t0 = 0; t0 < len; t0 += 4) {
+                            float4 pc[4];
+    #pragma unroll
+                            for (int q = 0; q < 4; q++)
+                                if (t0 + q < len) pc[q] = row[t0 + q];
+    #pragma unroll
+                            for (int q = 0; q < 4; q++)
+                                if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
+                        }
+                    } else {
+                        const float4 *row = pos + a;
+                        for (int t0 = 0; t0 < len; t0 += 4) {
+                            float4 pc[4];
+    #pragma unroll
+                            for (int q = 0; q < 4; q++)
+                                if (t0 + q < len) pc[q] = row[t0 + q];
+    #pragma unroll
+                            for (int q = 0; q < 4; q++)
+                                if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
                         }
                     }
-        } else {
-            // a row with more than 64 candidates: cell by cell
-            for (int dy = 0; dy < 3; dy++)
-                for (int dx = 0; dx < 3; dx++) {
-                    const int x = u[0] + dx - 1;
-                    if (x < 0 || x >= nx) continue;
-                    for (int dz = 0; dz < 3; dz++) {
-                        const int yy = u[1] + dy - 1, zz = u[2] + dz - 1;
-                        if (yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
-                        const int cell = x + yy * nx + zz * nxy;
-                        const int sr = staged ? rdelta[dz * 3 + dy] : 0;
-                        const int b = start[cell], e = b + count[cell];
-                        for (int j = b; j < e; j++) {
-                            const float4 pj = staged ? sx[j + sr] : pos[j];
-                            if (in_bucket(c, pi, pj)) emit(j, sr);
-                        }
-                    }
+                    M[r] = m;
                 }
+                // pass 2: emit in the reference's order, cells (y, x, z), chain
+                // order inside each cell.  x runs as a loop with its cell bounds
+                // selected, which keeps the kernel's code (nine emission sites
+                // instead of 27) small.
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll 1
+                    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                        for (int dz = 0; dz < 3; dz++) {
+                            const int r = dz * 3 + dy;
+                            const int lo = dx == 0 ? B[r][0] : (dx == 1 ? B[r][1] : B[r][2]);
+                            const int hi = dx == 0 ? B[r][1] : (dx == 1 ? B[r][2] : B[r][3]);
+                            const int w = hi - lo;
+                            if (w <= 0) continue;
+                            unsigned long long m = M[r] >> (lo - B[r][0]);
+                            if (w < 64) m &= (1ull << w) - 1ull;
+                            while (m) {
+                                const int b = __builtin_ctzll(m);
+                                m &= m - 1ull;
+                                emit(lo + b, sh[r]);
+                            }
+                        }
+            } else {
+                // a row with more than 64 candidates: cell by cell
+                for (int dy = 0; dy < 3; dy++)
+                    for (int dx = 0; dx < 3; dx++) {
+                        const int x = u[0] + dx - 1;
+                        if (x < 0 || x >= nx) continue;
+                        for (int dz = 0; dz < 3; dz++) {
+                            const int yy = u[1] + dy - 1, zz = u[2] + dz - 1;
+                            if (yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+                            const int cell = x + yy * nx + zz * nxy;
+                            const int sr = kStaged ? rdelta[dz * 3 + dy] : 0;
+                            const int b = start[cell], e = b + count[cell];
+                            for (int j = b; j < e; j++) {
+                                float4 pj;
+                                if (kStaged)
+                                    pj = sx[j + sr];
+                                else
+                                    pj = pos[j];
+                                if (in_bucket(c, pi, pj)) emit(j, sr);
+                            }
+                        }
+                    }
+            }
         }
-    }
-    if (cnt & 3) flush_quad(cnt);
-    nbr_cnt[i] = cnt;
-    float di = sum * c.mass;
-    rho[i] = di;
-    if (kPressure) {
-        // ComputePressureValue (src/equations/sph_equations3.cpp:7-18)
-        float p = c.eos_scale / c.eos_exp * (powf(di / c.rho0, c.eos_exp) - 1.f);
-        if (p < 0.f) p *= c.nps;
-        pres[i] = p;
-    }
-    if (dropped) atomicAdd(&counters[CNT_OVERFLOW], dropped);
+        if (cnt & 3) flush_quad(cnt);
+        nbr_cnt[i] = cnt;
+        float di = sum * c.mass;
+        rho[i] = di;
+        if (kPressure) {
+            // ComputePressureValue (src/equations/sph_equations3.cpp:7-18)
+            float p = c.eos_scale / c.eos_exp * (powf(di / c.rho0, c.eos_exp) - 1.f);
+            if (p < 0.f) p *= c.nps;
+            pres[i] = p;
+        }
+        if (dropped) atomicAdd(&counters[CNT_OVERFLOW], dropped);
+    };
+    with_staging(staged, per_particle);
 }
 
 // density over the existing buckets (after an upload of positions)
@@ -445,10 +481,13 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
     const float4 pi = pos[i];
     float sum = 0.f, comp = 0.f;
     const int cnt = nbr_cnt[i];
-    for_each_neighbour<2>(nbr16, nbr32, staged, i, cnt, c.stride, [&](int j) {
-        const float4 pj = staged ? sx[j] : pos[j];
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+    with_staging(staged, [&](auto tag) {
+        constexpr bool kS = decltype(tag)::value;
+        for_each_neighbour<2>(nbr16, nbr32, kS, i, cnt, c.stride, [&](int j) {
+            const float4 pj = kS ? sx[j] : pos[j];
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+        });
     });
     rho[i] = sum * c.mass;
 }

@@ -13,66 +13,12 @@
 // layer is therefore the B layout of the next, so activations never leave
 // registers in the forward pass.  Weight gradients contract over samples;
 // their operands are transposed through a per-wave LDS tile.
-#include "bbx_common.h"
-#include "bbx_device.h"
+#include "bbx_mlp_device.h"
 
 namespace bbx {
 
-constexpr int kH = BBX_MLP_HIDDEN;  // 64
-constexpr int kF = BBX_NFEAT;       // 16
-constexpr int OFF_W1 = 0;                 // [64][16]
-constexpr int OFF_B1 = OFF_W1 + kH * kF;  // [64]
-constexpr int OFF_W2 = OFF_B1 + kH;       // [64][64]
-constexpr int OFF_B2 = OFF_W2 + kH * kH;  // [64]
-constexpr int OFF_W3 = OFF_B2 + kH;       // [64]
-constexpr int OFF_B3 = OFF_W3 + kH;       // [1]
-constexpr int kTrainable = OFF_B3 + 1;    // 5313
-constexpr int OFF_MEAN = kTrainable;      // feature mean [16]
-constexpr int OFF_ISTD = OFF_MEAN + kF;   // feature 1/std [16]
-constexpr int OFF_OSCALE = OFF_ISTD + kF; // target scale [1]
-constexpr int OFF_ALPHA = OFF_OSCALE + 1; // warm start = alpha * clamp(prediction, 0, pcap)
-constexpr int OFF_PCAP = OFF_ALPHA + 1;
-constexpr int OFF_RESID = OFF_PCAP + 1;   // prediction = net * out_scale + resid * x[PPREV]
-constexpr int kParams = OFF_RESID + 1;    // 5349
-
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kTilesPerWave = 4;  // training tiles per wave per step
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// BF16 round-to-nearest-even conversions (v_cvt_pk_bf16_f32 on gfx950)
-__device__ __forceinline__ unsigned short f2bf(float f) {
-    return __builtin_bit_cast(unsigned short, (__bf16)f);
-}
-__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
-
-__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
-    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-    const bf4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
-    return __builtin_bit_cast(bf16x4, v);
-}
-
-// GELU, tanh form: 0.5 z (1 + tanh(u)) = z s with s = 1 / (1 + exp(-2u)) and
-// u = sqrt(2/pi) (z + 0.044715 z^3); exp and the reciprocal are the hardware
-// approximations (about 1 ulp), far below the BF16 rounding that follows.
-__device__ __forceinline__ float gelu_sig(float z) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float u = k0 * (z + k1 * (z * z) * z);
-    return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
-}
-
-// value and derivative: gelu'(z) = s + 2 z s (1 - s) u'(z)
-__device__ __forceinline__ void gelu_fwd(float z, float &g, float &dg) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float sg = gelu_sig(z);
-    g = z * sg;
-    dg = sg + 2.f * z * sg * (1.f - sg) * k0 * (1.f + 3.f * k1 * (z * z));
-}
-
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
 
 // Weights in MFMA A layout, loaded once per wave.
 struct Weights {
@@ -153,62 +99,6 @@ __device__ __forceinline__ void forward_tile(const Weights &w, const float x4[4]
 // output layer in LDS) and runs two 16-sample tiles at a time, so the MFMA
 // and GELU chains of one tile overlap those of the other.
 // ---------------------------------------------------------------------------
-template <int T>
-__device__ __forceinline__ void forward_infer_tiles(const bf16x4 (&w1)[4], const bf16x4 (&w2)[4][4],
-                                                    const float *sb1, const float *sb2,
-                                                    const float *sw3, float b3, const float (&x)[T][4],
-                                                    const float (&mean)[4], const float (&istd)[4],
-                                                    int q, float (&y)[T]) {
-    bf16x4 xb[T];
-#pragma unroll
-    for (int t = 0; t < T; t++)
-        xb[t] = pack4((x[t][0] - mean[0]) * istd[0], (x[t][1] - mean[1]) * istd[1],
-                      (x[t][2] - mean[2]) * istd[2], (x[t][3] - mean[3]) * istd[3]);
-    bf16x4 h1b[T][4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        const float4 b = *reinterpret_cast<const float4 *>(sb1 + 16 * m + 4 * q);
-        f32x4 acc[T];
-#pragma unroll
-        for (int t = 0; t < T; t++) acc[t] = mfma(w1[m], xb[t], f32x4{0.f, 0.f, 0.f, 0.f});
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-            const float z0 = acc[t][0] + b.x, z1 = acc[t][1] + b.y, z2 = acc[t][2] + b.z,
-                        z3 = acc[t][3] + b.w;
-            h1b[t][m] = pack4(z0 * gelu_sig(z0), z1 * gelu_sig(z1), z2 * gelu_sig(z2), z3 * gelu_sig(z3));
-        }
-    }
-    float ys[T];
-#pragma unroll
-    for (int t = 0; t < T; t++) ys[t] = 0.f;
-#pragma unroll
-    for (int m2 = 0; m2 < 4; m2++) {
-        const float4 b = *reinterpret_cast<const float4 *>(sb2 + 16 * m2 + 4 * q);
-        const float4 o = *reinterpret_cast<const float4 *>(sw3 + 16 * m2 + 4 * q);
-        f32x4 acc[T];
-#pragma unroll
-        for (int t = 0; t < T; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kc = 0; kc < 4; kc++)
-#pragma unroll
-            for (int t = 0; t < T; t++) acc[t] = mfma(w2[m2][kc], h1b[t][kc], acc[t]);
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-            const float z0 = acc[t][0] + b.x, z1 = acc[t][1] + b.y, z2 = acc[t][2] + b.z,
-                        z3 = acc[t][3] + b.w;
-            ys[t] += o.x * (z0 * gelu_sig(z0)) + o.y * (z1 * gelu_sig(z1)) +
-                     o.z * (z2 * gelu_sig(z2)) + o.w * (z3 * gelu_sig(z3));
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        float v = ys[t];
-        v += __shfl_xor(v, 16, 64);
-        v += __shfl_xor(v, 32, 64);
-        y[t] = v + b3;
-    }
-}
-
 __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restrict__ P,
                                                            const float *__restrict__ x, int n,
                                                            int fstride, int sstride, float scale,
@@ -224,21 +114,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_infer(const float *__restri
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * kWaves;
-    const int q = lane >> 4, j = lane & 15, r = lane & 15;
+    const int q = lane >> 4, j = lane & 15;
     bf16x4 w1[4], w2[4][4];
     float mean[4], istd[4];
-    for (int m = 0; m < 4; m++) {
-        const float *row = P + OFF_W1 + (16 * m + r) * kF + 4 * q;
-        w1[m] = pack4(row[0], row[1], row[2], row[3]);
-        for (int kc = 0; kc < 4; kc++) {
-            const float *r2 = P + OFF_W2 + (16 * m + r) * kH + 16 * kc + 4 * q;
-            w2[m][kc] = pack4(r2[0], r2[1], r2[2], r2[3]);
-        }
-    }
-    for (int v = 0; v < 4; v++) {
-        mean[v] = P[OFF_MEAN + 4 * q + v];
-        istd[v] = P[OFF_ISTD + 4 * q + v];
-    }
+    load_infer_operands(P, lane, w1, w2, mean, istd);
     const float b3 = P[OFF_B3];
     const float oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA], pcap = P[OFF_PCAP], resid = P[OFF_RESID];
     __syncthreads();
@@ -517,12 +396,6 @@ static int grad_launch(bbx_mlp *m, const float *xs, const float *ts, int n, int 
     return BBX_OK;
 }
 
-int mlp_infer_device(bbx_mlp *m, const float *feat, int n, int stride, float scale, float *out,
-                     hipStream_t st) {
-    if (!m) return fail(BBX_E_INVALID, "no MLP");
-    // out = scale * max(0, MLP(features)); the solver passes scale = delta * rho0
-    return infer_launch(m, feat, n, stride, 1, scale, 1, out, st);
-}
 
 }  // namespace bbx
 
@@ -692,6 +565,7 @@ int bbx_mlp_train(bbx_mlp *m, const float *x, const float *t, int n, int steps, 
 }  // extern "C"
 
 namespace bbx {
+const float *mlp_params_device(const bbx_mlp *m) { return m ? m->params : nullptr; }
 void mlp_retain(bbx_mlp *m) {
     if (m) m->refs++;
 }

@@ -15,6 +15,7 @@
 //   pressure force + integrate       PredictPressureForceFor + AccumulateForcesFor + TimeIntegrationFor
 //
 // Arithmetic is FP32; density and pressure-force sums use Kahan compensation.
+#include "bbx_mlp_device.h"
 #include "bbx_tile.h"
 
 namespace bbx {
@@ -75,8 +76,10 @@ __device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float
 // ---------------------------------------------------------------------------
 // non-pressure force (src/equations/sph_equations3.cpp:80-110)
 // mode: 0 = store force only, 1 = + predict(k=0) with f_p = 0,
-//       2 = + warm-start features
-// LDS: neighbour position, (velocity, density)
+//       2 = + warm-start features, 3 = + features, warm-start network
+//       inference and the k = 0 initialisation (p = p_warm, rho* = rho)
+// LDS: neighbour position, (velocity, density); in mode 3 the same LDS then
+// holds the block's features for the MFMA inference.
 // ---------------------------------------------------------------------------
 template <int kMode>
 __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
@@ -86,7 +89,12 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
                                                         float4 *__restrict__ force,
                                                         float4 *__restrict__ xstar,
                                                         float *__restrict__ rho_star,
-                                                        float *__restrict__ feat) {
+                                                        float *__restrict__ feat,
+                                                        const float *__restrict__ P,
+                                                        float *__restrict__ pres,
+                                                        float *__restrict__ q,
+                                                        float *__restrict__ pwarm) {
+    constexpr bool kFeat = kMode >= 2;
     __shared__ float4 sx[kTileCap2];
     __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
@@ -104,91 +112,153 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     }
     __syncthreads();
     const int i = tile * kBlock + threadIdx.x;
-    if (i >= c.n) return;
-    const int self = staged ? tab_local(tt, i) : i;
-    const float4 xi = pos[i];
-    const float4 vi = vel[i];
-    const float coef = c.visc * c.mass2;
-    float fx = c.mass * c.gx - c.drag * vi.x;
-    float fy = c.mass * c.gy - c.drag * vi.y;
-    float fz = c.mass * c.gz - c.drag * vi.z;
-    const int cnt = nb.cnt[i];
-    float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
-        float4 xj, vj;
-        if (in_lds) {
-            xj = sx[j];
-            vj = sv[j];
-        } else {
-            const int g = staged ? tab_global(tt, j) : j;
-            xj = pos[g];
-            float4 v = vel[g];
-            vj = make_float4(v.x, v.y, v.z, rho[g]);
+    const bool valid = i < c.n;
+    if (kMode != 3 && !valid) return;  // mode 3 keeps the block whole for its barriers
+    float fv[BBX_NFEAT];
+    if (valid) {
+        const int self = staged ? tab_local(tt, i) : i;
+        const float4 xi = pos[i];
+        const float4 vi = vel[i];
+        const float coef = c.visc * c.mass2;
+        float fx = c.mass * c.gx - c.drag * vi.x;
+        float fy = c.mass * c.gy - c.drag * vi.y;
+        float fz = c.mass * c.gz - c.drag * vi.z;
+        const int cnt = nb.cnt[i];
+        float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
+        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+            float4 xj, vj;
+            if (in_lds) {
+                xj = sx[j];
+                vj = sv[j];
+            } else {
+                const int g = staged ? tab_global(tt, j) : j;
+                xj = pos[g];
+                float4 v = vel[g];
+                vj = make_float4(v.x, v.y, v.z, rho[g]);
+            }
+            const float rj = vj.w;
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            float w = 0.f, x = 0.f;
+            if (spiky_in(c, r)) {
+                x = 1.f - r * c.inv_h;
+                w = c.spiky_lap * x;
+            }
+            float s = coef * w / rj;
+            fx += (vj.x - vi.x) * s;
+            fy += (vj.y - vi.y) * s;
+            fz += (vj.z - vi.z) * s;
+            if (kFeat && j != self) {
+                float u = r * c.inv_h;
+                m1 += u;
+                m2 += u * u;
+                m3 += u * u * u;
+                cx -= dx * c.inv_h;
+                cy -= dy * c.inv_h;
+                cz -= dz * c.inv_h;
+                if (r > 0.f && x > 0.f) {
+                    // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
+                    float g = -c.spiky_grad * x * x / r;
+                    float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+                    divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
+                }
+            }
+        });
+        force[i] = make_float4(fx, fy, fz, 0.f);
+        if (kMode == 1) {
+            xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
+            rho_star[i] = rho[i];
         }
-        const float rj = vj.w;
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        float w = 0.f, x = 0.f;
-        if (spiky_in(c, r)) {
-            x = 1.f - r * c.inv_h;
-            w = c.spiky_lap * x;
+        if (kFeat) {
+            const size_t S = (size_t)c.stride;
+            float nn = cnt > 1 ? (float)(cnt - 1) : 1.f;
+            float inv = 1.f / nn;
+            float wall = 3.4e38f;
+            const float p3[3] = {xi.x, xi.y, xi.z};
+            for (int a = 0; a < 3; a++) {
+                float lo = p3[a] - (c.tank_c[a] - c.tank_h[a]);
+                float hi = (c.tank_c[a] + c.tank_h[a]) - p3[a];
+                wall = fminf(wall, fminf(lo, hi));
+            }
+            float vn = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
+            float fn = sqrtf(fx * fx + fy * fy + fz * fz);
+            float a_scale = st.dt * st.dt / (c.mass * c.h);
+            fv[BBX_F_DENSITY_ERR] = rho[i] / c.rho0 - 1.f;
+            fv[BBX_F_DIVV] = st.dt * divv;
+            fv[BBX_F_COUNT] = (float)cnt;
+            fv[BBX_F_M1] = m1 * inv;
+            fv[BBX_F_M2] = m2 * inv;
+            fv[BBX_F_M3] = m3 * inv;
+            fv[BBX_F_CX] = cx * inv;
+            fv[BBX_F_CY] = cy * inv;
+            fv[BBX_F_CZ] = cz * inv;
+            fv[BBX_F_CNORM] = sqrtf(cx * cx + cy * cy + cz * cz) * inv;
+            fv[BBX_F_PPREV] = st.delta != 0.f ? vi.w / (st.delta * c.rho0) : 0.f;
+            fv[BBX_F_VY] = st.dt * vi.y * c.inv_h;
+            fv[BBX_F_VNORM] = st.dt * vn * c.inv_h;
+            fv[BBX_F_AY] = fy * a_scale;
+            fv[BBX_F_ANORM] = fn * a_scale;
+            fv[BBX_F_WALL] = fminf(fmaxf(wall * c.inv_h, 0.f), 4.f);
+#pragma unroll
+            for (int f = 0; f < BBX_NFEAT; f++) feat[(size_t)f * S + i] = fv[f];
+            rho_star[i] = rho[i];
         }
-        float s = coef * w / rj;
-        fx += (vj.x - vi.x) * s;
-        fy += (vj.y - vi.y) * s;
-        fz += (vj.z - vi.z) * s;
-        if (kMode == 2 && j != self) {
-            float u = r * c.inv_h;
-            m1 += u;
-            m2 += u * u;
-            m3 += u * u * u;
-            cx -= dx * c.inv_h;
-            cy -= dy * c.inv_h;
-            cz -= dz * c.inv_h;
-            if (r > 0.f && x > 0.f) {
-                // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
-                float g = -c.spiky_grad * x * x / r;
-                float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-                divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
+    }
+    if (kMode == 3) {
+        // The block's 256 feature rows go through LDS into the MFMA layout:
+        // each wave infers its own 64 particles as four 16-sample tiles.
+        __syncthreads();  // every thread is done with the staged neighbours
+        constexpr int kRow = BBX_NFEAT + 1;  // padded feature row
+        float *lf = reinterpret_cast<float *>(sx);
+        float *lp = lf + kBlock * kRow;
+        float *lb = reinterpret_cast<float *>(sv);  // b1 | b2 | w3
+#pragma unroll
+        for (int f = 0; f < BBX_NFEAT; f++) lf[threadIdx.x * kRow + f] = valid ? fv[f] : 0.f;
+        if (threadIdx.x < kH) {
+            lb[threadIdx.x] = P[OFF_B1 + threadIdx.x];
+            lb[kH + threadIdx.x] = P[OFF_B2 + threadIdx.x];
+            lb[2 * kH + threadIdx.x] = P[OFF_W3 + threadIdx.x];
+        }
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int qq = lane >> 4, jj = lane & 15;
+        bf16x4 w1[4], w2[4][4];
+        float mean[4], istd[4];
+        load_infer_operands(P, lane, w1, w2, mean, istd);
+        const float b3 = P[OFF_B3], oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA];
+        const float pcap = P[OFF_PCAP], resid = P[OFF_RESID];
+#pragma unroll
+        for (int g = 0; g < 4; g += 2) {
+            float xv[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int smp = wv * 64 + (g + t) * 16 + jj;
+#pragma unroll
+                for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
+            }
+            float yn[2];
+            forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, mean, istd, qq, yn);
+            if (lane < 16) {
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int smp = wv * 64 + (g + t) * 16 + jj;
+                    // the same expression as the standalone inference kernel
+                    float y = yn[t] * oscale;
+                    if (resid != 0.f) y += resid * lf[smp * kRow + BBX_F_PPREV];
+                    y = alpha * fminf(fmaxf(y, 0.f), pcap);
+                    lp[smp] = y * st.warm_scale;
+                }
             }
         }
-    });
-    force[i] = make_float4(fx, fy, fz, 0.f);
-    if (kMode == 1) {
-        xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
-        rho_star[i] = rho[i];
-    }
-    if (kMode == 2) {
-        const size_t S = (size_t)c.stride;
-        float nn = cnt > 1 ? (float)(cnt - 1) : 1.f;
-        float inv = 1.f / nn;
-        float wall = 3.4e38f;
-        const float p3[3] = {xi.x, xi.y, xi.z};
-        for (int a = 0; a < 3; a++) {
-            float lo = p3[a] - (c.tank_c[a] - c.tank_h[a]);
-            float hi = (c.tank_c[a] + c.tank_h[a]) - p3[a];
-            wall = fminf(wall, fminf(lo, hi));
+        __syncthreads();
+        if (valid) {
+            const float p = lp[threadIdx.x];
+            const float d = rho[i];
+            const float d2 = d * d;
+            pwarm[i] = p;
+            pres[i] = p;
+            q[i] = (fabsf(d2) < 1e-8f) ? 0.f : p / d2;
         }
-        float vn = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
-        float fn = sqrtf(fx * fx + fy * fy + fz * fz);
-        float a_scale = st.dt * st.dt / (c.mass * c.h);
-        feat[BBX_F_DENSITY_ERR * S + i] = rho[i] / c.rho0 - 1.f;
-        feat[BBX_F_DIVV * S + i] = st.dt * divv;
-        feat[BBX_F_COUNT * S + i] = (float)cnt;
-        feat[BBX_F_M1 * S + i] = m1 * inv;
-        feat[BBX_F_M2 * S + i] = m2 * inv;
-        feat[BBX_F_M3 * S + i] = m3 * inv;
-        feat[BBX_F_CX * S + i] = cx * inv;
-        feat[BBX_F_CY * S + i] = cy * inv;
-        feat[BBX_F_CZ * S + i] = cz * inv;
-        feat[BBX_F_CNORM * S + i] = sqrtf(cx * cx + cy * cy + cz * cz) * inv;
-        feat[BBX_F_PPREV * S + i] = st.delta != 0.f ? vi.w / (st.delta * c.rho0) : 0.f;
-        feat[BBX_F_VY * S + i] = st.dt * vi.y * c.inv_h;
-        feat[BBX_F_VNORM * S + i] = st.dt * vn * c.inv_h;
-        feat[BBX_F_AY * S + i] = fy * a_scale;
-        feat[BBX_F_ANORM * S + i] = fn * a_scale;
-        feat[BBX_F_WALL * S + i] = fminf(fmaxf(wall * c.inv_h, 0.f), 4.f);
-        rho_star[i] = rho[i];
     }
 }
 
@@ -221,10 +291,13 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         const float4 xi = xstar[i];
         float sum = 0.f, comp = 0.f;
         const int cnt = nb.cnt[i];
-        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
-            const float4 xj = staged ? sx[j] : xstar[j];
-            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+        with_staging(staged, [&](auto tag) {
+            constexpr bool kS = decltype(tag)::value;
+            for_each_neighbour<2>(nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j) {
+                const float4 xj = kS ? sx[j] : xstar[j];
+                float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+            });
         });
         float density = sum * c.mass;
         // FP64 error and pressure increment (delta is FP64 on the host)
@@ -523,20 +596,6 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
                              (1.f - f) * vi.z + f * sz_, vi.w);
 }
 
-// warm start: p = MLP output, densityPredicted = density (k = 0 init)
-__global__ void k_warm_init(SimConst c, const float *__restrict__ pwarm,
-                            const float *__restrict__ rho, float *__restrict__ pres,
-                            float *__restrict__ rho_star, float *__restrict__ q) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    float p = pwarm[i];
-    float d = rho[i];
-    pres[i] = p;
-    rho_star[i] = d;
-    float d2 = d * d;
-    q[i] = (fabsf(d2) < 1e-8f) ? 0.f : p / d2;
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -548,22 +607,34 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
     if (s->n == 0) return BBX_OK;
     const unsigned nb = nblocks(s->n);
     const NbrTab t = nbr_tab(s);
-    if (features) {
+    if (features == 2) {
+        if (!s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+        BBX_PROF_BEGIN("nonpressure_features_warm");
+        k_nonpressure<3><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+                                                       s->xstar, s->rho_star, s->feat,
+                                                       mlp_params_device(s->mlp), s->pres, s->q,
+                                                       s->pwarm);
+        BBX_HIP_CHECK(hipGetLastError());
+        BBX_PROF_END();
+    } else if (features) {
         BBX_PROF_BEGIN("nonpressure_features");
         k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat);
+                                                       s->xstar, s->rho_star, s->feat, nullptr,
+                                                       nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     } else if (predict) {
         BBX_PROF_BEGIN("nonpressure_predict");
         k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat);
+                                                       s->xstar, s->rho_star, s->feat, nullptr,
+                                                       nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     } else {
         BBX_PROF_BEGIN("nonpressure");
         k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat);
+                                                       s->xstar, s->rho_star, s->feat, nullptr,
+                                                       nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     }
@@ -632,12 +703,5 @@ int run_xsph(bbx_solver *s, const StepConst &st) {
     return BBX_OK;
 }
 
-int run_warm_init(bbx_solver *s) {
-    if (s->n == 0) return BBX_OK;
-    k_warm_init<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, s->pwarm, s->rho, s->pres,
-                                                         s->rho_star, s->q);
-    BBX_HIP_CHECK(hipGetLastError());
-    return BBX_OK;
-}
 
 }  // namespace bbx

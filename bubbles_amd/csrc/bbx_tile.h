@@ -13,6 +13,8 @@
 // 32-bit storage indices and reads neighbours from global memory.
 #pragma once
 
+#include <type_traits>
+
 #include "bbx_device.h"
 
 namespace bbx {
@@ -142,6 +144,18 @@ __device__ __forceinline__ void tab_merge(const int *row_a, const int *row_b, in
                 if (row_a[r] >= t.start[q] && row_a[r] < t.start[q] + t.len[q]) rdelta[r] = t.off[q] - t.start[q];
 }
 
+// Run f(std::true_type) for an LDS-staged tile and f(std::false_type)
+// otherwise, so code inside f that reads neighbours from LDS or from global
+// memory is compiled separately and never selects between the two address
+// spaces at run time (a select compiles to a slower generic load).
+template <typename F>
+__device__ __forceinline__ void with_staging(bool staged, F f) {
+    if (staged)
+        f(std::true_type{});
+    else
+        f(std::false_type{});
+}
+
 // Walk bucket slots 0..cnt-1 of particle i in bucket order.  Entries are
 // read a quad (4 entries, one 8 B or 16 B load) at a time and software-
 // pipelined: the next D quads are in flight while the current D are
@@ -186,21 +200,28 @@ __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restr
 // LDS slots t, t + blockDim, ...; each thread issues kBatch global loads
 // before its first LDS store, keeping that many loads in flight instead of
 // waiting out one memory latency per staged particle.
+// The interval of a slot is found with a fixed chain of compare-selects over
+// the (block-uniform) interval offsets held in registers: no search loop and
+// no table reads per slot.
 template <typename T, typename Fn>
 __device__ __forceinline__ void tab_stage(const TileTab &t, T *__restrict__ dst, Fn fn) {
     constexpr int kBatch = 8;
-    const int total = t.total;
-    int k = 0;  // interval holding the current slot (slots only increase)
+    const int total = t.total, nivl = t.n_ivl;
+    int off[kMaxIvl], shift[kMaxIvl];  // LDS offset and storage - LDS shift of each interval
+#pragma unroll
+    for (int k = 0; k < kMaxIvl; k++) {
+        off[k] = k < nivl ? t.off[k] : 0x7fffffff;
+        shift[k] = k < nivl ? t.start[k] - t.off[k] : 0;
+    }
     for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
         int src[kBatch];
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
             const int e = base + b * (int)blockDim.x;
-            src[b] = -1;
-            if (e < total) {
-                while (e >= t.off[k] + t.len[k]) k++;
-                src[b] = t.start[k] + (e - t.off[k]);
-            }
+            int d = shift[0];
+#pragma unroll
+            for (int k = 1; k < kMaxIvl; k++) d = e >= off[k] ? shift[k] : d;
+            src[b] = e < total ? e + d : -1;
         }
         T v[kBatch];
 #pragma unroll

@@ -33,7 +33,8 @@ SHORT = {
     "k_scan_block": "grid_scan", "k_scan_add": "grid_scan_add",
     "k_buckets_density<false>": "buckets_density", "k_buckets_density<true>": "buckets_density_tait",
     "k_nonpressure<0>": "nonpressure", "k_nonpressure<1>": "nonpressure_predict",
-    "k_nonpressure<2>": "nonpressure_features", "k_mlp_infer": "mlp_infer",
+    "k_nonpressure<2>": "nonpressure_features", "k_nonpressure<3>": "nonpressure_features_warm",
+    "k_mlp_infer": "mlp_infer",
     "k_warm_init": "warm_init", "k_predict_pressure": "predict_pressure",
     "k_pressure_force": "pressure_force", "k_stream_copy": "stream_copy",
     "k_sph_force": "sph_force", "k_integrate": "integrate", "k_xsph": "xsph",

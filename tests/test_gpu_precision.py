@@ -1,0 +1,61 @@
+"""Precision study at the benchmark geometry (dam break at spacing 0.01,
+h 0.02, 2x1x2 tank): FP32 + Kahan on the GPU against the FP64 oracle from the
+same FP32 start, REFERENCE_EXACT (a fixed iteration count, so the two
+trajectories differ by arithmetic alone), at 262,144 (64^3) and 1,048,576
+(128x64x128) particles.  Reported per checkpoint: max and mean relative
+density error, pressure-force error relative to max |f_p|, and position error
+in units of h.  Stated bounds at the last checkpoint: density max 1e-3 /
+mean 1e-5, pressure force 5e-2 of max |f_p| (mean 1e-3), position max 1e-2 h.
+
+With BBX_PRECISION_OUT set, the table is also written there as JSON.
+"""
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _errors(g, o, h):
+    rg, ro = g.get("density"), o.get("density")
+    drel = np.abs(rg - ro) / ro
+    fg, fo = g.get("pressure_force"), o.get("pressure_force")
+    fscale = max(np.abs(fo).max(), 1e-30)
+    ferr = np.abs(fg - fo).max(axis=1) / fscale
+    perr = np.abs(g.get("position") - o.get("position")).max(axis=1) / h
+    return {"density_rel_max": float(drel.max()), "density_rel_mean": float(drel.mean()),
+            "fp_rel_max": float(ferr.max()), "fp_rel_mean": float(ferr.mean()),
+            "pos_over_h_max": float(perr.max()), "pos_over_h_mean": float(perr.mean())}
+
+
+@pytest.mark.parametrize("dims,steps", [((64, 64, 64), 100), ((128, 64, 128), 40)])
+def test_fp32_kahan_against_fp64(bbx, oracle, dims, steps):
+    sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position").astype(np.float64)), threads=16)
+    table = {}
+    checkpoints = {1, 10, steps // 2, steps}
+    for k in range(1, steps + 1):
+        g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
+        o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+        if k in checkpoints:
+            table[k] = _errors(g, o, sc.kernel_radius)
+    print(json.dumps({"n": sc.n, "errors": table}))
+    out = os.environ.get("BBX_PRECISION_OUT")
+    if out:
+        rec = {}
+        if os.path.exists(out):
+            with open(out) as f:
+                rec = json.load(f)
+        rec[str(sc.n)] = table
+        with open(out, "w") as f:
+            json.dump(rec, f, indent=1)
+    last = table[steps]
+    assert last["density_rel_max"] <= 1e-3 and last["density_rel_mean"] <= 1e-5, last
+    assert last["fp_rel_max"] <= 5e-2 and last["fp_rel_mean"] <= 1e-3, last
+    assert last["pos_over_h_max"] <= 1e-2, last
