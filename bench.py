@@ -43,7 +43,7 @@ KERNEL_BYTES = {
     "grid_hash": 36,                   # x, old cell -> new cell, key, rank, hash
     "grid_scatter": 16,                # new cell, rank, start -> slot index
     "grid_rank_move": 76,              # slot, x+id, v+p -> x+id, v+p, cell, home
-    "buckets_density": 16,             # x -> rho
+    "buckets_density": 20,             # x -> rho, count; + 2 B per bucket entry (see below)
     "nonpressure_predict": 56,         # x, v, rho -> f, x*, rho*
     "nonpressure_features": 108,       # x, v, rho -> f, rho*, 16 features
     "nonpressure_features_warm": 120,  # x, v, rho -> f, rho*, 16 features, p_warm, p, q
@@ -54,6 +54,15 @@ KERNEL_BYTES = {
     "pressure_force_integrate": 96,    # x, q, v, f, p -> x, v+p, f, f_p
 }
 LOOP_KERNELS = ("predict_pressure", "pressure_force")
+
+
+def algorithmic_bytes(name: str, mean_bucket: float):
+    """Bytes per particle of `name`; the bucket build also writes the bucket
+    itself (the reference's Bucket output), 2 B per entry at the mean count."""
+    b = KERNEL_BYTES.get(name)
+    if b is not None and name == "buckets_density":
+        b = b + 2.0 * mean_bucket
+    return b
 
 
 def substep_bytes(k: float) -> float:
@@ -135,6 +144,7 @@ def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
     t1 = time.perf_counter()
     live = solver.profile_get() if dominant else {}
     solver.profile(False)
+    mean_bucket = float(np.mean(solver.bucket_counts()))
     solver.close()
     elapsed = t1 - t0
     errs = np.asarray(errs)
@@ -142,7 +152,7 @@ def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
            "mean_iterations": float(np.mean(iters)), "max_iterations": int(np.max(iters)),
            "max_density_error": float(errs.max()),
            "converged_fraction": float(np.mean(errs < args.tol)),
-           "kernels": table, "dominant": dominant}
+           "kernels": table, "dominant": dominant, "mean_bucket": mean_bucket}
     if dominant:
         rec = live.get(dominant)
         out["dominant_live"] = (rec["total_ms"] / rec["launches"], rec["launches"]) if rec else None
@@ -154,7 +164,7 @@ def roofline_for(res, n, stream_gbs):
     if name is None:
         return None
     avg_ms, launches = res["dominant_live"] or (res["kernels"][name]["avg_ms"], 0)
-    per_unit = KERNEL_BYTES.get(name)
+    per_unit = algorithmic_bytes(name, res["mean_bucket"])
     achieved = per_unit * n / (avg_ms * 1e-3) / 1e9 if per_unit else None
     traffic = load_traffic(name, n)
     return {"bound": "hbm", "kernel": name,
@@ -163,7 +173,8 @@ def roofline_for(res, n, stream_gbs):
             "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
             "traffic": traffic,
             "avg_ms": round(avg_ms, 5), "timed_launches": launches,
-            "bytes_per_particle": per_unit, "units_per_launch": n,
+            "bytes_per_particle": round(per_unit, 2) if per_unit else None, "units_per_launch": n,
+            "mean_bucket_count": round(res["mean_bucket"], 2),
             "measured_stream_copy_GBs": round(stream_gbs, 1),
             "frac_of_measured_copy": round(achieved / stream_gbs, 4) if achieved and stream_gbs else None}
 
@@ -223,7 +234,7 @@ def main():
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -267,6 +267,11 @@ class Solver:
             m = np.zeros(1, dtype=np.int32)
         check(load().bbx_grid_set_chains(self.h, _ip(s), _ip(m)), "bbx_grid_set_chains")
 
+    def bucket_counts(self) -> np.ndarray:
+        counts = np.zeros(self.n, dtype=np.int32)
+        check(load().bbx_neighbors_get(self.h, _ip(counts), None), "bbx_neighbors_get")
+        return counts
+
     def buckets(self):
         counts = np.zeros(self.n, dtype=np.int32)
         ids = np.zeros((self.n, BUCKET), dtype=np.int32)

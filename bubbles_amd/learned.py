@@ -164,6 +164,46 @@ def compare(scene, net, steps=200, warmup=0, max_it=8, tol=0.01):
     return res
 
 
+def run_variant(net, alpha, max_it, seed, steps=480, warmup=100, dims=(128, 64, 128), tol=0.01):
+    """Iterations, error and throughput of one (network, alpha, cap) on a dam
+    break; net None = plain PCISPH.  A run whose density error exceeds 50%
+    is stopped and reported as exploded."""
+    sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=seed)
+    s = abi.Solver(sc)
+    warm = abi.WARM_NONE
+    if net is not None:
+        p = net.get_params()
+        p[OFF_ALPHA] = alpha
+        variant = abi.Mlp(0)
+        variant.set_params(p)
+        s.attach_mlp(variant)
+        warm = abi.WARM_MLP
+    exploded = False
+    for _ in range(warmup):
+        if s.pcisph_substep(sc.dt, max_it, tol, abi.MODE_TEXTBOOK, warm).max_density_error > 0.5:
+            exploded = True
+            break
+    its, errs = [], []
+    s.synchronize()
+    t0 = time.perf_counter()
+    if not exploded:
+        for _ in range(steps):
+            st = s.pcisph_substep(sc.dt, max_it, tol, abi.MODE_TEXTBOOK, warm)
+            its.append(st.iterations)
+            errs.append(st.max_density_error)
+            if st.max_density_error > 0.5:
+                exploded = True
+                break
+    s.synchronize()
+    el = time.perf_counter() - t0
+    s.close()
+    if exploded:
+        return {"exploded": True}
+    return {"mean_it": round(float(np.mean(its)), 3), "max_err": round(float(np.max(errs)), 4),
+            "conv": round(float(np.mean(np.array(errs) < tol)), 3),
+            "psps": round(sc.n * steps / el / 1e6, 1)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -184,7 +224,25 @@ def main(argv=None):
     ev.add_argument("--ny", type=int, default=64)
     ev.add_argument("--nz", type=int, default=128)
     ev.add_argument("--seed", type=int, default=HELDOUT_SEEDS[0])
+    se = sub.add_parser("select", help="sweep alpha and the iteration cap on given seeds")
+    se.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    se.add_argument("--alphas", default="0.6,0.7,0.8")
+    se.add_argument("--caps", default="8,64")
+    se.add_argument("--seeds", default="2000,2")
     args = ap.parse_args(argv)
+    if args.cmd == "select":
+        net = load(args.weights)
+        caps = [int(c) for c in args.caps.split(",")]
+        for seed in [int(x) for x in args.seeds.split(",")]:
+            for cap in caps:
+                print(json.dumps({"run": "plain", "seed": seed, "cap": cap, **run_variant(None, 0.0, cap, seed)}),
+                      flush=True)
+            for alpha in [float(a) for a in args.alphas.split(",")]:
+                for cap in caps:
+                    r = run_variant(net, alpha, cap, seed)
+                    print(json.dumps({"run": "learned", "alpha": alpha, "seed": seed, "cap": cap, **r}),
+                          flush=True)
+        return 0
     if args.cmd == "train":
         t0 = time.perf_counter()
         seeds = TRAIN_SEEDS[:4] if args.quick else TRAIN_SEEDS
