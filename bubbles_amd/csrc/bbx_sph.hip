@@ -103,6 +103,17 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     __syncthreads();
     const bool staged = tt.staged != 0;                  // buckets hold LDS-local indices
     const bool in_lds = staged && tt.total <= c.tile_cap2;  // and they fit this kernel's LDS
+    const int i = tile * kBlock + threadIdx.x;
+    const bool valid = i < c.n;
+    NbrQuads<kNbrDepth> first{};
+    int cnt = 0;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+    if (valid) {  // independent of the staging: requested before it
+        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
+        cnt = nb.cnt[i];
+        xi = pos[i];
+        vi = vel[i];
+    }
     if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return pos[j]; });
         tab_stage(tt, sv, [&](int j) {
@@ -111,21 +122,16 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
         });
     }
     __syncthreads();
-    const int i = tile * kBlock + threadIdx.x;
-    const bool valid = i < c.n;
     if (kMode != 3 && !valid) return;  // mode 3 keeps the block whole for its barriers
     float fv[BBX_NFEAT];
     if (valid) {
         const int self = staged ? tab_local(tt, i) : i;
-        const float4 xi = pos[i];
-        const float4 vi = vel[i];
         const float coef = c.visc * c.mass2;
         float fx = c.mass * c.gx - c.drag * vi.x;
         float fy = c.mass * c.gy - c.drag * vi.y;
         float fz = c.mass * c.gz - c.drag * vi.z;
-        const int cnt = nb.cnt[i];
         float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
             float4 xj, vj;
             if (in_lds) {
                 xj = sx[j];
@@ -283,17 +289,26 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
+    const int i = tile * kBlock + threadIdx.x;
+    const bool valid = i < c.n;
+    // the particle's own data and first bucket quads do not depend on the
+    // staging: request them first so their latency overlaps it
+    NbrQuads<kNbrDepth> first{};
+    int cnt = 0;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
+        cnt = nb.cnt[i];
+        xi = xstar[i];
+    }
     if (staged) tab_stage(tt, sx, [&](int j) { return xstar[j]; });
     __syncthreads();
-    const int i = tile * kBlock + threadIdx.x;
     unsigned long long ebits = 0ull;
-    if (i < c.n) {
-        const float4 xi = xstar[i];
+    if (valid) {
         float sum = 0.f, comp = 0.f;
-        const int cnt = nb.cnt[i];
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
-            for_each_neighbour<2>(nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j) {
+            for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j) {
                 const float4 xj = kS ? sx[j] : xstar[j];
                 float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
@@ -384,21 +399,29 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
     tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
+    const int i = tile * kBlock + threadIdx.x;
+    const bool valid = i < c.n;
+    NbrQuads<kNbrDepth> first{};
+    int cnt = 0;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float qi = 0.f;
+    if (valid) {  // independent of the staging: requested before it
+        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
+        cnt = nb.cnt[i];
+        xi = pos[i];
+        qi = q[i];
+    }
     if (staged)
         tab_stage(tt, sx, [&](int j) {
             float4 p = pos[j];
             return make_float4(p.x, p.y, p.z, q[j]);
         });
     __syncthreads();
-    const int i = tile * kBlock + threadIdx.x;
     unsigned fbits = 0u;
-    if (i < c.n) {
+    if (valid) {
         const int self = staged ? tab_local(tt, i) : i;
-        const float4 xi = pos[i];
-        const float qi = q[i];
         float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-        const int cnt = nb.cnt[i];
-        for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
             if (j == self) return;
             float4 xj;
             if (staged) {

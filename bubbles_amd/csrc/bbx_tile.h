@@ -156,32 +156,58 @@ __device__ __forceinline__ void with_staging(bool staged, F f) {
         f(std::false_type{});
 }
 
-// Walk bucket slots 0..cnt-1 of particle i in bucket order.  Entries are
-// read a quad (4 entries, one 8 B or 16 B load) at a time and software-
-// pipelined: the next D quads are in flight while the current D are
-// processed.  body(j) gets an LDS-local index (staged tile) or a storage
-// index.
+// Bucket entries of particle i come a quad (4 entries, one 8 B or 16 B
+// load) at a time.  The first D quads can be requested early -- before the
+// tile is staged, which they do not depend on -- so their latency overlaps
+// the staging; entries past the bucket count are loaded but never used
+// (every particle has 25 quads of table).
+constexpr int kNbrDepth = 4;  // quads (16 entries) in flight per lane in the hot gathers
+
+template <int D>
+struct NbrQuads {
+    int4 q[D];
+};
+
+__device__ __forceinline__ int4 nbr_quad(const unsigned short *__restrict__ n16,
+                                         const int *__restrict__ n32, bool staged, int i,
+                                         int stride, int q) {
+    const size_t at = (size_t)q * stride + i;
+    if (staged) {
+        const ushort4 v = reinterpret_cast<const ushort4 *>(n16)[at];
+        return make_int4(v.x, v.y, v.z, v.w);
+    }
+    return reinterpret_cast<const int4 *>(n32)[at];
+}
+
+template <int D>
+__device__ __forceinline__ NbrQuads<D> nbr_prefetch(const unsigned short *__restrict__ n16,
+                                                    const int *__restrict__ n32, bool staged,
+                                                    int i, int stride) {
+    NbrQuads<D> p;
+#pragma unroll
+    for (int d = 0; d < D; d++) p.q[d] = nbr_quad(n16, n32, staged, i, stride, d);
+    return p;
+}
+
+// Walk bucket slots 0..cnt-1 of particle i in bucket order, starting from
+// the prefetched first D quads and keeping the next D quads in flight while
+// the current D are processed.  body(j) gets an LDS-local index (staged
+// tile) or a storage index.
 template <int D, typename Body>
-__device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
-                                                   const int *__restrict__ n32, bool staged,
-                                                   int i, int cnt, int stride, Body body) {
+__device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first,
+                                                        const unsigned short *__restrict__ n16,
+                                                        const int *__restrict__ n32, bool staged,
+                                                        int i, int cnt, int stride, Body body) {
     const int nq = (cnt + 3) >> 2;
-    auto quad = [&](int q) -> int4 {
-        const size_t at = (size_t)q * stride + i;
-        if (staged) {
-            const ushort4 v = reinterpret_cast<const ushort4 *>(n16)[at];
-            return make_int4(v.x, v.y, v.z, v.w);
-        }
-        return reinterpret_cast<const int4 *>(n32)[at];
-    };
     int4 cur[D];
 #pragma unroll
-    for (int d = 0; d < D; d++) cur[d] = d < nq ? quad(d) : make_int4(0, 0, 0, 0);
+    for (int d = 0; d < D; d++) cur[d] = first.q[d];
     for (int q = 0; q < nq; q += D) {
         int4 nxt[D];
 #pragma unroll
         for (int d = 0; d < D; d++)
-            nxt[d] = (q + D + d) < nq ? quad(q + D + d) : make_int4(0, 0, 0, 0);
+            nxt[d] = (q + D + d) < nq ? nbr_quad(n16, n32, staged, i, stride, q + D + d)
+                                      : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const int k = (q + d) * 4;
@@ -193,6 +219,14 @@ __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restr
 #pragma unroll
         for (int d = 0; d < D; d++) cur[d] = nxt[d];
     }
+}
+
+template <int D, typename Body>
+__device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
+                                                   const int *__restrict__ n32, bool staged,
+                                                   int i, int cnt, int stride, Body body) {
+    const NbrQuads<D> first = nbr_prefetch<D>(n16, n32, staged, i, stride);
+    for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body);
 }
 
 // Cooperative copy of the tile's intervals into LDS: dst[off + e] = fn(start + e).
