@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
 // (nbr32).  Candidates are visited in the reference's order: neighbour cells
 // y outer, x, z inner (src/core/grid.h:568-591), each cell in chain order.
 template <bool kPressure>
-__global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
+__global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
                                                             const float4 *__restrict__ pos,
                                                             const int *__restrict__ home,
                                                             const int *__restrict__ start,
@@ -265,7 +265,9 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             float *__restrict__ pres,
                                                             int *__restrict__ tile_tab,
                                                             int *__restrict__ counters) {
-    __shared__ float4 sx[kTileCap];
+    // staged candidate positions as x, y, z arrays (12 B per particle):
+    // five workgroups share a CU's LDS
+    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap];
     __shared__ TileTab tt;
     __shared__ int rdelta[9];
     __shared__ int smin, smax;
@@ -295,7 +297,13 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     __syncthreads();
     tab_store(tile_tab + (size_t)tile * kTabInts, tt);
     const bool staged = tt.staged != 0;
-    if (staged) tab_stage(tt, sx, [&](int j) { return pos[j]; });
+    if (staged)
+        tab_stage_with(tt, [&](int j) { return pos[j]; }, [&](int e, const float4 &v) {
+            sxx[e] = v.x;
+            sxy[e] = v.y;
+            sxz[e] = v.z;
+        });
+    auto lds_pos = [&](int e) { return make_float4(sxx[e], sxy[e], sxz[e], 0.f); };
     __syncthreads();
     if (!valid) return;
     // The particle's work is instantiated once for LDS-staged and once for
@@ -330,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             const int jl = j + sh;
             float4 pj;
             if (kStaged)
-                pj = sx[jl];
+                pj = lds_pos(jl);
             else
                 pj = pos[j];
             const unsigned v = kStaged ? (unsigned)jl : (unsigned)j;
@@ -348,21 +356,26 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             // Row r = (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are
             // contiguous in storage.  B[r] = start of x-1, x, x+1 and end of x+1;
             // cells and rows outside the grid are empty ranges.
-            int B[9][4], sh[9];
+            // Row r: storage start A[r] of cell x-1 and the offsets of cells x,
+            // x+1 and the row end packed as bytes in O[r] (narrow rows hold at
+            // most 64 candidates); sh[r] = LDS - storage shift of the row.
+            int A[9], sh[9];
+            unsigned O[9];
             bool narrow = true;
     #pragma unroll
             for (int r = 0; r < 9; r++) {
                 const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
                 sh[r] = kStaged ? rdelta[r] : 0;
-                B[r][0] = B[r][1] = B[r][2] = B[r][3] = 0;
+                A[r] = 0;
+                O[r] = 0u;
                 if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
                     const int cx = u[0] + yy * nx + zz * nxy;
                     const int b1 = start[cx], b2 = start[cx + 1];
-                    B[r][0] = u[0] > 0 ? start[cx - 1] : b1;
-                    B[r][1] = b1;
-                    B[r][2] = b2;
-                    B[r][3] = u[0] + 1 < nx ? start[cx + 2] : b2;
-                    narrow = narrow && (B[r][3] - B[r][0] <= c.row_max);
+                    const int b0 = u[0] > 0 ? start[cx - 1] : b1;
+                    const int b3 = u[0] + 1 < nx ? start[cx + 2] : b2;
+                    A[r] = b0;
+                    O[r] = (unsigned)(b1 - b0) | ((unsigned)(b2 - b0) << 8) | ((unsigned)(b3 - b0) << 16);
+                    narrow = narrow && (b3 - b0 <= c.row_max);
                 }
             }
             if (narrow) {
@@ -371,16 +384,16 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     #pragma unroll
                 for (int r = 0; r < 9; r++) {
                     unsigned long long m = 0ull;
-                    const int a = B[r][0], len = B[r][3] - B[r][0], s = sh[r];
+                    const int a = A[r], len = (int)(O[r] >> 16), s = sh[r];
                     // separate LDS and global loops: a select between the two
                     // addresses would become a generic (flat) load
                     if (kStaged) {
-                        const float4 *row = sx + a + s;
+                        const int row = a + s;
                         for (int t0 = 0; t0 < len; t0 += 4) {
                             float4 pc[4];
     #pragma unroll
                             for (int q = 0; q < 4; q++)
-                                if (t0 + q < len) pc[q] = row[t0 + q];
+                                if (t0 + q < len) pc[q] = lds_pos(row + t0 + q);
     #pragma unroll
                             for (int q = 0; q < 4; q++)
                                 if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
@@ -410,16 +423,16 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
 #pragma unroll
                         for (int dz = 0; dz < 3; dz++) {
                             const int r = dz * 3 + dy;
-                            const int lo = dx == 0 ? B[r][0] : (dx == 1 ? B[r][1] : B[r][2]);
-                            const int hi = dx == 0 ? B[r][1] : (dx == 1 ? B[r][2] : B[r][3]);
+                            const int lo = dx == 0 ? 0 : (int)((O[r] >> (8 * (dx - 1))) & 0xffu);
+                            const int hi = (int)((O[r] >> (8 * dx)) & 0xffu);
                             const int w = hi - lo;
                             if (w <= 0) continue;
-                            unsigned long long m = M[r] >> (lo - B[r][0]);
+                            unsigned long long m = M[r] >> lo;
                             if (w < 64) m &= (1ull << w) - 1ull;
                             while (m) {
                                 const int b = __builtin_ctzll(m);
                                 m &= m - 1ull;
-                                emit(lo + b, sh[r]);
+                                emit(A[r] + lo + b, sh[r]);
                             }
                         }
             } else {
@@ -437,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                             for (int j = b; j < e; j++) {
                                 float4 pj;
                                 if (kStaged)
-                                    pj = sx[j + sr];
+                                    pj = lds_pos(j + sr);
                                 else
                                     pj = pos[j];
                                 if (in_bucket(c, pi, pj)) emit(j, sr);

@@ -80,20 +80,27 @@ __device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float
 //       inference and the k = 0 initialisation (p = p_warm, rho* = rho)
 // LDS: neighbour position, (velocity, density); in mode 3 the same LDS then
 // holds the block's features for the MFMA inference.
+//
+// Two lanes per particle: the two LDS arrays allow only two workgroups per
+// CU, so each particle's bucket is split between lanes 2p and 2p + 1 (even
+// and odd quads) to double the waves in flight; the two partial sums are
+// added with one lane exchange.
 // ---------------------------------------------------------------------------
+constexpr int kNpThreads = 2 * kBlock;
+
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
-                                                        const float4 *__restrict__ pos,
-                                                        const float4 *__restrict__ vel,
-                                                        const float *__restrict__ rho,
-                                                        float4 *__restrict__ force,
-                                                        float4 *__restrict__ xstar,
-                                                        float *__restrict__ rho_star,
-                                                        float *__restrict__ feat,
-                                                        const float *__restrict__ P,
-                                                        float *__restrict__ pres,
-                                                        float *__restrict__ q,
-                                                        float *__restrict__ pwarm) {
+__global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
+                                                            const float4 *__restrict__ pos,
+                                                            const float4 *__restrict__ vel,
+                                                            const float *__restrict__ rho,
+                                                            float4 *__restrict__ force,
+                                                            float4 *__restrict__ xstar,
+                                                            float *__restrict__ rho_star,
+                                                            float *__restrict__ feat,
+                                                            const float *__restrict__ P,
+                                                            float *__restrict__ pres,
+                                                            float *__restrict__ q,
+                                                            float *__restrict__ pwarm) {
     constexpr bool kFeat = kMode >= 2;
     __shared__ float4 sx[kTileCap2];
     __shared__ float4 sv[kTileCap2];
@@ -103,16 +110,18 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
     __syncthreads();
     const bool staged = tt.staged != 0;                  // buckets hold LDS-local indices
     const bool in_lds = staged && tt.total <= c.tile_cap2;  // and they fit this kernel's LDS
-    const int i = tile * kBlock + threadIdx.x;
+    const int pl = threadIdx.x >> 1, half = threadIdx.x & 1;
+    const int i = tile * kBlock + pl;
     const bool valid = i < c.n;
-    NbrQuads<kNbrDepth> first{};
     int cnt = 0;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+    int4 q0 = make_int4(0, 0, 0, 0), q1 = q0;  // this lane's first two quads: half, half + 2
     if (valid) {  // independent of the staging: requested before it
-        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
         cnt = nb.cnt[i];
         xi = pos[i];
         vi = vel[i];
+        q0 = nbr_quad(nb.n16, nb.n32, staged, i, c.stride, half);
+        q1 = nbr_quad(nb.n16, nb.n32, staged, i, c.stride, half + 2);
     }
     if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return pos[j]; });
@@ -122,16 +131,14 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
         });
     }
     __syncthreads();
-    if (kMode != 3 && !valid) return;  // mode 3 keeps the block whole for its barriers
+    if (kMode != 3 && !valid) return;  // the lane pair shares validity; mode 3 keeps the block whole
     float fv[BBX_NFEAT];
-    if (valid) {
-        const int self = staged ? tab_local(tt, i) : i;
+    {
+        const int self = staged && valid ? tab_local(tt, i) : i;
         const float coef = c.visc * c.mass2;
-        float fx = c.mass * c.gx - c.drag * vi.x;
-        float fy = c.mass * c.gy - c.drag * vi.y;
-        float fz = c.mass * c.gz - c.drag * vi.z;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
         float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-        for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+        auto body = [&](int j) {
             float4 xj, vj;
             if (in_lds) {
                 xj = sx[j];
@@ -169,13 +176,50 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
                     divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
                 }
             }
-        });
-        force[i] = make_float4(fx, fy, fz, 0.f);
-        if (kMode == 1) {
-            xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
-            rho_star[i] = rho[i];
+        };
+        // quads half, half + 2, half + 4, ... of the bucket, two in flight
+        const int nq = (cnt + 3) >> 2;
+        for (int qd = half; qd < nq; qd += 4) {
+            const int4 n0 = (qd + 4) < nq ? nbr_quad(nb.n16, nb.n32, staged, i, c.stride, qd + 4)
+                                          : make_int4(0, 0, 0, 0);
+            const int4 n1 = (qd + 6) < nq ? nbr_quad(nb.n16, nb.n32, staged, i, c.stride, qd + 6)
+                                          : make_int4(0, 0, 0, 0);
+            const int k0 = qd * 4, k1 = (qd + 2) * 4;
+            if (k0 < cnt) body(q0.x);
+            if (k0 + 1 < cnt) body(q0.y);
+            if (k0 + 2 < cnt) body(q0.z);
+            if (k0 + 3 < cnt) body(q0.w);
+            if (k1 < cnt) body(q1.x);
+            if (k1 + 1 < cnt) body(q1.y);
+            if (k1 + 2 < cnt) body(q1.z);
+            if (k1 + 3 < cnt) body(q1.w);
+            q0 = n0;
+            q1 = n1;
         }
+        // the pair's partial sums
+        fx += __shfl_xor(fx, 1, 64);
+        fy += __shfl_xor(fy, 1, 64);
+        fz += __shfl_xor(fz, 1, 64);
+        fx += c.mass * c.gx - c.drag * vi.x;
+        fy += c.mass * c.gy - c.drag * vi.y;
+        fz += c.mass * c.gz - c.drag * vi.z;
         if (kFeat) {
+            m1 += __shfl_xor(m1, 1, 64);
+            m2 += __shfl_xor(m2, 1, 64);
+            m3 += __shfl_xor(m3, 1, 64);
+            cx += __shfl_xor(cx, 1, 64);
+            cy += __shfl_xor(cy, 1, 64);
+            cz += __shfl_xor(cz, 1, 64);
+            divv += __shfl_xor(divv, 1, 64);
+        }
+        if (valid && half == 0) {
+            force[i] = make_float4(fx, fy, fz, 0.f);
+            if (kMode == 1) {
+                xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
+                rho_star[i] = rho[i];
+            }
+        }
+        if (kFeat && valid) {
             const size_t S = (size_t)c.stride;
             float nn = cnt > 1 ? (float)(cnt - 1) : 1.f;
             float inv = 1.f / nn;
@@ -205,21 +249,25 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
             fv[BBX_F_AY] = fy * a_scale;
             fv[BBX_F_ANORM] = fn * a_scale;
             fv[BBX_F_WALL] = fminf(fmaxf(wall * c.inv_h, 0.f), 4.f);
+            if (half == 0) {
 #pragma unroll
-            for (int f = 0; f < BBX_NFEAT; f++) feat[(size_t)f * S + i] = fv[f];
-            rho_star[i] = rho[i];
+                for (int f = 0; f < BBX_NFEAT; f++) feat[(size_t)f * S + i] = fv[f];
+                rho_star[i] = rho[i];
+            }
         }
     }
     if (kMode == 3) {
         // The block's 256 feature rows go through LDS into the MFMA layout:
-        // each wave infers its own 64 particles as four 16-sample tiles.
+        // each of the eight waves infers 32 particles as two 16-sample tiles.
         __syncthreads();  // every thread is done with the staged neighbours
         constexpr int kRow = BBX_NFEAT + 1;  // padded feature row
         float *lf = reinterpret_cast<float *>(sx);
         float *lp = lf + kBlock * kRow;
         float *lb = reinterpret_cast<float *>(sv);  // b1 | b2 | w3
+        if (half == 0) {
 #pragma unroll
-        for (int f = 0; f < BBX_NFEAT; f++) lf[threadIdx.x * kRow + f] = valid ? fv[f] : 0.f;
+            for (int f = 0; f < BBX_NFEAT; f++) lf[pl * kRow + f] = valid ? fv[f] : 0.f;
+        }
         if (threadIdx.x < kH) {
             lb[threadIdx.x] = P[OFF_B1 + threadIdx.x];
             lb[kH + threadIdx.x] = P[OFF_B2 + threadIdx.x];
@@ -233,32 +281,29 @@ __global__ void __launch_bounds__(kBlock) k_nonpressure(SimConst c, StepConst st
         load_infer_operands(P, lane, w1, w2, mean, istd);
         const float b3 = P[OFF_B3], oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA];
         const float pcap = P[OFF_PCAP], resid = P[OFF_RESID];
+        float xv[2][4];
 #pragma unroll
-        for (int g = 0; g < 4; g += 2) {
-            float xv[2][4];
+        for (int t = 0; t < 2; t++) {
+            const int smp = wv * 32 + t * 16 + jj;
+#pragma unroll
+            for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
+        }
+        float yn[2];
+        forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, mean, istd, qq, yn);
+        if (lane < 16) {
 #pragma unroll
             for (int t = 0; t < 2; t++) {
-                const int smp = wv * 64 + (g + t) * 16 + jj;
-#pragma unroll
-                for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
-            }
-            float yn[2];
-            forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, mean, istd, qq, yn);
-            if (lane < 16) {
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int smp = wv * 64 + (g + t) * 16 + jj;
-                    // the same expression as the standalone inference kernel
-                    float y = yn[t] * oscale;
-                    if (resid != 0.f) y += resid * lf[smp * kRow + BBX_F_PPREV];
-                    y = alpha * fminf(fmaxf(y, 0.f), pcap);
-                    lp[smp] = y * st.warm_scale;
-                }
+                const int smp = wv * 32 + t * 16 + jj;
+                // the same expression as the standalone inference kernel
+                float y = yn[t] * oscale;
+                if (resid != 0.f) y += resid * lf[smp * kRow + BBX_F_PPREV];
+                y = alpha * fminf(fmaxf(y, 0.f), pcap);
+                lp[smp] = y * st.warm_scale;
             }
         }
         __syncthreads();
-        if (valid) {
-            const float p = lp[threadIdx.x];
+        if (valid && half == 0) {
+            const float p = lp[pl];
             const float d = rho[i];
             const float d2 = d * d;
             pwarm[i] = p;
@@ -281,7 +326,9 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
                                                              float *__restrict__ rho_star,
                                                              float *__restrict__ q,
                                                              LoopState *__restrict__ loop, int k) {
-    __shared__ float4 sx[kTileCap];
+    // x*, y*, z* as separate arrays (12 B per particle): five workgroups
+    // share a CU's LDS instead of four
+    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap];
     __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kBlock / 64];
     if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
@@ -301,7 +348,12 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         cnt = nb.cnt[i];
         xi = xstar[i];
     }
-    if (staged) tab_stage(tt, sx, [&](int j) { return xstar[j]; });
+    if (staged)
+        tab_stage_with(tt, [&](int j) { return xstar[j]; }, [&](int e, const float4 &v) {
+            sxx[e] = v.x;
+            sxy[e] = v.y;
+            sxz[e] = v.z;
+        });
     __syncthreads();
     unsigned long long ebits = 0ull;
     if (valid) {
@@ -309,7 +361,13 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
             for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j) {
-                const float4 xj = kS ? sx[j] : xstar[j];
+                float3 xj;
+                if (kS) {
+                    xj = make_float3(sxx[j], sxy[j], sxz[j]);
+                } else {
+                    const float4 g = xstar[j];
+                    xj = make_float3(g.x, g.y, g.z);
+                }
                 float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
             });
@@ -633,7 +691,7 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
     if (features == 2) {
         if (!s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
         BBX_PROF_BEGIN("nonpressure_features_warm");
-        k_nonpressure<3><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+        k_nonpressure<3><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat,
                                                        mlp_params_device(s->mlp), s->pres, s->q,
                                                        s->pwarm);
@@ -641,21 +699,21 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
         BBX_PROF_END();
     } else if (features) {
         BBX_PROF_BEGIN("nonpressure_features");
-        k_nonpressure<2><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+        k_nonpressure<2><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat, nullptr,
                                                        nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     } else if (predict) {
         BBX_PROF_BEGIN("nonpressure_predict");
-        k_nonpressure<1><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+        k_nonpressure<1><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat, nullptr,
                                                        nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     } else {
         BBX_PROF_BEGIN("nonpressure");
-        k_nonpressure<0><<<nb, kBlock, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
+        k_nonpressure<0><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat, nullptr,
                                                        nullptr, nullptr, nullptr);
         BBX_HIP_CHECK(hipGetLastError());

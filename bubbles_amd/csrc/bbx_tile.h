@@ -237,8 +237,11 @@ __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restr
 // The interval of a slot is found with a fixed chain of compare-selects over
 // the (block-uniform) interval offsets held in registers: no search loop and
 // no table reads per slot.
-template <typename T, typename Fn>
-__device__ __forceinline__ void tab_stage(const TileTab &t, T *__restrict__ dst, Fn fn) {
+// store(e, v) writes the value loaded for LDS slot e (tab_stage below stores
+// it whole into one array).
+template <typename Fn, typename Store>
+__device__ __forceinline__ void tab_stage_with(const TileTab &t, Fn fn, Store store) {
+    using T = decltype(fn(0));
     constexpr int kBatch = 8;
     const int total = t.total, nivl = t.n_ivl;
     int off[kMaxIvl], shift[kMaxIvl];  // LDS offset and storage - LDS shift of each interval
@@ -263,8 +266,13 @@ __device__ __forceinline__ void tab_stage(const TileTab &t, T *__restrict__ dst,
             if (src[b] >= 0) v[b] = fn(src[b]);
 #pragma unroll
         for (int b = 0; b < kBatch; b++)
-            if (src[b] >= 0) dst[base + b * (int)blockDim.x] = v[b];
+            if (src[b] >= 0) store(base + b * (int)blockDim.x, v[b]);
     }
+}
+
+template <typename T, typename Fn>
+__device__ __forceinline__ void tab_stage(const TileTab &t, T *__restrict__ dst, Fn fn) {
+    tab_stage_with(t, fn, [&](int e, const T &v) { dst[e] = v; });
 }
 
 }  // namespace bbx
