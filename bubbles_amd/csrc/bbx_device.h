@@ -72,19 +72,26 @@ __device__ __forceinline__ bool in_bucket(const SimConst &c, float4 a, float4 b)
     return within_std_fp64(a, b, c.h2d);
 }
 
+// The reference's cutoff rule "zero when |of| < 1e-8 or of > 0"
+// (src/core/kernel.cpp:123-132,181-226) is one comparison: for any
+// non-NaN of it is "inside iff of <= -1e-8"; a NaN is outside.
+__device__ __forceinline__ bool cutoff_in(float of) { return of <= -1e-8f; }
+
 // poly6 W with its own cutoff (src/core/kernel.cpp:123-132), FP32.
 __device__ __forceinline__ float poly6_w(const SimConst &c, float d2) {
-    float of = d2 - c.h2;
-    if (fabsf(of) < 1e-8f || of > 0.f) return 0.f;
-    float x = 1.f - d2 * c.inv_h2;
-    return c.poly6 * x * x * x;
+    const float x = 1.f - d2 * c.inv_h2;
+    return cutoff_in(d2 - c.h2) ? c.poly6 * x * x * x : 0.f;
 }
 
 // spiky cutoff (src/core/kernel.cpp:181-226): zero when |r-h| < 1e-8 or r > h
-__device__ __forceinline__ bool spiky_in(const SimConst &c, float r) {
-    float of = r - c.h;
-    return !(fabsf(of) < 1e-8f || of > 0.f);
-}
+__device__ __forceinline__ bool spiky_in(const SimConst &c, float r) { return cutoff_in(r - c.h); }
+
+// Hardware reciprocal, square root and reciprocal square root (about 1 ulp,
+// one instruction each) for the gather sums; the FP64 bucket rule and the
+// hash never use them.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Kahan compensated add (FP32)
 __device__ __forceinline__ void kahan_add(float &s, float &comp, float v) {

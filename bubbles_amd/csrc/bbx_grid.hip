@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
 // (nbr32).  Candidates are visited in the reference's order: neighbour cells
 // y outer, x, z inner (src/core/grid.h:568-591), each cell in chain order.
 template <bool kPressure>
-__global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
+__global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             const float4 *__restrict__ pos,
                                                             const int *__restrict__ home,
                                                             const int *__restrict__ start,
@@ -265,9 +265,9 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
                                                             float *__restrict__ pres,
                                                             int *__restrict__ tile_tab,
                                                             int *__restrict__ counters) {
-    // staged candidate positions as x, y, z arrays (12 B per particle):
-    // five workgroups share a CU's LDS
-    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap];
+    // staged candidate positions (16-byte LDS DMA); four slack slots let the
+    // candidate test read whole groups of four past the end of a row
+    __shared__ float4 sx[kTileCap + 4];
     __shared__ TileTab tt;
     __shared__ int rdelta[9];
     __shared__ int smin, smax;
@@ -297,13 +297,46 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
     __syncthreads();
     tab_store(tile_tab + (size_t)tile * kTabInts, tt);
     const bool staged = tt.staged != 0;
-    if (staged)
-        tab_stage_with(tt, [&](int j) { return pos[j]; }, [&](int e, const float4 &v) {
-            sxx[e] = v.x;
-            sxy[e] = v.y;
-            sxz[e] = v.z;
-        });
-    auto lds_pos = [&](int e) { return make_float4(sxx[e], sxy[e], sxz[e], 0.f); };
+    // The particle's position and its nine row bounds do not depend on the
+    // staging: they are loaded before it.  Row r = (dz+1)*3 + (dy+1): the
+    // cells x-1, x, x+1 of a row are contiguous in storage; A[r] = storage
+    // start of cell x-1, and O[r] packs the offsets of cells x, x+1 and of
+    // the row end as bytes (rows outside the grid are empty).  A row of at
+    // most 64 candidates whose cells hold at most 32 each is "narrow": it is
+    // tested into a bit mask; otherwise the particle walks cell by cell.
+    const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    int u[3] = {0, 0, 0};
+    int A[9];
+    unsigned O[9];
+    bool narrow = true;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        A[r] = 0;
+        O[r] = 0u;
+    }
+    if (valid) pi = pos[i];
+    if (hc >= 0) {
+        unlinear(c.g, hc, u);
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
+            if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                const int cx = u[0] + yy * nx + zz * nxy;
+                const int b1 = start[cx], b2 = start[cx + 1];
+                const int b0 = u[0] > 0 ? start[cx - 1] : b1;
+                const int b3 = u[0] + 1 < nx ? start[cx + 2] : b2;
+                A[r] = b0;
+                O[r] = (unsigned)(b1 - b0) | ((unsigned)(b2 - b0) << 8) | ((unsigned)(b3 - b0) << 16);
+                narrow = narrow && (b3 - b0 <= c.row_max) && (b1 - b0 <= 32) && (b2 - b1 <= 32) &&
+                         (b3 - b2 <= 32);
+            }
+        }
+    }
+    if (staged) {
+        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(pos + j, sx + e); });
+        lds_dma_wait();
+    }
     __syncthreads();
     if (!valid) return;
     // The particle's work is instantiated once for LDS-staged and once for
@@ -311,23 +344,27 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
     // (that would compile to a slower generic load).
     auto per_particle = [&](auto staged_tag) {
         constexpr bool kStaged = decltype(staged_tag)::value;
-        const float4 pi = pos[i];
-        int u[3] = {0, 0, 0};
-        if (hc >= 0) unlinear(c.g, hc, u);
         int cnt = 0, dropped = 0;
         float sum = 0.f, comp = 0.f;
-        // the current quad of bucket entries, stored once complete
-        unsigned e0 = 0, e1 = 0, e2 = 0, e3 = 0;
-        auto flush_quad = [&](int k) {  // k = index of the quad's last entry + 1
+        // The current quad of bucket entries.  Staged: a 64-bit shift
+        // register of 16-bit entries held as two words (entry k of the quad
+        // ends in bits 16k..16k+15); global: four 32-bit entries.
+        unsigned qlo = 0u, qhi = 0u;
+        int e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+        auto store_quad = [&](int k, int nfill) {  // k = entries so far, nfill = entries in the quad
             const size_t at = (size_t)((k - 1) >> 2) * c.stride + i;
-            if (kStaged)
-                reinterpret_cast<ushort4 *>(nbr16)[at] =
-                    make_ushort4((unsigned short)e0, (unsigned short)e1, (unsigned short)e2,
-                                 (unsigned short)e3);
-            else
-                reinterpret_cast<int4 *>(nbr32)[at] = make_int4((int)e0, (int)e1, (int)e2, (int)e3);
+            if (kStaged) {
+                unsigned long long q = ((unsigned long long)qhi << 32) | qlo;
+                q >>= 16 * (4 - nfill);  // a partial quad moves down to slot 0
+                reinterpret_cast<uint2 *>(nbr16)[at] = make_uint2((unsigned)q, (unsigned)(q >> 32));
+            } else {
+                int4 v = make_int4(e0, e1, e2, e3);
+                if (nfill == 3) v = make_int4(e1, e2, e3, 0);
+                else if (nfill == 2) v = make_int4(e2, e3, 0, 0);
+                else if (nfill == 1) v = make_int4(e3, 0, 0, 0);
+                reinterpret_cast<int4 *>(nbr32)[at] = v;
+            }
         };
-        const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
         // append accepted candidate j (storage index; LDS index j + sh) to the
         // bucket, up to the cap, and to the density sum
         auto emit = [&](int j, int sh) {
@@ -337,85 +374,81 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
             }
             const int jl = j + sh;
             float4 pj;
-            if (kStaged)
-                pj = lds_pos(jl);
-            else
+            if (kStaged) {
+                pj = sx[jl];
+                qlo = __builtin_amdgcn_alignbit(qhi, qlo, 16);
+                qhi = __builtin_amdgcn_alignbit((unsigned)jl, qhi, 16);
+            } else {
                 pj = pos[j];
-            const unsigned v = kStaged ? (unsigned)jl : (unsigned)j;
-            const int slot = cnt & 3;
-            e0 = slot == 0 ? v : e0;
-            e1 = slot == 1 ? v : e1;
-            e2 = slot == 2 ? v : e2;
-            e3 = slot == 3 ? v : e3;
+                e0 = e1;
+                e1 = e2;
+                e2 = e3;
+                e3 = j;
+            }
             cnt++;
-            if (slot == 3) flush_quad(cnt);
-            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            if ((cnt & 3) == 0) store_quad(cnt, 4);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
         };
         if (hc >= 0) {
-            // Row r = (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are
-            // contiguous in storage.  B[r] = start of x-1, x, x+1 and end of x+1;
-            // cells and rows outside the grid are empty ranges.
-            // Row r: storage start A[r] of cell x-1 and the offsets of cells x,
-            // x+1 and the row end packed as bytes in O[r] (narrow rows hold at
-            // most 64 candidates); sh[r] = LDS - storage shift of the row.
-            int A[9], sh[9];
-            unsigned O[9];
-            bool narrow = true;
-    #pragma unroll
-            for (int r = 0; r < 9; r++) {
-                const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
-                sh[r] = kStaged ? rdelta[r] : 0;
-                A[r] = 0;
-                O[r] = 0u;
-                if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-                    const int cx = u[0] + yy * nx + zz * nxy;
-                    const int b1 = start[cx], b2 = start[cx + 1];
-                    const int b0 = u[0] > 0 ? start[cx - 1] : b1;
-                    const int b3 = u[0] + 1 < nx ? start[cx + 2] : b2;
-                    A[r] = b0;
-                    O[r] = (unsigned)(b1 - b0) | ((unsigned)(b2 - b0) << 8) | ((unsigned)(b3 - b0) << 16);
-                    narrow = narrow && (b3 - b0 <= c.row_max);
-                }
-            }
+            int sh[9];
+#pragma unroll
+            for (int r = 0; r < 9; r++) sh[r] = kStaged ? rdelta[r] : 0;
             if (narrow) {
-                // pass 1: each row's candidates tested in storage order into a mask
+                // Pass 1: each row's candidates, four at a time with no
+                // branches, into an acceptance mask.  The FP32 squared
+                // distance accepts below lo2 and rejects above hi2; a
+                // candidate between the two (rare) makes the lane re-test its
+                // row with the reference's FP64 rule.
                 unsigned long long M[9];
-    #pragma unroll
+#pragma unroll
                 for (int r = 0; r < 9; r++) {
-                    unsigned long long m = 0ull;
-                    const int a = A[r], len = (int)(O[r] >> 16), s = sh[r];
-                    // separate LDS and global loops: a select between the two
-                    // addresses would become a generic (flat) load
-                    if (kStaged) {
-                        const int row = a + s;
-                        for (int t0 = 0; t0 < len; t0 += 4) {
-                            float4 pc[4];
-    #pragma unroll
-                            for (int q = 0; q < 4; q++)
-                                if (t0 + q < len) pc[q] = lds_pos(row + t0 + q);
-    #pragma unroll
-                            for (int q = 0; q < 4; q++)
-                                if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
+                    const int len = (int)(O[r] >> 16);
+                    unsigned mlo = 0u, mhi = 0u;
+                    bool band = false;
+                    for (int t0 = 0; t0 < len; t0 += 4) {
+                        float4 pc[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            if (kStaged)
+                                pc[q] = sx[A[r] + sh[r] + t0 + q];  // slack slots cover the overrun
+                            else
+                                pc[q] = pos[min(A[r] + t0 + q, c.n - 1)];
                         }
-                    } else {
-                        const float4 *row = pos + a;
-                        for (int t0 = 0; t0 < len; t0 += 4) {
-                            float4 pc[4];
-    #pragma unroll
-                            for (int q = 0; q < 4; q++)
-                                if (t0 + q < len) pc[q] = row[t0 + q];
-    #pragma unroll
-                            for (int q = 0; q < 4; q++)
-                                if (t0 + q < len && in_bucket(c, pi, pc[q])) m |= 1ull << (t0 + q);
+                        unsigned bits = 0u;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const float dx = pi.x - pc[q].x, dy = pi.y - pc[q].y, dz = pi.z - pc[q].z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            bits |= (d2 <= c.hi2 ? 1u : 0u) << q;
+                            band = band || (d2 >= c.lo2 && d2 <= c.hi2);
+                        }
+                        if (t0 < 32)
+                            mlo |= bits << t0;
+                        else
+                            mhi |= bits << (t0 - 32);
+                    }
+                    unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+                    if (len < 64) m &= (1ull << len) - 1ull;
+                    if (band) {
+                        // candidates inside the band: the exact FP64 rule
+                        for (int t = 0; t < len; t++) {
+                            const float4 pj = kStaged ? sx[A[r] + sh[r] + t] : pos[A[r] + t];
+                            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 >= c.lo2 && d2 <= c.hi2) {
+                                const unsigned long long bit = 1ull << t;
+                                m = within_std_fp64(pi, pj, c.h2d) ? (m | bit) : (m & ~bit);
+                            }
                         }
                     }
                     M[r] = m;
                 }
-                // pass 2: emit in the reference's order, cells (y, x, z), chain
+                // Pass 2: emit in the reference's order, cells (y, x, z), chain
                 // order inside each cell.  x runs as a loop with its cell bounds
                 // selected, which keeps the kernel's code (nine emission sites
-                // instead of 27) small.
+                // instead of 27) small.  A narrow row's cells hold <= 32
+                // candidates, so each cell's bits fit one 32-bit word.
 #pragma unroll
                 for (int dy = 0; dy < 3; dy++)
 #pragma unroll 1
@@ -427,16 +460,16 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
                             const int hi = (int)((O[r] >> (8 * dx)) & 0xffu);
                             const int w = hi - lo;
                             if (w <= 0) continue;
-                            unsigned long long m = M[r] >> lo;
-                            if (w < 64) m &= (1ull << w) - 1ull;
+                            unsigned m = (unsigned)(M[r] >> lo);
+                            if (w < 32) m &= (1u << w) - 1u;
                             while (m) {
-                                const int b = __builtin_ctzll(m);
-                                m &= m - 1ull;
+                                const int b = __builtin_ctz(m);
+                                m &= m - 1u;
                                 emit(A[r] + lo + b, sh[r]);
                             }
                         }
             } else {
-                // a row with more than 64 candidates: cell by cell
+                // a wide row or a crowded cell: cell by cell
                 for (int dy = 0; dy < 3; dy++)
                     for (int dx = 0; dx < 3; dx++) {
                         const int x = u[0] + dx - 1;
@@ -445,21 +478,17 @@ __global__ void __launch_bounds__(kBlock, 5) k_buckets_density(SimConst c,
                             const int yy = u[1] + dy - 1, zz = u[2] + dz - 1;
                             if (yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
                             const int cell = x + yy * nx + zz * nxy;
-                            const int sr = kStaged ? rdelta[dz * 3 + dy] : 0;
+                            const int sr = sh[dz * 3 + dy];
                             const int b = start[cell], e = b + count[cell];
                             for (int j = b; j < e; j++) {
-                                float4 pj;
-                                if (kStaged)
-                                    pj = lds_pos(j + sr);
-                                else
-                                    pj = pos[j];
+                                const float4 pj = kStaged ? sx[j + sr] : pos[j];
                                 if (in_bucket(c, pi, pj)) emit(j, sr);
                             }
                         }
                     }
             }
         }
-        if (cnt & 3) flush_quad(cnt);
+        if (cnt & 3) store_quad(cnt, cnt & 3);
         nbr_cnt[i] = cnt;
         float di = sum * c.mass;
         rho[i] = di;
@@ -496,10 +525,11 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
     const int cnt = nbr_cnt[i];
     with_staging(staged, [&](auto tag) {
         constexpr bool kS = decltype(tag)::value;
-        for_each_neighbour<2>(nbr16, nbr32, kS, i, cnt, c.stride, [&](int j) {
+        for_each_neighbour<2>(nbr16, nbr32, kS, i, cnt, c.stride, [&](int j, bool live) {
             const float4 pj = kS ? sx[j] : pos[j];
             float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+            const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
+            kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
         });
     });
     rho[i] = sum * c.mass;

@@ -89,7 +89,7 @@ __device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float
 constexpr int kNpThreads = 2 * kBlock;
 
 template <int kMode>
-__global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
+__global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu(4))) k_nonpressure(SimConst c, StepConst st, NbrTab nb,
                                                             const float4 *__restrict__ pos,
                                                             const float4 *__restrict__ vel,
                                                             const float *__restrict__ rho,
@@ -102,8 +102,11 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
                                                             float *__restrict__ q,
                                                             float *__restrict__ pwarm) {
     constexpr bool kFeat = kMode >= 2;
-    __shared__ float4 sx[kTileCap2];
-    __shared__ float4 sv[kTileCap2];
+    // neighbour (position) and (velocity, density) as float4 arrays; in mode
+    // 3 the same pool then holds the block's features
+    __shared__ float pool[8 * kTileCap2];
+    float4 *const sx = reinterpret_cast<float4 *>(pool);
+    float4 *const sv = reinterpret_cast<float4 *>(pool + 4 * kTileCap2);
     __shared__ TileTab tt;
     const int tile = tile_of_block();
     tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
@@ -115,18 +118,22 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
     const bool valid = i < c.n;
     int cnt = 0;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
-    int4 q0 = make_int4(0, 0, 0, 0), q1 = q0;  // this lane's first two quads: half, half + 2
+    float rhoi = 0.f;
+    // this lane's first two quads: half, half + 2
+    NbrQuads<2> first{};
     if (valid) {  // independent of the staging: requested before it
         cnt = nb.cnt[i];
         xi = pos[i];
         vi = vel[i];
-        q0 = nbr_quad(nb.n16, nb.n32, staged, i, c.stride, half);
-        q1 = nbr_quad(nb.n16, nb.n32, staged, i, c.stride, half + 2);
+        rhoi = rho[i];
+        first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
     }
+    // staged through registers: the (velocity, density) pair is assembled on
+    // the way, and seven 4-byte LDS DMA streams measured slower here
     if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return pos[j]; });
         tab_stage(tt, sv, [&](int j) {
-            float4 v = vel[j];
+            const float4 v = vel[j];
             return make_float4(v.x, v.y, v.z, rho[j]);
         });
     }
@@ -138,64 +145,59 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
         const float coef = c.visc * c.mass2;
         float fx = 0.f, fy = 0.f, fz = 0.f;
         float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-        auto body = [&](int j) {
-            float4 xj, vj;
-            if (in_lds) {
-                xj = sx[j];
-                vj = sv[j];
-            } else {
-                const int g = staged ? tab_global(tt, j) : j;
-                xj = pos[g];
-                float4 v = vel[g];
-                vj = make_float4(v.x, v.y, v.z, rho[g]);
-            }
-            const float rj = vj.w;
-            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            float w = 0.f, x = 0.f;
-            if (spiky_in(c, r)) {
-                x = 1.f - r * c.inv_h;
-                w = c.spiky_lap * x;
-            }
-            float s = coef * w / rj;
-            fx += (vj.x - vi.x) * s;
-            fy += (vj.y - vi.y) * s;
-            fz += (vj.z - vi.z) * s;
-            if (kFeat && j != self) {
-                float u = r * c.inv_h;
-                m1 += u;
-                m2 += u * u;
-                m3 += u * u * u;
-                cx -= dx * c.inv_h;
-                cy -= dy * c.inv_h;
-                cz -= dz * c.inv_h;
-                if (r > 0.f && x > 0.f) {
-                    // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
-                    float g = -c.spiky_grad * x * x / r;
-                    float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-                    divv += c.mass / rj * g * (dvx * dx + dvy * dy + dvz * dz);
+        with_staging(in_lds, [&](auto tag) {
+            constexpr bool kL = decltype(tag)::value;
+            auto body = [&](int j, bool live) {
+                float xjx, xjy, xjz, vjx, vjy, vjz, rj;
+                if (kL) {
+                    const float4 a = sx[j], b = sv[j];
+                    xjx = a.x;
+                    xjy = a.y;
+                    xjz = a.z;
+                    vjx = b.x;
+                    vjy = b.y;
+                    vjz = b.z;
+                    rj = b.w;
+                } else {
+                    const int g = staged ? tab_global(tt, j) : j;
+                    const float4 p = pos[g], v = vel[g];
+                    xjx = p.x;
+                    xjy = p.y;
+                    xjz = p.z;
+                    vjx = v.x;
+                    vjy = v.y;
+                    vjz = v.z;
+                    rj = rho[g];
                 }
-            }
-        };
-        // quads half, half + 2, half + 4, ... of the bucket, two in flight
-        const int nq = (cnt + 3) >> 2;
-        for (int qd = half; qd < nq; qd += 4) {
-            const int4 n0 = (qd + 4) < nq ? nbr_quad(nb.n16, nb.n32, staged, i, c.stride, qd + 4)
-                                          : make_int4(0, 0, 0, 0);
-            const int4 n1 = (qd + 6) < nq ? nbr_quad(nb.n16, nb.n32, staged, i, c.stride, qd + 6)
-                                          : make_int4(0, 0, 0, 0);
-            const int k0 = qd * 4, k1 = (qd + 2) * 4;
-            if (k0 < cnt) body(q0.x);
-            if (k0 + 1 < cnt) body(q0.y);
-            if (k0 + 2 < cnt) body(q0.z);
-            if (k0 + 3 < cnt) body(q0.w);
-            if (k1 < cnt) body(q1.x);
-            if (k1 + 1 < cnt) body(q1.y);
-            if (k1 + 2 < cnt) body(q1.z);
-            if (k1 + 3 < cnt) body(q1.w);
-            q0 = n0;
-            q1 = n1;
-        }
+                const float dx = xi.x - xjx, dy = xi.y - xjy, dz = xi.z - xjz;
+                const float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+                const bool in = live && spiky_in(c, r);
+                const float x = in ? 1.f - r * c.inv_h : 0.f;
+                const float irj = fast_rcp(rj);
+                // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j; dead entries add exact zeros
+                const float s = coef * (c.spiky_lap * x) * irj;
+                const float dvx = vjx - vi.x, dvy = vjy - vi.y, dvz = vjz - vi.z;
+                fx += dvx * s;
+                fy += dvy * s;
+                fz += dvz * s;
+                if (kFeat && live && j != self) {
+                    const float u = r * c.inv_h;
+                    m1 += u;
+                    m2 += u * u;
+                    m3 += u * u * u;
+                    cx -= dx * c.inv_h;
+                    cy -= dy * c.inv_h;
+                    cz -= dz * c.inv_h;
+                    if (r > 0.f && x > 0.f) {
+                        // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
+                        const float g = -c.spiky_grad * x * x * fast_rcp(r);
+                        divv += c.mass * irj * g * (dvx * dx + dvy * dy + dvz * dz);
+                    }
+                }
+            };
+            // quads half, half + 2, half + 4, ... of the bucket, two in flight
+            for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, body, half);
+        });
         // the pair's partial sums
         fx += __shfl_xor(fx, 1, 64);
         fy += __shfl_xor(fy, 1, 64);
@@ -216,7 +218,7 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
             force[i] = make_float4(fx, fy, fz, 0.f);
             if (kMode == 1) {
                 xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
-                rho_star[i] = rho[i];
+                rho_star[i] = rhoi;
             }
         }
         if (kFeat && valid) {
@@ -233,7 +235,7 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
             float vn = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
             float fn = sqrtf(fx * fx + fy * fy + fz * fz);
             float a_scale = st.dt * st.dt / (c.mass * c.h);
-            fv[BBX_F_DENSITY_ERR] = rho[i] / c.rho0 - 1.f;
+            fv[BBX_F_DENSITY_ERR] = rhoi / c.rho0 - 1.f;
             fv[BBX_F_DIVV] = st.dt * divv;
             fv[BBX_F_COUNT] = (float)cnt;
             fv[BBX_F_M1] = m1 * inv;
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
             if (half == 0) {
 #pragma unroll
                 for (int f = 0; f < BBX_NFEAT; f++) feat[(size_t)f * S + i] = fv[f];
-                rho_star[i] = rho[i];
+                rho_star[i] = rhoi;
             }
         }
     }
@@ -261,9 +263,10 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
         // each of the eight waves infers 32 particles as two 16-sample tiles.
         __syncthreads();  // every thread is done with the staged neighbours
         constexpr int kRow = BBX_NFEAT + 1;  // padded feature row
-        float *lf = reinterpret_cast<float *>(sx);
-        float *lp = lf + kBlock * kRow;
-        float *lb = reinterpret_cast<float *>(sv);  // b1 | b2 | w3
+        float *lf = pool;               // 256 padded feature rows
+        float *lp = lf + kBlock * kRow; // 256 warm-start pressures
+        float *lb = pool + 5 * kTileCap2;  // b1 | b2 | w3
+        static_assert(kBlock * (kRow + 1) <= 5 * kTileCap2 && 3 * kH <= 3 * kTileCap2, "mode-3 LDS layout");
         if (half == 0) {
 #pragma unroll
             for (int f = 0; f < BBX_NFEAT; f++) lf[pl * kRow + f] = valid ? fv[f] : 0.f;
@@ -304,7 +307,7 @@ __global__ void __launch_bounds__(kNpThreads) k_nonpressure(SimConst c, StepCons
         __syncthreads();
         if (valid && half == 0) {
             const float p = lp[pl];
-            const float d = rho[i];
+            const float d = rhoi;
             const float d2 = d * d;
             pwarm[i] = p;
             pres[i] = p;
@@ -326,9 +329,9 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
                                                              float *__restrict__ rho_star,
                                                              float *__restrict__ q,
                                                              LoopState *__restrict__ loop, int k) {
-    // x*, y*, z* as separate arrays (12 B per particle): five workgroups
-    // share a CU's LDS instead of four
-    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap];
+    // predicted positions, copied by 16-byte LDS DMA (the w lane is the
+    // unused fourth word of x*)
+    __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kBlock / 64];
     if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
@@ -348,28 +351,28 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
         cnt = nb.cnt[i];
         xi = xstar[i];
     }
-    if (staged)
-        tab_stage_with(tt, [&](int j) { return xstar[j]; }, [&](int e, const float4 &v) {
-            sxx[e] = v.x;
-            sxy[e] = v.y;
-            sxz[e] = v.z;
-        });
+    if (staged) {
+        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
+        lds_dma_wait();
+    }
     __syncthreads();
     unsigned long long ebits = 0ull;
     if (valid) {
         float sum = 0.f, comp = 0.f;
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
-            for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j) {
+            for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
                 float3 xj;
                 if (kS) {
-                    xj = make_float3(sxx[j], sxy[j], sxz[j]);
+                    const float4 a = sx[j];
+                    xj = make_float3(a.x, a.y, a.z);
                 } else {
                     const float4 g = xstar[j];
                     xj = make_float3(g.x, g.y, g.z);
                 }
                 float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
+                const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
+                kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
             });
         });
         float density = sum * c.mass;
@@ -428,7 +431,8 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            unsigned *__restrict__ max_force,
                                                            LoopState *__restrict__ loop, int k,
                                                            int max_it, int exact, double tol) {
-    __shared__ float4 sx[kTileCap];
+    // position x, y, z and q = p / rho*^2 as four arrays
+    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap], sq[kTileCap];
     __shared__ TileTab tt;
     if (k > 0 && loop->done[k - 1]) {
         // an earlier iteration integrated: pass the flag on and exit (uniform)
@@ -461,46 +465,65 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
     const bool valid = i < c.n;
     NbrQuads<kNbrDepth> first{};
     int cnt = 0;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), fi = xi, vi = xi;
     float qi = 0.f;
     if (valid) {  // independent of the staging: requested before it
         first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
         cnt = nb.cnt[i];
         xi = pos[i];
         qi = q[i];
+        fi = force[i];
+        vi = vel[i];
     }
     if (staged)
-        tab_stage(tt, sx, [&](int j) {
-            float4 p = pos[j];
+        tab_stage_with(tt, [&](int j) {
+            const float4 p = pos[j];
             return make_float4(p.x, p.y, p.z, q[j]);
+        }, [&](int e, const float4 &v) {
+            sxx[e] = v.x;
+            sxy[e] = v.y;
+            sxz[e] = v.z;
+            sq[e] = v.w;
         });
     __syncthreads();
     unsigned fbits = 0u;
     if (valid) {
-        const int self = staged ? tab_local(tt, i) : i;
+        // pairs j != i with 1e-8 <= r and r inside the spiky cutoff
+        // (src/equations/pcisph_equations3.cpp:128-150); r = d2 * rsqrt(d2)
+        // is NaN for d2 = 0 (the particle itself or a coincident one), which
+        // fails both tests
+        const float kg = c.mass2 * c.spiky_grad;
         float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-        for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
-            if (j == self) return;
-            float4 xj;
-            if (staged) {
-                xj = sx[j];
-            } else {
-                float4 p = pos[j];
-                xj = make_float4(p.x, p.y, p.z, q[j]);
-            }
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (!spiky_in(c, r) || !(r > 0.f) || fabsf(r) < 1e-8f) return;
-            float x = 1.f - r * c.inv_h;
-            // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r
-            float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
-            kahan_add(tx, ex, g * dx);
-            kahan_add(ty, ey, g * dy);
-            kahan_add(tz, ez, g * dz);
+        with_staging(staged, [&](auto tag) {
+            constexpr bool kS = decltype(tag)::value;
+            for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
+                float xjx, xjy, xjz, qj;
+                if (kS) {
+                    xjx = sxx[j];
+                    xjy = sxy[j];
+                    xjz = sxz[j];
+                    qj = sq[j];
+                } else {
+                    const float4 p = pos[j];
+                    xjx = p.x;
+                    xjy = p.y;
+                    xjz = p.z;
+                    qj = q[j];
+                }
+                const float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float ir = fast_rsq(d2);
+                const float r = d2 * ir;
+                const bool in = live && spiky_in(c, r) && r >= 1e-8f;
+                const float x = 1.f - r * c.inv_h;
+                // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r
+                const float g = in ? kg * (qi + qj) * x * x * ir : 0.f;  // dead: exact zero
+                kahan_add(tx, ex, g * dx);
+                kahan_add(ty, ey, g * dy);
+                kahan_add(tz, ez, g * dz);
+            });
         });
         const float fpx = -tx, fpy = -ty, fpz = -tz;
-        const float4 fi = force[i];
-        const float4 vi = vel[i];
         if (!integrate) {
             xstar[i] = predict_position(c, st, xi, vi, fi.x + fpx, fi.y + fpy, fi.z + fpz);
         } else {
@@ -562,8 +585,8 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     float fz = c.mass * c.gz - c.drag * vi.z;
     float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
     const int cnt = nb.cnt[i];
-    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
-        if (j == self) return;
+    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j, bool live) {
+        if (!live || j == self) return;
         float4 xj, vj;
         if (in_lds) {
             xj = sx[j];
@@ -643,7 +666,8 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     float sx_ = 0.f, sy_ = 0.f, sz_ = 0.f, ws = 0.f;
     const float spiky_w = c.spiky_lap * c.h * c.h / 6.f;  // 15 / (pi h^3)
     const int cnt = nb.cnt[i];
-    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j) {
+    for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j, bool live) {
+        if (!live) return;
         float4 xj, vj;
         if (in_lds) {
             xj = sx[j];

@@ -179,42 +179,56 @@ __device__ __forceinline__ int4 nbr_quad(const unsigned short *__restrict__ n16,
     return reinterpret_cast<const int4 *>(n32)[at];
 }
 
-template <int D>
+// quads q0, q0 + step, ..., q0 + (D - 1) step of particle i (q0 + (D-1) step < 25)
+template <int D, int kStep = 1>
 __device__ __forceinline__ NbrQuads<D> nbr_prefetch(const unsigned short *__restrict__ n16,
                                                     const int *__restrict__ n32, bool staged,
-                                                    int i, int stride) {
+                                                    int i, int stride, int q0 = 0) {
     NbrQuads<D> p;
 #pragma unroll
-    for (int d = 0; d < D; d++) p.q[d] = nbr_quad(n16, n32, staged, i, stride, d);
+    for (int d = 0; d < D; d++) p.q[d] = nbr_quad(n16, n32, staged, i, stride, q0 + d * kStep);
     return p;
+}
+
+// The four entries of one quad go through body(j, live) without branches:
+// an entry at or past the bucket count gets j = 0 (a valid index in either
+// index space) and live = false.  Straight-line code lets the four entries'
+// neighbour loads issue together instead of one load-wait per entry.  The
+// body must make a dead entry contribute nothing (or an exact zero).
+template <typename Body>
+__device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &body) {
+    body(k < cnt ? v.x : 0, k < cnt);
+    body(k + 1 < cnt ? v.y : 0, k + 1 < cnt);
+    body(k + 2 < cnt ? v.z : 0, k + 2 < cnt);
+    body(k + 3 < cnt ? v.w : 0, k + 3 < cnt);
 }
 
 // Walk bucket slots 0..cnt-1 of particle i in bucket order, starting from
 // the prefetched first D quads and keeping the next D quads in flight while
-// the current D are processed.  body(j) gets an LDS-local index (staged
-// tile) or a storage index.
-template <int D, typename Body>
+// the current D are processed (quads 0, step, 2 step, ... from quad `first0`;
+// step > 1 splits a bucket between lanes).  body(j, live) gets an LDS-local
+// index (staged tile) or a storage index.
+template <int D, int kStep = 1, typename Body>
 __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first,
                                                         const unsigned short *__restrict__ n16,
                                                         const int *__restrict__ n32, bool staged,
-                                                        int i, int cnt, int stride, Body body) {
+                                                        int i, int cnt, int stride, Body body,
+                                                        int first0 = 0) {
     const int nq = (cnt + 3) >> 2;
     int4 cur[D];
 #pragma unroll
     for (int d = 0; d < D; d++) cur[d] = first.q[d];
-    for (int q = 0; q < nq; q += D) {
+    for (int q = first0; q < nq; q += D * kStep) {
         int4 nxt[D];
 #pragma unroll
-        for (int d = 0; d < D; d++)
-            nxt[d] = (q + D + d) < nq ? nbr_quad(n16, n32, staged, i, stride, q + D + d)
-                                      : make_int4(0, 0, 0, 0);
+        for (int d = 0; d < D; d++) {
+            const int qn = q + (D + d) * kStep;
+            nxt[d] = qn < nq ? nbr_quad(n16, n32, staged, i, stride, qn) : make_int4(0, 0, 0, 0);
+        }
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            const int k = (q + d) * 4;
-            if (k < cnt) body(cur[d].x);
-            if (k + 1 < cnt) body(cur[d].y);
-            if (k + 2 < cnt) body(cur[d].z);
-            if (k + 3 < cnt) body(cur[d].w);
+            const int qc = q + d * kStep;
+            if (qc < nq) quad_body(cur[d], qc * 4, cnt, body);
         }
 #pragma unroll
         for (int d = 0; d < D; d++) cur[d] = nxt[d];
@@ -228,6 +242,36 @@ __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restr
     const NbrQuads<D> first = nbr_prefetch<D>(n16, n32, staged, i, stride);
     for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body);
 }
+
+// LDS DMA (gfx950 global_load_lds): each active lane of the wave copies
+// 16 bytes from its own global address to LDS at wave_dst + lane * 16,
+// wave_dst being the same for the whole wave.  No VGPR round trip and no
+// LDS store instruction; every load of the tile can be in flight at once.
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void lds_dma16(const void *src, float4 *wave_dst) {
+    __builtin_amdgcn_global_load_lds(src, (lds_void *)wave_dst, 16, 0, 0);
+}
+
+// Stage the tile's intervals with LDS DMA: copy(j, e) is called for storage
+// index j by lanes whose LDS slot is e + lane, e being the slot of the wave's
+// lane 0 (wave-uniform).  Each interval is cut into 64-slot runs, one per
+// wave per pass.  The caller then waits (lds_dma_wait) and synchronises.
+template <typename Copy>
+__device__ __forceinline__ void tab_stage_dma(const TileTab &t, Copy copy) {
+    const int lane = (int)threadIdx.x & 63;
+    const int w0 = (int)threadIdx.x - lane;
+    const int nivl = t.n_ivl;
+    for (int k = 0; k < nivl; k++) {
+        const int len = t.len[k], s0 = t.start[k], o0 = t.off[k];
+        for (int b = w0; b < len; b += (int)blockDim.x)
+            if (b + lane < len) copy(s0 + b + lane, o0 + b);
+    }
+}
+
+// The wave's outstanding loads (LDS DMA included) have landed; a barrier
+// must follow before other waves read them.
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Cooperative copy of the tile's intervals into LDS: dst[off + e] = fn(start + e).
 // The LDS image is the concatenation of the intervals, so thread t copies
