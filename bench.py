@@ -43,6 +43,7 @@ KERNEL_BYTES = {
     "grid_hash": 36,                   # x, old cell -> new cell, key, rank, hash
     "grid_scatter": 16,                # new cell, rank, start -> slot index
     "grid_rank_move": 76,              # slot, x+id, v+p -> x+id, v+p, cell, home
+    "tile_tables": 4,                  # home -> per-tile interval tables
     "buckets_density": 20,             # x -> rho, count; + 2 B per bucket entry (see below)
     "nonpressure_predict": 56,         # x, v, rho -> f, x*, rho*
     "nonpressure_features": 108,       # x, v, rho -> f, rho*, 16 features

@@ -287,6 +287,11 @@ void fill_sim_const(bbx_solver *s) {
         c.lo2 = -1.f;         // always take the FP64 path
         c.hi2 = 3.4e38f;
     }
+    // one-comparison form of the band: every d2 in [lo2, hi2] has
+    // |d2 - band_c| <= band_w (d2 - band_c is exact near the cutoff)
+    c.band_c = (float)(((double)c.lo2 + (double)c.hi2) / 2.0);
+    c.band_w = (float)(((double)c.hi2 - (double)c.lo2) / 2.0 * 1.001);
+    for (int k = 0; k < 4; k++) c.band_w = std::nextafter(c.band_w, 3.4e38f);
     c.mass = (float)s->mass;
     c.mass2 = (float)(s->mass * s->mass);
     c.rho0 = (float)s->prm.target_density;

@@ -39,7 +39,7 @@ int fail(int code, const std::string &msg);
 constexpr int kBucket = BBX_BUCKET_SIZE;   // reference bucket cap
 constexpr int kBlock = 256;                // threads per workgroup (4 wave64)
 constexpr int kScanItems = 8;              // items per thread in the cell scan
-constexpr int kTabIntsHost = 32;           // ints per tile table (see bbx_tile.h)
+constexpr int kTabIntsHost = 48;           // ints per tile table (see bbx_tile.h)
 // LDS tile capacity in particles (bbx_tile.h): 2528 float4 (+ the tile
 // table) lets four single-array or two two-array workgroups share a CU's
 // 160 KB of LDS
@@ -77,6 +77,7 @@ struct SimConst {
     double h2d;        // FP64 kernel radius squared for the bucket rule
     double rho0d;      // FP64 rest density (convergence ratio)
     float lo2, hi2;    // FP32 prefilter band around the FP64 cutoff
+    float band_c, band_w;  // the same band as |d2 - band_c| <= band_w (slightly wider)
     float mass, mass2, rho0, visc, drag, nps;
     float gx, gy, gz;
     float radius;      // collision radius

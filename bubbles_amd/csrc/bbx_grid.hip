@@ -244,6 +244,43 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
 }
 
 // ---------------------------------------------------------------------------
+// 5. tile tables: one wave per tile of kBlock particles
+// ---------------------------------------------------------------------------
+// The tile's home cells span [cmin, cmax] (min and max over its particles;
+// storage is sorted by cell, except particles in no chain at the end); its
+// nine neighbour rows become at most nine merged storage intervals
+// (tab_row, tab_merge).  Every gather of the substep reads this table.
+constexpr int kTabWaves = kBlock / 64;
+__global__ void __launch_bounds__(kBlock) k_tile_tables(SimConst c, const int *__restrict__ home,
+                                                        const int *__restrict__ start, int ntiles,
+                                                        int *__restrict__ tile_tab) {
+    __shared__ TileTab tt[kTabWaves];
+    __shared__ int row_a[kTabWaves][9], row_b[kTabWaves][9];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * kTabWaves + w;  // wave-uniform
+    int lo = 0x7fffffff, hi = -1;
+    if (tile < ntiles) {
+#pragma unroll
+        for (int k = 0; k < kBlock / 64; k++) {
+            const int i = tile * kBlock + k * 64 + lane;
+            const int hc = i < c.n ? home[i] : -1;
+            if (hc >= 0) {
+                lo = min(lo, hc);
+                hi = max(hi, hc);
+            }
+        }
+    }
+    lo = wave_min_i32(lo);
+    hi = wave_max_i32(hi);
+    if (tile < ntiles && lane < 9) tab_row(c.g, hi >= 0 ? lo : 0, hi, start, lane, row_a[w][lane], row_b[w][lane]);
+    __syncthreads();
+    if (tile < ntiles && lane == 0) tab_merge(row_a[w], row_b[w], c.tile_cap, tt[w]);
+    __syncthreads();
+    if (tile < ntiles && lane < kTabWords)
+        tile_tab[(size_t)tile * kTabInts + lane] = reinterpret_cast<const int *>(&tt[w])[lane];
+}
+
+// ---------------------------------------------------------------------------
 // 5. buckets + density (+ Tait pressure for SPH)
 // ---------------------------------------------------------------------------
 // ComputeDensityFor (src/equations/sph_equations3.cpp:25-58) sums poly6 over
@@ -263,47 +300,24 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             int *__restrict__ nbr_cnt,
                                                             float *__restrict__ rho,
                                                             float *__restrict__ pres,
-                                                            int *__restrict__ tile_tab,
+                                                            const int *__restrict__ tile_tab,
                                                             int *__restrict__ counters) {
     // staged candidate positions (16-byte LDS DMA); four slack slots let the
     // candidate test read whole groups of four past the end of a row
     __shared__ float4 sx[kTileCap + 4];
     __shared__ TileTab tt;
-    __shared__ int rdelta[9];
-    __shared__ int smin, smax;
-    __shared__ int row_a[9], row_b[9];
     const int tile = tile_of_block();
     const int i = tile * kBlock + threadIdx.x;
     const bool valid = i < c.n;
     const int hc = valid ? home[i] : -1;
-    if (threadIdx.x == 0) {
-        smin = 0x7fffffff;
-        smax = -1;
-    }
-    __syncthreads();
-    {
-        const int wmin = wave_min_i32(hc >= 0 ? hc : 0x7fffffff);
-        const int wmax = wave_max_i32(hc);
-        if ((threadIdx.x & 63) == 0 && wmax >= 0) {
-            atomicMin(&smin, wmin);
-            atomicMax(&smax, wmax);
-        }
-    }
-    __syncthreads();
-    // the nine row ranges in parallel, then one thread merges them
-    if (threadIdx.x < 9) tab_row(c.g, smin, smax, start, threadIdx.x, row_a[threadIdx.x], row_b[threadIdx.x]);
-    __syncthreads();
-    if (threadIdx.x == 0) tab_merge(row_a, row_b, c.tile_cap, tt, rdelta);
-    __syncthreads();
-    tab_store(tile_tab + (size_t)tile * kTabInts, tt);
-    const bool staged = tt.staged != 0;
     // The particle's position and its nine row bounds do not depend on the
-    // staging: they are loaded before it.  Row r = (dz+1)*3 + (dy+1): the
-    // cells x-1, x, x+1 of a row are contiguous in storage; A[r] = storage
-    // start of cell x-1, and O[r] packs the offsets of cells x, x+1 and of
-    // the row end as bytes (rows outside the grid are empty).  A row of at
-    // most 64 candidates whose cells hold at most 32 each is "narrow": it is
-    // tested into a bit mask; otherwise the particle walks cell by cell.
+    // tile table or the staging: they are requested first.  Row r =
+    // (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are contiguous in
+    // storage; A[r] = storage start of cell x-1, and O[r] packs the offsets
+    // of cells x, x+1 and of the row end as bytes (rows outside the grid are
+    // empty).  A row of at most 64 candidates whose cells hold at most 32
+    // each is "narrow": it is tested into a bit mask; otherwise the particle
+    // walks cell by cell.
     const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     int u[3] = {0, 0, 0};
@@ -333,6 +347,9 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
         }
     }
+    tab_load_shared(tile_tab + (size_t)tile * kTabInts, tt);
+    __syncthreads();
+    const bool staged = tt.staged != 0;
     if (staged) {
         tab_stage_dma(tt, [&](int j, int e) { lds_dma16(pos + j, sx + e); });
         lds_dma_wait();
@@ -366,20 +383,18 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
         };
         // append accepted candidate j (storage index; LDS index j + sh) to the
-        // bucket, up to the cap, and to the density sum
+        // bucket, up to the cap.  Only the index is handled here: the emission
+        // loops run as long as the wave's longest cell, so the density sum
+        // runs afterwards over the finished bucket, one uniform pass.
         auto emit = [&](int j, int sh) {
             if (cnt >= kBucket) {
                 dropped++;
                 return;
             }
-            const int jl = j + sh;
-            float4 pj;
             if (kStaged) {
-                pj = sx[jl];
                 qlo = __builtin_amdgcn_alignbit(qhi, qlo, 16);
-                qhi = __builtin_amdgcn_alignbit((unsigned)jl, qhi, 16);
+                qhi = __builtin_amdgcn_alignbit((unsigned)(j + sh), qhi, 16);
             } else {
-                pj = pos[j];
                 e0 = e1;
                 e1 = e2;
                 e2 = e3;
@@ -387,13 +402,11 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
             cnt++;
             if ((cnt & 3) == 0) store_quad(cnt, 4);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            kahan_add(sum, comp, poly6_w(c, dx * dx + dy * dy + dz * dz));
         };
         if (hc >= 0) {
             int sh[9];
 #pragma unroll
-            for (int r = 0; r < 9; r++) sh[r] = kStaged ? rdelta[r] : 0;
+            for (int r = 0; r < 9; r++) sh[r] = kStaged ? tt.rdelta[r] : 0;
             if (narrow) {
                 // Pass 1: each row's candidates, four at a time with no
                 // branches, into an acceptance mask.  The FP32 squared
@@ -405,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                 for (int r = 0; r < 9; r++) {
                     const int len = (int)(O[r] >> 16);
                     unsigned mlo = 0u, mhi = 0u;
-                    bool band = false;
+                    float bdist = 3.4e38f;
                     for (int t0 = 0; t0 < len; t0 += 4) {
                         float4 pc[4];
 #pragma unroll
@@ -421,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                             const float dx = pi.x - pc[q].x, dy = pi.y - pc[q].y, dz = pi.z - pc[q].z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
                             bits |= (d2 <= c.hi2 ? 1u : 0u) << q;
-                            band = band || (d2 >= c.lo2 && d2 <= c.hi2);
+                            bdist = fminf(bdist, fabsf(d2 - c.band_c));  // closest to the cutoff
                         }
                         if (t0 < 32)
                             mlo |= bits << t0;
@@ -430,7 +443,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                     }
                     unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
                     if (len < 64) m &= (1ull << len) - 1ull;
-                    if (band) {
+                    if (bdist <= c.band_w) {
                         // candidates inside the band: the exact FP64 rule
                         for (int t = 0; t < len; t++) {
                             const float4 pj = kStaged ? sx[A[r] + sh[r] + t] : pos[A[r] + t];
@@ -489,6 +502,16 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
         }
         if (cnt & 3) store_quad(cnt, cnt & 3);
+        // ComputeDensityFor over the bucket in bucket order, read back from
+        // the table this lane has just written (its own quads: no other lane
+        // or wave writes them, and no earlier load in this kernel cached them)
+        __builtin_amdgcn_s_waitcnt(0);
+        for_each_neighbour<kNbrDepth>(nbr16, nbr32, kStaged, i, cnt, c.stride, [&](int j, bool live) {
+            const float4 pj = kStaged ? sx[j] : pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
+            kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
+        });
         nbr_cnt[i] = cnt;
         float di = sum * c.mass;
         rho[i] = di;
@@ -582,6 +605,13 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
         std::swap(s->cellA, s->cellB);
     }
     if (build_buckets && n > 0) {
+        {
+            BBX_PROF_BEGIN("tile_tables");
+            k_tile_tables<<<(unsigned)((s->ntiles + kTabWaves - 1) / kTabWaves), kBlock, 0, s->stream>>>(
+                c, s->home, s->cell_start, s->ntiles, s->tile_tab);
+            BBX_HIP_CHECK(hipGetLastError());
+            BBX_PROF_END();
+        }
         BBX_PROF_BEGIN("buckets_density");
         if (with_pressure)
             k_buckets_density<true><<<nblocks(n), kBlock, 0, s->stream>>>(

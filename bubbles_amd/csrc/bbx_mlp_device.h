@@ -50,10 +50,13 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
 // GELU, tanh form: 0.5 z (1 + tanh(u)) = z s with s = 1 / (1 + exp(-2u)) and
 // u = sqrt(2/pi) (z + 0.044715 z^3); exp and the reciprocal are the hardware
 // approximations (about 1 ulp), far below the BF16 rounding that follows.
+// The exponent is folded into one polynomial for the hardware exp2:
+// -2 u log2(e) = z (a + b z^2), a = -2 sqrt(2/pi) log2(e), b = 0.044715 a.
 __device__ __forceinline__ float gelu_sig(float z) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float u = k0 * (z + k1 * (z * z) * z);
-    return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
+    const float a = -2.f * 0.7978845608028654f * 1.4426950408889634f;
+    const float b = a * 0.044715f;
+    const float t = z * fmaf(b, z * z, a);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 
 // value and derivative: gelu'(z) = s + 2 z s (1 - s) u'(z)

@@ -40,14 +40,17 @@ __device__ __forceinline__ int tile_of_block() {
 }
 constexpr int kTabInts = kTabIntsHost;  // per-tile table stride in ints
 
-// per-tile table, stored in global memory with the same 30-int layout
-// (padded to kTabInts): intervals, staged flag, staged total, then the
-// storage start, length and LDS offset of each interval
+// per-tile table, stored in global memory with the same 39-int layout
+// (padded to kTabInts): intervals, staged flag, staged total, the storage
+// start, length and LDS offset of each interval, then for each of the nine
+// rows (see tab_row) LDS index - storage index inside that row's interval
 struct TileTab {
     int n_ivl, staged, total;
     int start[kMaxIvl], len[kMaxIvl], off[kMaxIvl];
+    int rdelta[9];
 };
-constexpr int kTabWords = 3 + 3 * kMaxIvl;
+constexpr int kTabWords = 3 + 3 * kMaxIvl + 9;
+static_assert(kTabWords <= kTabInts, "tile table stride");
 static_assert(sizeof(TileTab) == kTabWords * sizeof(int), "TileTab layout");
 
 // the block's threads copy a __shared__ TileTab to global memory
@@ -99,10 +102,10 @@ __device__ __forceinline__ void tab_row(const GridGeom &g, int cmin, int cmax,
 }
 
 // Merge the nine row ranges of a tile into its interval table (single
-// thread, no memory traffic).  rdelta[r] receives, for each row, LDS index -
-// storage index of that row's interval.
-__device__ __forceinline__ void tab_merge(const int *row_a, const int *row_b, int cap, TileTab &t,
-                                          int *__restrict__ rdelta) {
+// thread, no memory traffic).  t.rdelta[r] receives, for each row, LDS index
+// - storage index of that row's interval.
+__device__ __forceinline__ void tab_merge(const int *row_a, const int *row_b, int cap, TileTab &t) {
+    int *const rdelta = t.rdelta;
     int s0[kMaxIvl], s1[kMaxIvl], m = 0;
     for (int r = 0; r < 9; r++) {
         rdelta[r] = 0;

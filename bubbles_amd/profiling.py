@@ -29,7 +29,7 @@ from collections import defaultdict
 COPY_BYTES = float(2 ** 30)
 
 SHORT = {
-    "k_hash": "grid_hash", "k_scatter": "grid_scatter", "k_rank_move": "grid_rank_move",
+    "k_hash": "grid_hash", "k_tile_tables": "tile_tables", "k_scatter": "grid_scatter", "k_rank_move": "grid_rank_move",
     "k_scan_block": "grid_scan", "k_scan_add": "grid_scan_add",
     "k_buckets_density<false>": "buckets_density", "k_buckets_density<true>": "buckets_density_tait",
     "k_nonpressure<0>": "nonpressure", "k_nonpressure<1>": "nonpressure_predict",
