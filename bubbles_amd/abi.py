@@ -119,6 +119,7 @@ def load():
         "bbx_mlp_get_params": (i, [vp, fp]),
         "bbx_mlp_set_params": (i, [vp, fp]),
         "bbx_mlp_infer": (i, [vp, fp, i, fp]),
+        "bbx_mlp_set_expectile": (i, [vp, C.c_float]),
         "bbx_mlp_gradients": (i, [vp, fp, fp, i, fp, fp]),
         "bbx_mlp_train_step": (i, [vp, fp, fp, i, C.c_float, fp]),
         "bbx_mlp_train": (i, [vp, fp, fp, i, i, i, C.c_float, i, fp]),
@@ -403,6 +404,10 @@ class Mlp:
     def set_params(self, p):
         a = np.ascontiguousarray(p, dtype=np.float32)
         check(load().bbx_mlp_set_params(self.h, _fp(a)), "bbx_mlp_set_params")
+
+    def set_expectile(self, tau: float):
+        """Training loss: expectile tau (0.5 = mean squared error)."""
+        check(load().bbx_mlp_set_expectile(self.h, float(tau)), "bbx_mlp_set_expectile")
 
     def infer(self, x) -> np.ndarray:
         a = np.ascontiguousarray(x, dtype=np.float32)

@@ -381,7 +381,7 @@ static void free_all(bbx_solver *s) {
     dfree(s->cell_count); dfree(s->cell_start); dfree(s->scan_tmp);
     dfree(s->key); dfree(s->newc); dfree(s->rank_tmp); dfree(s->tmp_idx); dfree(s->hash_tmp);
     dfree(s->nbr16); dfree(s->nbr32); dfree(s->nbr_cnt); dfree(s->tile_tab);
-    dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->q);
+    dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->xq);
     dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
     dfree(s->loop); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
     if (s->h_loop) (void)hipHostFree(s->h_loop);
@@ -515,7 +515,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->rho, N);
     if (!rc) rc = dalloc(&s->pres, N);
     if (!rc) rc = dalloc(&s->rho_star, N);
-    if (!rc) rc = dalloc(&s->q, N);
+    if (!rc) rc = dalloc(&s->xq, N);
     if (!rc) rc = dalloc(&s->force, N);
     if (!rc) rc = dalloc(&s->fp, N);
     if (!rc) rc = dalloc(&s->xstar, N);
@@ -555,7 +555,7 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (e == hipSuccess) e = hipMemset(s->rho, 0, sizeof(float) * N);
     if (e == hipSuccess) e = hipMemset(s->pres, 0, sizeof(float) * N);
     if (e == hipSuccess) e = hipMemset(s->rho_star, 0, sizeof(float) * N);
-    if (e == hipSuccess) e = hipMemset(s->q, 0, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMemset(s->xq, 0, sizeof(float4) * N);
     if (e == hipSuccess) e = hipMemset(s->nbr_cnt, 0, sizeof(int) * N);
     if (e == hipSuccess) e = hipMemset(s->tile_tab, 0, sizeof(int) * (size_t)s->ntiles * kTabIntsHost);
     if (e == hipSuccess) e = hipMemset(s->feat, 0, sizeof(float) * N * BBX_NFEAT);

@@ -204,7 +204,8 @@ struct bbx_solver {
     int *tile_tab = nullptr;   // per-tile interval tables (kTabInts ints each)
     int ntiles = 0;
     // per-particle fields
-    float *rho = nullptr, *pres = nullptr, *rho_star = nullptr, *q = nullptr;
+    float *rho = nullptr, *pres = nullptr, *rho_star = nullptr;
+    float4 *xq = nullptr;        // position xyz and q = p / rho*^2 (the pressure-force gather input)
     float4 *force = nullptr, *fp = nullptr, *xstar = nullptr;
     float *feat = nullptr;       // [16][stride]
     float *pwarm = nullptr;      // MLP output

@@ -163,6 +163,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_grad(const float *__restric
                                                           const float *__restrict__ xs,
                                                           const float *__restrict__ ts, int n,
                                                           int batch, int random, unsigned long long seed,
+                                                          float expectile,
                                                           float *__restrict__ gpart,
                                                           float *__restrict__ lpart) {
     __shared__ unsigned short lds[kWaves][3][kH * 16];  // dz2, h1, dz1 tiles [hidden][sample]
@@ -201,9 +202,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_grad(const float *__restric
         forward_tile(w, x4, a, lane);
         const float tj = valid ? ts[idx] * inv_os : 0.f;
         const float diff = valid ? (a.y - tj) : 0.f;
-        const float dy = diff * inv_b;
+        // expectile weight: 2 (1 - tau) above the target, 2 tau below (1 at tau = 0.5)
+        const float wgt = diff > 0.f ? 2.f * (1.f - expectile) : 2.f * expectile;
+        const float dy = wgt * diff * inv_b;
         if (lane < 16) {
-            loss += 0.5f * diff * diff * inv_b;
+            loss += 0.5f * wgt * diff * diff * inv_b;
             gb3 += dy;
         }
         // layer 3 and layer-2 activations
@@ -328,6 +331,7 @@ struct bbx_mlp {
     float *tmp_x = nullptr, *tmp_y = nullptr;
     int tmp_n = 0;
     int adam_t = 0;
+    float expectile = 0.5f;    // training loss: 0.5 = mean squared error
     hipStream_t stream = nullptr;
     int refs = 0;          // solvers this network is attached to
     bool dead = false;     // destroyed while attached: freed at the last release
@@ -381,6 +385,7 @@ static int grad_launch(bbx_mlp *m, const float *xs, const float *ts, int n, int 
     const int nwaves = blocks * kWaves;
     BBX_CHECK(ensure_gpart(m, nwaves));
     k_mlp_grad<<<blocks, 64 * kWaves, 0, m->stream>>>(m->params, xs, ts, n, batch, random, seed,
+                                                       m->expectile,
                                                        m->gpart, m->lpart);
     BBX_HIP_CHECK(hipGetLastError());
     float bc1 = 1.f, bc2 = 1.f;
@@ -404,6 +409,13 @@ extern "C" {
 int bbx_mlp_param_count(int *count) {
     if (!count) return fail(BBX_E_INVALID, "null count");
     *count = kParams;
+    return BBX_OK;
+}
+
+int bbx_mlp_set_expectile(bbx_mlp *m, float tau) {
+    if (!m) return fail(BBX_E_INVALID, "null network");
+    if (!(tau > 0.f && tau < 1.f)) return fail(BBX_E_INVALID, "expectile must lie in (0, 1)");
+    m->expectile = tau;
     return BBX_OK;
 }
 

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
                                                             float *__restrict__ feat,
                                                             const float *__restrict__ P,
                                                             float *__restrict__ pres,
-                                                            float *__restrict__ q,
+                                                            float4 *__restrict__ xq,
                                                             float *__restrict__ pwarm) {
     constexpr bool kFeat = kMode >= 2;
     // neighbour (position) and (velocity, density) as float4 arrays; in mode
@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
             const float d2 = d * d;
             pwarm[i] = p;
             pres[i] = p;
-            q[i] = (fabsf(d2) < 1e-8f) ? 0.f : p / d2;
+            xq[i] = make_float4(xi.x, xi.y, xi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
         }
     }
 }
@@ -325,9 +325,10 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
 __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepConst st, double delta,
                                                              NbrTab nb,
                                                              const float4 *__restrict__ xstar,
+                                                             const float4 *__restrict__ pos,
                                                              float *__restrict__ pres,
                                                              float *__restrict__ rho_star,
-                                                             float *__restrict__ q,
+                                                             float4 *__restrict__ xq,
                                                              LoopState *__restrict__ loop, int k) {
     // predicted positions, copied by 16-byte LDS DMA (the w lane is the
     // unused fourth word of x*)
@@ -345,11 +346,14 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     // staging: request them first so their latency overlaps it
     NbrQuads<kNbrDepth> first{};
     int cnt = 0;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), pi = xi;
+    float pold = 0.f;
     if (valid) {
         first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
         cnt = nb.cnt[i];
         xi = xstar[i];
+        pi = pos[i];
+        if (!st.first) pold = pres[i];
     }
     if (staged) {
         tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
@@ -383,11 +387,11 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
             dp *= (double)c.nps;
             err *= (double)c.nps;
         }
-        float p = (st.first ? 0.f : pres[i]) + (float)dp;
+        float p = pold + (float)dp;
         pres[i] = p;
         rho_star[i] = density;
         float d2 = density * density;
-        q[i] = (fabsf(d2) < 1e-8f) ? 0.f : p / d2;
+        xq[i] = make_float4(pi.x, pi.y, pi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
         // |err| with a cleared sign bit: -0.0 must not win the unsigned max
         // (the reference's AbsfMax keeps the first of equal squares)
         const double ae = fabs(err);
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            const float4 *__restrict__ pos,
                                                            const float4 *__restrict__ vel,
                                                            const float4 *__restrict__ force,
-                                                           const float *__restrict__ q,
+                                                           const float4 *__restrict__ xq,
                                                            const float *__restrict__ pres,
                                                            float4 *__restrict__ xstar,
                                                            float4 *__restrict__ fp_out,
@@ -431,8 +435,8 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            unsigned *__restrict__ max_force,
                                                            LoopState *__restrict__ loop, int k,
                                                            int max_it, int exact, double tol) {
-    // position x, y, z and q = p / rho*^2 as four arrays
-    __shared__ float sxx[kTileCap], sxy[kTileCap], sxz[kTileCap], sq[kTileCap];
+    // (position, q = p / rho*^2) of the neighbours, copied by 16-byte LDS DMA
+    __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
     if (k > 0 && loop->done[k - 1]) {
         // an earlier iteration integrated: pass the flag on and exit (uniform)
@@ -471,20 +475,14 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
         cnt = nb.cnt[i];
         xi = pos[i];
-        qi = q[i];
+        qi = xq[i].w;
         fi = force[i];
         vi = vel[i];
     }
-    if (staged)
-        tab_stage_with(tt, [&](int j) {
-            const float4 p = pos[j];
-            return make_float4(p.x, p.y, p.z, q[j]);
-        }, [&](int e, const float4 &v) {
-            sxx[e] = v.x;
-            sxy[e] = v.y;
-            sxz[e] = v.z;
-            sq[e] = v.w;
-        });
+    if (staged) {
+        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xq + j, sx + e); });
+        lds_dma_wait();
+    }
     __syncthreads();
     unsigned fbits = 0u;
     if (valid) {
@@ -497,20 +495,9 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
             for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
-                float xjx, xjy, xjz, qj;
-                if (kS) {
-                    xjx = sxx[j];
-                    xjy = sxy[j];
-                    xjz = sxz[j];
-                    qj = sq[j];
-                } else {
-                    const float4 p = pos[j];
-                    xjx = p.x;
-                    xjy = p.y;
-                    xjz = p.z;
-                    qj = q[j];
-                }
-                const float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
+                const float4 a = kS ? sx[j] : xq[j];  // (x_j, q_j)
+                const float qj = a.w;
+                const float dx = a.x - xi.x, dy = a.y - xi.y, dz = a.z - xi.z;
                 const float d2 = dx * dx + dy * dy + dz * dz;
                 const float ir = fast_rsq(d2);
                 const float r = d2 * ir;
@@ -717,7 +704,7 @@ int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int feature
         BBX_PROF_BEGIN("nonpressure_features_warm");
         k_nonpressure<3><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
                                                        s->xstar, s->rho_star, s->feat,
-                                                       mlp_params_device(s->mlp), s->pres, s->q,
+                                                       mlp_params_device(s->mlp), s->pres, s->xq,
                                                        s->pwarm);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
@@ -752,7 +739,8 @@ int run_predict_pressure(bbx_solver *s, const StepConst &st, int k) {
     // the pressure increment uses the host's FP64 delta (s->delta64)
     BBX_PROF_BEGIN_IT("predict_pressure", k);
     k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
-                                                                s->pres, s->rho_star, s->q, s->loop, k);
+                                                                s->posA, s->pres, s->rho_star, s->xq,
+                                                                s->loop, k);
     BBX_HIP_CHECK(hipGetLastError());
     BBX_PROF_END();
     return BBX_OK;
@@ -763,7 +751,7 @@ int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, in
     if (k >= kMaxIt || max_it > kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
     BBX_PROF_BEGIN_IT("pressure_force", k < 0 ? -1 : k);
     k_pressure_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(
-        s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->q, s->pres, s->xstar, s->fp, s->posB,
+        s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->xq, s->pres, s->xstar, s->fp, s->posB,
         s->velB, s->force, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol);
     BBX_HIP_CHECK(hipGetLastError());
     BBX_PROF_END();

@@ -85,13 +85,16 @@ OFF_OSCALE = OFF_ISTD + 16
 OFF_ALPHA, OFF_PCAP, OFF_RESID = OFF_OSCALE + 1, OFF_OSCALE + 2, OFF_OSCALE + 3
 
 
-def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0, alpha=0.7):
+def fit(x, t, steps=20000, batch=4096, lr=1e-3, seed=0, alpha=0.7, expectile=0.5):
     """Train the network on the residual t - p_prev (the previous substep's
     converged pressure is the strongest single predictor); the solver then
     starts from alpha * clamp(p_prev + net, 0, pcap).  Under-estimating is
     safe -- the unchanged loop adds pressure -- while an over-estimate can
-    never be removed (negative increments are scaled by 0), hence alpha < 1."""
+    never be removed (negative increments are scaled by 0), hence alpha < 1
+    and/or an expectile below 0.5 (a loss that weighs over-estimates more)."""
     net = abi.Mlp(seed)
+    if expectile != 0.5:
+        net.set_expectile(expectile)
     p = net.get_params()
     mean = x.mean(axis=0)
     std = x.std(axis=0)
@@ -216,6 +219,8 @@ def main(argv=None):
     tr.add_argument("--harvest-max-it", type=int, default=32,
                     help="iteration cap of the plain runs that produce the targets")
     tr.add_argument("--alpha", type=float, default=0.7)
+    tr.add_argument("--expectile", type=float, default=0.5,
+                    help="training loss expectile (0.5 = mean squared error)")
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--steps", type=int, default=480)
@@ -252,12 +257,13 @@ def main(argv=None):
         t1 = time.perf_counter()
         print(f"harvested {x.shape[0]} samples in {t1 - t0:.1f} s; target mean {t.mean():.4f} std {t.std():.4f}")
         net, loss = fit(x, t, steps=200 if args.quick else args.steps, batch=args.batch, lr=args.lr,
-                        alpha=args.alpha)
+                        alpha=args.alpha, expectile=args.expectile)
         t2 = time.perf_counter()
         print(f"trained {args.steps} Adam steps in {t2 - t1:.1f} s, final batch loss {loss:.5f}")
         save(net, args.out, {"samples": int(x.shape[0]), "steps": args.steps, "batch": args.batch,
                              "lr": args.lr, "seeds": seeds, "final_loss": loss,
-                             "harvest_max_it": args.harvest_max_it, "alpha": args.alpha})
+                             "harvest_max_it": args.harvest_max_it, "alpha": args.alpha,
+                             "expectile": args.expectile})
         print("saved", args.out)
     else:
         net = load(args.weights)

@@ -208,7 +208,12 @@ int bbx_mlp_get_params(bbx_mlp *m, float *out);
 int bbx_mlp_set_params(bbx_mlp *m, const float *in);
 /* y[n] = MLP(x[n*16]) on the device (host arrays). */
 int bbx_mlp_infer(bbx_mlp *m, const float *x, int n, float *y);
-/* gradients of 0.5*mean((y - t)^2) w.r.t. the FP32 params (host arrays) */
+/* Training loss: the expectile loss 0.5*mean(w (y - t)^2) with w = 2(1 - tau)
+ * where y > t and w = 2 tau where y < t (tau = 0.5, the default, is the mean
+ * squared error).  tau < 0.5 makes over-estimates costlier: the network then
+ * learns a low expectile of the target.  tau must lie in (0, 1). */
+int bbx_mlp_set_expectile(bbx_mlp *m, float tau);
+/* gradients of the training loss w.r.t. the FP32 params (host arrays) */
 int bbx_mlp_gradients(bbx_mlp *m, const float *x, const float *t, int n, float *grad,
                       float *loss);
 /* one Adam step on a batch (host arrays); returns the batch loss */

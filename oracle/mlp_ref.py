@@ -6,7 +6,8 @@ Parameter layout (FP32 vector, see include/bubbles_amd.h):
   W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1] | mean[16] inv_std[16] out_scale[1]
 Forward:  xn = (x - mean) * inv_std;  z1 = W1 xn + b1;  h1 = gelu(z1)
           z2 = W2 h1 + b2;  h2 = gelu(z2);  y = W3 . h2 + b3
-Loss:     0.5 / B * sum (y - t / out_scale)^2
+Loss:     0.5 / B * sum w (y - t / out_scale)^2, w = 2(1 - tau) above the
+          target and 2 tau below (expectile tau; 0.5 = mean squared error)
 With ``emulate=True`` the values the device feeds to BF16 MFMA operands are
 rounded to bfloat16 (round to nearest even) at the same points: xn, W1, h1,
 W2 in the forward pass; dz2 (for dH1 and dW2), h1, dz1 and xn in the backward
@@ -100,15 +101,18 @@ def warm_pressure(params, x, emulate=True):
     return p[OFF_ALPHA] * np.clip(predict(params, x, emulate), 0.0, p[OFF_PCAP])
 
 
-def loss_and_grad(params, x, t, emulate=True):
+def loss_and_grad(params, x, t, emulate=True, expectile=0.5):
+    """Expectile loss 0.5 mean(w (y - t)^2), w = 2(1 - tau) where y > t and
+    2 tau where y < t (tau = 0.5: mean squared error), and its gradient."""
     r = bf16 if emulate else (lambda a: np.asarray(a, dtype=np.float64))
     y, c = forward(params, x, emulate)
     P = c["P"]
     B = y.shape[0]
     tn = np.asarray(t, dtype=np.float64) / P["oscale"]
     diff = y - tn
-    loss = 0.5 * np.sum(diff * diff) / B
-    dy = diff / B                                       # (n,)
+    w = np.where(diff > 0, 2.0 * (1.0 - expectile), 2.0 * expectile)
+    loss = 0.5 * np.sum(w * diff * diff) / B
+    dy = w * diff / B                                   # (n,)
     g = np.zeros(N_PARAMS)
     g[OFF_W3:OFF_B3] = c["h2"].T @ dy
     g[OFF_B3] = dy.sum()

@@ -55,16 +55,18 @@ def test_infer_matches_oracle(bbx):
     assert np.abs(y - ref64).max() <= 5e-2 * np.abs(ref64).max()
 
 
-@pytest.mark.parametrize("n", [16, 1000, 5000])
-def test_gradients_match_oracle(bbx, n):
+@pytest.mark.parametrize("n,tau", [(16, 0.5), (1000, 0.5), (5000, 0.5), (1000, 0.2), (5000, 0.85)])
+def test_gradients_match_oracle(bbx, n, tau):
     rng = np.random.default_rng(n)
     p = random_params(1)
     net = bbx.Mlp(1)
     net.set_params(p)
+    if tau != 0.5:
+        net.set_expectile(tau)
     x = rng.normal(size=(n, M.F)).astype(np.float32)
     t = rng.normal(size=n).astype(np.float32)
     g, loss = net.gradients(x, t)
-    lref, gref = M.loss_and_grad(p, x, t, emulate=True)
+    lref, gref = M.loss_and_grad(p, x, t, emulate=True, expectile=tau)
     assert loss == pytest.approx(lref, rel=1e-4)
     for name, (a, b) in TENSORS.items():
         scale = np.abs(gref[a:b]).max()

@@ -57,6 +57,59 @@ def test_backward_matches_finite_differences():
     assert np.all(g[M.N_TRAIN:] == 0)
 
 
+@pytest.mark.parametrize("tau", [0.2, 0.8])
+def test_expectile_backward_matches_finite_differences(tau):
+    rng = np.random.default_rng(2)
+    p = random_params(2)
+    x = rng.normal(size=(24, M.F))
+    t = rng.normal(size=24)
+    _, g = M.loss_and_grad(p, x, t, emulate=False, expectile=tau)
+    e = 1e-6
+    for i in rng.choice(M.N_TRAIN, 40, replace=False).tolist() + [M.OFF_B3]:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += e
+        pm[i] -= e
+        fd = (M.loss_and_grad(pp, x, t, False, tau)[0] - M.loss_and_grad(pm, x, t, False, tau)[0]) / (2 * e)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def test_expectile_half_is_mean_squared_error():
+    rng = np.random.default_rng(3)
+    p = random_params(3)
+    x = rng.normal(size=(30, M.F))
+    t = rng.normal(size=30)
+    la, ga = M.loss_and_grad(p, x, t, emulate=False)
+    lb, gb = M.loss_and_grad(p, x, t, emulate=False, expectile=0.5)
+    y, _ = M.forward(p, x, emulate=False)
+    assert la == lb == pytest.approx(0.5 * np.mean((y - t / p[M.OFF_OSCALE]) ** 2), rel=1e-12)
+    assert np.array_equal(ga, gb)
+
+
+@pytest.mark.parametrize("tau", [0.15, 0.5, 0.9])
+def test_expectile_gradient_vanishes_at_the_sample_expectile(tau):
+    """A network whose output is the constant b3 (all weights 0) has zero
+    b3-gradient exactly at the tau-expectile of the targets: the c with
+    tau * sum_{t > c} (t - c) = (1 - tau) * sum_{t < c} (c - t), found by
+    bisection.  A swapped weight would put the zero at the 1 - tau expectile."""
+    rng = np.random.default_rng(4)
+    t = rng.gamma(2.0, size=400)
+    lo, hi = t.min(), t.max()
+    for _ in range(200):
+        c = 0.5 * (lo + hi)
+        f = tau * np.sum(np.clip(t - c, 0, None)) - (1 - tau) * np.sum(np.clip(c - t, 0, None))
+        lo, hi = (c, hi) if f > 0 else (lo, c)
+    p = np.zeros(M.N_PARAMS)
+    p[M.OFF_ISTD:M.OFF_OSCALE] = 1.0
+    p[M.OFF_OSCALE] = 1.0
+    p[M.OFF_B3] = c
+    x = rng.normal(size=(t.size, M.F))
+    _, g = M.loss_and_grad(p, x, t, emulate=False, expectile=tau)
+    assert abs(g[M.OFF_B3]) < 1e-9
+    p[M.OFF_B3] = c + 0.01
+    _, g = M.loss_and_grad(p, x, t, emulate=False, expectile=tau)
+    assert g[M.OFF_B3] > 0
+
+
 def test_bf16_rounding_is_nearest_even():
     a = np.array([1.0, 1.00390625, 1.005859375, 3.0e38, -2.5], dtype=np.float32)
     r = M.bf16(a)
