@@ -3,6 +3,7 @@
   python -m bubbles_amd.profiling stats KERNEL_STATS_CSV --command CMD --out OUT.md
   python -m bubbles_amd.profiling traffic FETCH_CSV WRITE_CSV --n N --command CMD \
       --json OUT.json --out OUT.md
+  python -m bubbles_amd.profiling sq SQ_CSV --command CMD --out OUT.md
 
 `stats` turns a `rocprofv3 --kernel-trace --stats -f csv` summary into a
 markdown table. `traffic` turns two `rocprofv3 --pmc FETCH_SIZE` /
@@ -124,6 +125,56 @@ def traffic_markdown(rec: dict, command: str, algorithmic: dict | None = None) -
     return "\n".join(lines) + "\n"
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM",
+               "SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_BUSY_CYCLES")
+
+
+def sq_summary(path: str, min_waves: int = 1000):
+    """Per kernel: mean duration and the SQ counters per wave, over the
+    dispatches with at least `min_waves` waves (launches that exit at entry
+    are left out)."""
+    per = defaultdict(dict)
+    for r in csv.DictReader(open(path)):
+        key = (short_name(r["Kernel_Name"]), int(r["Dispatch_Id"]))
+        per[key][r["Counter_Name"]] = float(r["Counter_Value"])
+        per[key]["_us"] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    agg, cnt = defaultdict(lambda: defaultdict(float)), defaultdict(int)
+    for (name, _), c in per.items():
+        if c.get("SQ_WAVES", 0) < min_waves:
+            continue
+        cnt[name] += 1
+        for k, v in c.items():
+            agg[name][k] += v
+    out = {}
+    for name, c in agg.items():
+        w = c["SQ_WAVES"]
+        row = {"dispatches": cnt[name], "us": c["_us"] / cnt[name], "waves": w / cnt[name]}
+        for k in ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM"):
+            if k in c:
+                row[k] = c[k] / w
+        if "SQ_WAIT_ANY" in c and c.get("SQ_WAVE_CYCLES"):
+            row["wait_frac"] = c["SQ_WAIT_ANY"] / c["SQ_WAVE_CYCLES"]
+        out[name] = row
+    return out
+
+
+def sq_markdown(rows: dict, command: str, simds: int = 1024, clock_ghz: float = 2.4) -> str:
+    lines = ["# SQ counters per kernel (instructions per wave)", "", "Command:", "", "```", command,
+             "```", "",
+             "`VALU share` = VALU instructions x 4 cycles (a wave64 instruction on a 16-lane SIMD) "
+             f"x waves / ({simds} SIMDs x {clock_ghz} GHz), divided by the kernel's duration under "
+             "the profiler: how much of the kernel the vector ALU alone would need.", "",
+             "| kernel | dispatches | us | waves | VALU/wave | LDS/wave | SALU/wave | VMEM/wave | wait % | VALU share |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    for name, r in sorted(rows.items(), key=lambda kv: -kv[1]["us"] * kv[1]["dispatches"]):
+        valu = r.get("SQ_INSTS_VALU", 0.0)
+        share = valu * 4 * r["waves"] / (simds * clock_ghz * 1e3) / r["us"] if r["us"] else 0.0
+        lines.append(f"| {name} | {r['dispatches']} | {r['us']:.1f} | {r['waves']:.0f} | {valu:.0f} | "
+                     f"{r.get('SQ_INSTS_LDS', 0):.0f} | {r.get('SQ_INSTS_SALU', 0):.0f} | "
+                     f"{r.get('SQ_INSTS_VMEM', 0):.0f} | {100 * r.get('wait_frac', 0):.0f} | {share:.2f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -138,8 +189,15 @@ def main(argv=None):
     tr.add_argument("--command", required=True)
     tr.add_argument("--json", required=True)
     tr.add_argument("--out", required=True)
+    sq = sub.add_parser("sq")
+    sq.add_argument("csv")
+    sq.add_argument("--command", required=True)
+    sq.add_argument("--out", required=True)
     args = ap.parse_args(argv)
-    if args.cmd == "stats":
+    if args.cmd == "sq":
+        with open(args.out, "w") as f:
+            f.write(sq_markdown(sq_summary(args.csv), args.command))
+    elif args.cmd == "stats":
         with open(args.out, "w") as f:
             f.write(stats_markdown(args.csv, args.command))
     else:
