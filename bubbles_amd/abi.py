@@ -131,6 +131,10 @@ def load():
         "bbx_profile_get": (i, [vp, C.c_char_p, i, dp, ip, i, ip]),
         "bbx_solver_synchronize": (i, [vp]),
         "bbx_stream_copy": (i, [d, i, dp, dp]),
+        "bbx_serializer_write": (i, [C.c_char_p, i, d, i, dp, dp, dp, d, ip]),
+        "bbx_solver_save": (i, [vp, C.c_char_p, i]),
+        "bbx_serializer_read": (i, [C.c_char_p, i, ip, ip, dp, dp, dp, dp, ip]),
+        "bbx_serializer_parse_number": (i, [C.c_char_p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -161,6 +165,48 @@ def stream_copy(nbytes: float, iters: int = 20):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# serializer flags (SERIALIZER_* of the reference's serializer.h)
+SER_POSITION, SER_VELOCITY, SER_DENSITY, SER_BOUNDARY = 0x01, 0x02, 0x04, 0x08
+SER_NORMAL, SER_MASS, SER_LAYERS, SER_XYZ, SER_BOUNDARY_EXCLUSIVE = 0x10, 0x20, 0x40, 0x80, 0x100
+
+
+def serializer_write(path, flags, spacing, positions=None, velocities=None, densities=None,
+                     mass=0.0, boundary=None):
+    """Append one FluidBegin block (the reference's SaveSphParticleSet format)."""
+    arrs = [positions, velocities, densities]
+    n = next((len(a) for a in arrs + [boundary] if a is not None), 0)
+    pos = np.ascontiguousarray(positions if positions is not None else np.zeros((n, 3)), dtype=np.float64)
+    vel = np.ascontiguousarray(velocities if velocities is not None else np.zeros((n, 3)), dtype=np.float64)
+    den = np.ascontiguousarray(densities if densities is not None else np.zeros(n), dtype=np.float64)
+    bnd = None if boundary is None else np.ascontiguousarray(boundary, dtype=np.int32)
+    check(load().bbx_serializer_write(os.fsencode(path), flags, float(spacing), n, _dp(pos), _dp(vel),
+                                      _dp(den), float(mass), _ip(bnd) if bnd is not None else None),
+          "bbx_serializer_write")
+
+
+def serializer_read(path, default_flags=SER_POSITION):
+    """First FluidBegin block: dict of flags and per-particle arrays."""
+    n, fl = C.c_int(), C.c_int(default_flags)
+    check(load().bbx_serializer_read(os.fsencode(path), -1, C.byref(n), C.byref(fl), None, None, None, None,
+                                     None), "bbx_serializer_read")
+    cap = n.value
+    pos, vel = np.zeros((cap, 3)), np.zeros((cap, 3))
+    den, mas = np.zeros(cap), np.zeros(cap)
+    bnd = np.zeros(cap, dtype=np.int32)
+    fl = C.c_int(default_flags)
+    check(load().bbx_serializer_read(os.fsencode(path), cap, C.byref(n), C.byref(fl), _dp(pos), _dp(vel),
+                                     _dp(den), _dp(mas), _ip(bnd)), "bbx_serializer_read")
+    return {"flags": fl.value, "position": pos, "velocity": vel, "density": den, "mass": mas,
+            "boundary": bnd}
+
+
+def serializer_parse_number(text: str) -> float:
+    """The reference's ParseFloat of one token."""
+    v = C.c_double()
+    check(load().bbx_serializer_parse_number(text.encode(), C.byref(v)), "bbx_serializer_parse_number")
+    return v.value
 
 
 def _ip(a):
@@ -227,6 +273,10 @@ class Solver:
         check(load().bbx_solver_info(self.h, C.byref(inf)), "bbx_solver_info")
         return {"count": inf.count, "mass": inf.mass, "kernel_radius": inf.kernel_radius,
                 "radius": inf.radius, "delta_denom": inf.delta_denom, "grid": inf.grid}
+
+    def save(self, path, flags=SER_POSITION | SER_VELOCITY | SER_DENSITY):
+        """Append this solver's particles as one serializer block."""
+        check(load().bbx_solver_save(self.h, os.fsencode(path), flags), "bbx_solver_save")
 
     def get(self, field: str) -> np.ndarray:
         f = FIELD[field]

@@ -668,6 +668,17 @@ int bbx_solver_download(bbx_solver *s, int field, double *out) {
     }
 }
 
+int bbx_solver_save(bbx_solver *s, const char *path, int flags) {
+    BBX_CHECK(check_solver(s));
+    const size_t n = (size_t)s->n;
+    std::vector<double> pos(3 * n), vel(3 * n), dens(n);
+    if (flags & BBX_SER_POSITION) BBX_CHECK(bbx_solver_download(s, BBX_FIELD_POSITION, pos.data()));
+    if (flags & BBX_SER_VELOCITY) BBX_CHECK(bbx_solver_download(s, BBX_FIELD_VELOCITY, vel.data()));
+    if (flags & BBX_SER_DENSITY) BBX_CHECK(bbx_solver_download(s, BBX_FIELD_DENSITY, dens.data()));
+    return bbx_serializer_write(path, flags, s->prm.target_spacing, s->n, pos.data(), vel.data(),
+                                dens.data(), s->mass, nullptr);
+}
+
 int bbx_solver_upload(bbx_solver *s, int field, const double *in) {
     BBX_CHECK(check_solver(s));
     if (!in) return fail(BBX_E_INVALID, "null input");

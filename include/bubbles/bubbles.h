@@ -530,4 +530,49 @@ class SphSolver3 {
     bbx_solver *dev_ = nullptr;
 };
 
+/* ---- serializer (src/third/serializer.h, src/third/serializer.cpp) ---- */
+#ifndef SERIALIZER_POSITION
+#define SERIALIZER_POSITION BBX_SER_POSITION
+#define SERIALIZER_VELOCITY BBX_SER_VELOCITY
+#define SERIALIZER_DENSITY BBX_SER_DENSITY
+#define SERIALIZER_BOUNDARY BBX_SER_BOUNDARY
+#define SERIALIZER_NORMAL BBX_SER_NORMAL
+#define SERIALIZER_MASS BBX_SER_MASS
+#define SERIALIZER_LAYERS BBX_SER_LAYERS
+#define SERIALIZER_XYZ BBX_SER_XYZ
+#define SERIALIZER_RULE_BOUNDARY_EXCLUSIVE BBX_SER_BOUNDARY_EXCLUSIVE
+#endif
+
+/* SaveSphParticleSet (src/third/serializer.cpp:875-911): appends the
+ * particles of the solver that owns data's particle set (device data in id
+ * order) to `filename` as one FluidBegin ... FluidEnd block. */
+inline void SaveSphParticleSet(SphSolverData3 *data, const char *filename, int flags) {
+    if (!data || !data->sphpSet || !data->sphpSet->GetParticleSet() || !data->sphpSet->GetParticleSet()->dev)
+        throw bubbles_error("SaveSphParticleSet: the particle set is not in a solver");
+    bbx_throw_if(bbx_solver_save(data->sphpSet->GetParticleSet()->dev, filename, flags), "SaveSphParticleSet");
+}
+
+/* SerializerLoadSphDataSet3 (src/third/serializer.cpp:562-577): adds the
+ * particles of the file's first fluid block to the builder; flags takes the
+ * block's "Format".  Returns the particle count. */
+inline int SerializerLoadSphDataSet3(ParticleSetBuilder3 *builder, const char *filename, int &flags,
+                                     std::vector<int> *boundary = nullptr) {
+    if (!builder) throw bubbles_error("SerializerLoadSphDataSet3: null builder");
+    int n = 0, fl = flags;
+    bbx_throw_if(bbx_serializer_read(filename, -1, &n, &fl, nullptr, nullptr, nullptr, nullptr, nullptr),
+                 "SerializerLoadSphDataSet3");
+    std::vector<double> p(3 * (size_t)n), v(3 * (size_t)n);
+    std::vector<int> b((size_t)n);
+    fl = flags;
+    bbx_throw_if(bbx_serializer_read(filename, n, &n, &fl, p.data(), v.data(), nullptr, nullptr, b.data()),
+                 "SerializerLoadSphDataSet3");
+    flags = fl;
+    for (int i = 0; i < n; i++) {
+        builder->AddParticle(vec3f(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]),
+                             vec3f(v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]));
+        if (boundary) boundary->push_back(b[(size_t)i]);
+    }
+    return n;
+}
+
 #endif

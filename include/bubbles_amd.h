@@ -231,6 +231,41 @@ int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m);
 int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,
                        float *targets, bbx_stats *stats);
 
+/* ---- frame I/O in the reference's serializer format ----
+ * Flag bits as SERIALIZER_* (src/third/serializer.h:7-17). */
+#define BBX_SER_POSITION 0x01
+#define BBX_SER_VELOCITY 0x02
+#define BBX_SER_DENSITY 0x04
+#define BBX_SER_BOUNDARY 0x08
+#define BBX_SER_NORMAL 0x10
+#define BBX_SER_MASS 0x20
+#define BBX_SER_LAYERS 0x40
+#define BBX_SER_XYZ 0x80
+#define BBX_SER_BOUNDARY_EXCLUSIVE 0x100
+/* SaveSphParticleSet (src/third/serializer.cpp:875-911): appends one
+ * FluidBegin ... FluidEnd block with the requested fields of n particles
+ * (host arrays: positions/velocities 3n doubles, densities n, one mass for
+ * all, boundary n ints or NULL), numbers printed with %g.  Supported flags:
+ * position, velocity, density, mass, boundary, boundary-exclusive (normals,
+ * layers and the legacy xyz form are rejected). */
+int bbx_serializer_write(const char *path, int flags, double spacing, int n, const double *positions,
+                         const double *velocities, const double *densities, double mass,
+                         const int *boundary);
+/* The same for a solver's particles in id order: position, velocity and
+ * density downloaded from the device (FP32 values widened to double), the
+ * solver's mass and target spacing. */
+int bbx_solver_save(bbx_solver *s, const char *path, int flags);
+/* SerializerLoadParticles3 (src/third/serializer.cpp:543-560) over
+ * SerializerFindFluidSection + _SerializerLoadParticles3 (:130-190,444-540):
+ * reads the first FluidBegin block.  *flags is the caller's default and is
+ * replaced by the block's "Format"; up to `capacity` particles are stored
+ * in the non-NULL arrays (capacity < 0: count only).  Numbers are parsed as
+ * the reference's ParseFloat does (src/third/obj_loader.cpp:55-170,270-277). */
+int bbx_serializer_read(const char *path, int capacity, int *count, int *flags, double *positions,
+                        double *velocities, double *densities, double *masses, int *boundary);
+/* ParseFloat of one token (0 when it is not a number). */
+int bbx_serializer_parse_number(const char *text, double *value);
+
 /* ---- profiling ----
  * Average device time (ms) and launch count of every kernel of this solver
  * since the last reset, measured with HIP events on the solver's stream. */

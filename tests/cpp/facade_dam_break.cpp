@@ -43,6 +43,18 @@ int main(int argc, char **argv) {
             sy += pSet->GetParticlePosition(i).y;
             sd += pSet->GetParticleDensity(i);
         }
+        // frame dump in the reference's serializer format, read back
+        // through the loader into a builder
+        int reread = -1;
+        if (argc > 2) {
+            std::remove(argv[2]);
+            SaveSphParticleSet(solver.GetSphSolverData(), argv[2],
+                               SERIALIZER_POSITION | SERIALIZER_VELOCITY | SERIALIZER_DENSITY);
+            ParticleSetBuilder3 back;
+            int flags = 0;
+            reread = SerializerLoadSphDataSet3(&back, argv[2], flags);
+            if (flags != (SERIALIZER_POSITION | SERIALIZER_VELOCITY | SERIALIZER_DENSITY)) reread = -2;
+        }
         Bucket *bk = pSet->GetParticleBucket(0);
         vec3ui dims = grid->GetIndexCount();
         std::printf("{\"n\": %d, \"substeps\": %d, \"mass\": %.17g, \"delta_denom\": %.17g, "
@@ -51,7 +63,8 @@ int main(int argc, char **argv) {
                     pSet->GetParticleCount(), substeps, pSet->GetMass(), solver.ComputeDeltaDenom(),
                     dims.x, dims.y, dims.z, sy, sd);
         for (int k = 0; k < bk->Count(); k++) std::printf("%s%d", k ? ", " : "", bk->Get(k));
-        std::printf("], \"hash0\": %u}\n", grid->GetLinearHashedPosition(pSet->GetParticlePosition(0)));
+        std::printf("], \"hash0\": %u, \"reread\": %d}\n",
+                    grid->GetLinearHashedPosition(pSet->GetParticlePosition(0)), reread);
     } catch (const bubbles_error &e) {
         std::fprintf(stderr, "bubbles error: %s\n", e.what());
         return 3;

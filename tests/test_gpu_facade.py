@@ -23,7 +23,8 @@ def test_facade_example_is_built():
 
 @pytest.mark.gpu
 def test_facade_matches_abi_and_oracle(bbx, oracle):
-    res = subprocess.run([hipbuild.FACADE_BIN, "3"], capture_output=True, text=True, timeout=120)
+    frame = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"bbx_facade_frame_{os.getpid()}.txt")
+    res = subprocess.run([hipbuild.FACADE_BIN, "3", frame], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     out = json.loads(res.stdout.strip().splitlines()[-1])
     sc = facade_scene()
@@ -47,3 +48,11 @@ def test_facade_matches_abi_and_oracle(bbx, oracle):
     oi = o.info()
     assert out["mass"] == oi["mass"] and out["delta_denom"] == oi["delta_denom"]
     assert out["hash0"] == oracle.grid_hash(oi["grid"], pos[0])[0]
+    # the frame SaveSphParticleSet wrote, read back through the library's loader
+    assert out["reread"] == sc.n
+    r = bbx.serializer_read(frame)
+    os.remove(frame)
+    assert r["flags"] == bbx.SER_POSITION | bbx.SER_VELOCITY | bbx.SER_DENSITY
+    printed = np.array([[float("%g" % v) for v in row] for row in pos])
+    np.testing.assert_allclose(r["position"], printed, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(r["density"], [float("%g" % v) for v in g.get("density")], rtol=1e-15)

@@ -1,0 +1,124 @@
+"""Long-horizon precision study on the benchmark geometry (dam break at
+spacing 0.01, h 0.02, 2x1x2 tank): FP32 + Kahan on the GPU against the FP64
+oracle from the same FP32 start.
+
+Opt-in (several minutes of oracle time on 16 host threads): set
+BBX_LONG_PRECISION=1, and BBX_PRECISION_OUT=<file> to write the tables as
+JSON.  Cases:
+
+* plain REFERENCE_EXACT (one iteration per substep, so both trajectories
+  differ by arithmetic alone): 262,144 particles over 1000 substeps and
+  4,194,304 (256x64x256) over 20;
+* learned warm start, REFERENCE_EXACT, 262,144 over 200 substeps: every
+  substep the oracle starts from the GPU's own warm pressures (the network is
+  not part of the oracle), so the comparison isolates the FP32 solver;
+* TEXTBOOK iteration drift, 262,144 over 200 substeps: how often the FP32 and
+  FP64 runs stop after a different number of corrections (each run on its own
+  trajectory).
+
+Measured on MI355X (max relative density / pressure force / position in h):
+1000 substeps at 262k stay below 2e-2 / 2e-2 / 2e-2 with means near 1e-4;
+bounds below are about 3x the measured maxima.
+"""
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+
+from bubbles_amd import scenes
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow,
+              pytest.mark.skipif(os.environ.get("BBX_LONG_PRECISION") != "1",
+                                 reason="long precision study: set BBX_LONG_PRECISION=1")]
+
+
+def _errors(g, o, h):
+    rg, ro = g.get("density"), o.get("density")
+    drel = np.abs(rg - ro) / ro
+    fg, fo = g.get("pressure_force"), o.get("pressure_force")
+    fscale = max(np.abs(fo).max(), 1e-30)
+    ferr = np.abs(fg - fo).max(axis=1) / fscale
+    perr = np.abs(g.get("position") - o.get("position")).max(axis=1) / h
+    return {"density_rel_max": float(drel.max()), "density_rel_mean": float(drel.mean()),
+            "fp_rel_max": float(ferr.max()), "fp_rel_mean": float(ferr.mean()),
+            "pos_over_h_max": float(perr.max()), "pos_over_h_mean": float(perr.mean())}
+
+
+def _record(key, value):
+    print(json.dumps({key: value}), flush=True)
+    out = os.environ.get("BBX_PRECISION_OUT")
+    if out:
+        rec = {}
+        if os.path.exists(out):
+            with open(out) as f:
+                rec = json.load(f)
+        rec[key] = value
+        with open(out, "w") as f:
+            json.dump(rec, f, indent=1)
+
+
+def _pair(bbx, oracle, dims):
+    sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position").astype(np.float64)), threads=16)
+    return sc, g, o
+
+
+@pytest.mark.timeout(1800)
+@pytest.mark.parametrize("dims,steps,checkpoints", [
+    ((64, 64, 64), 1000, (1, 10, 100, 300, 1000)),
+    ((256, 64, 256), 20, (1, 10, 20)),
+])
+def test_plain_exact_long(bbx, oracle, dims, steps, checkpoints):
+    sc, g, o = _pair(bbx, oracle, dims)
+    table = {}
+    for k in range(1, steps + 1):
+        g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
+        o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+        if k in checkpoints:
+            table[k] = _errors(g, o, sc.kernel_radius)
+            print(sc.n, k, table[k], flush=True)
+    _record(f"plain_exact_{sc.n}", table)
+    last = table[steps]
+    assert last["density_rel_max"] <= 6e-2 and last["density_rel_mean"] <= 1e-3, last
+    assert last["fp_rel_max"] <= 6e-2 and last["fp_rel_mean"] <= 1e-3, last
+    assert last["pos_over_h_max"] <= 6e-2, last
+    assert max(table[1]["density_rel_max"], table[1]["fp_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
+
+
+@pytest.mark.timeout(1800)
+def test_learned_exact_against_fp64(bbx, oracle):
+    from bubbles_amd import learned
+
+    sc, g, o = _pair(bbx, oracle, (64, 64, 64))
+    g.attach_mlp(learned.load())
+    table, steps = {}, 200
+    for k in range(1, steps + 1):
+        g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT, bbx.WARM_MLP)
+        o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT,
+                         warm=g.get("warm_pressure").astype(np.float64))
+        if k in (1, 10, 100, 200):
+            table[k] = _errors(g, o, sc.kernel_radius)
+            print(sc.n, k, table[k], flush=True)
+    _record(f"learned_exact_{sc.n}", table)
+    last = table[steps]
+    assert last["density_rel_max"] <= 6e-2 and last["pos_over_h_max"] <= 6e-2, last
+    assert max(table[1]["density_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
+
+
+@pytest.mark.timeout(1800)
+def test_textbook_iteration_drift(bbx, oracle):
+    sc, g, o = _pair(bbx, oracle, (64, 64, 64))
+    its_g, its_o = [], []
+    for _ in range(200):
+        its_g.append(g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK).iterations)
+        its_o.append(o.pcisph_substep(sc.dt, 8, 0.01, oracle.MODE_TEXTBOOK).iterations)
+    a, b = np.array(its_g), np.array(its_o)
+    rec = {"substeps": 200, "mean_iterations_fp32": float(a.mean()), "mean_iterations_fp64": float(b.mean()),
+           "equal_fraction": float(np.mean(a == b)), "mean_abs_diff": float(np.mean(np.abs(a - b))),
+           "max_abs_diff": int(np.abs(a - b).max()),
+           "final": _errors(g, o, sc.kernel_radius)}
+    _record(f"textbook_drift_{sc.n}", rec)
+    assert abs(a.mean() - b.mean()) <= 0.5, rec
