@@ -133,6 +133,9 @@ def load():
         "bbx_stream_copy": (i, [d, i, dp, dp]),
         "bbx_serializer_write": (i, [C.c_char_p, i, d, i, dp, dp, dp, d, ip]),
         "bbx_solver_save": (i, [vp, C.c_char_p, i]),
+        "bbx_solver_reserve": (i, [vp, i]),
+        "bbx_solver_capacity": (i, [vp, ip]),
+        "bbx_solver_append": (i, [vp, dp, dp, i]),
         "bbx_serializer_read": (i, [C.c_char_p, i, ip, ip, dp, dp, dp, dp, ip]),
         "bbx_serializer_parse_number": (i, [C.c_char_p, dp]),
     }
@@ -273,6 +276,22 @@ class Solver:
         check(load().bbx_solver_info(self.h, C.byref(inf)), "bbx_solver_info")
         return {"count": inf.count, "mass": inf.mass, "kernel_radius": inf.kernel_radius,
                 "radius": inf.radius, "delta_denom": inf.delta_denom, "grid": inf.grid}
+
+    def reserve(self, capacity: int):
+        check(load().bbx_solver_reserve(self.h, int(capacity)), "bbx_solver_reserve")
+
+    def capacity(self) -> int:
+        c = C.c_int()
+        check(load().bbx_solver_capacity(self.h, C.byref(c)), "bbx_solver_capacity")
+        return c.value
+
+    def append(self, positions, velocities=None):
+        """Add particles (ids n, n+1, ...) at the tails of their cells' chains."""
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
+        check(load().bbx_solver_append(self.h, _dp(p), _dp(v) if v is not None else None, p.shape[0]),
+              "bbx_solver_append")
+        self.n += p.shape[0]
 
     def save(self, path, flags=SER_POSITION | SER_VELOCITY | SER_DENSITY):
         """Append this solver's particles as one serializer block."""

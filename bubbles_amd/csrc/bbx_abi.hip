@@ -668,6 +668,130 @@ int bbx_solver_download(bbx_solver *s, int field, double *out) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// growing particle sets (ContinuousParticleSetBuilder, src/core/grid.h:1246-1450)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+namespace {
+// replace *p by a buffer of new_n elements holding the first keep elements
+template <typename T>
+int regrow(bbx_solver *s, T *&p, size_t new_n, size_t keep) {
+    T *q = nullptr;
+    BBX_CHECK(dalloc(&q, new_n));
+    if (keep && p) {
+        hipError_t e = hipMemcpyAsync(q, p, sizeof(T) * keep, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) {
+            dfree(q);
+            return fail(BBX_E_HIP, std::string("regrow copy: ") + hipGetErrorString(e));
+        }
+    }
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    dfree(p);
+    p = q;
+    return BBX_OK;
+}
+}  // namespace
+extern "C" {
+
+int bbx_solver_reserve(bbx_solver *s, int capacity) {
+    BBX_CHECK(check_solver(s));
+    if (capacity < 0) return fail(BBX_E_INVALID, "negative capacity");
+    const size_t S0 = (size_t)s->stride;
+    const size_t S1 = (size_t)(((capacity + 63) / 64) * 64);
+    if (S1 <= S0) return BBX_OK;
+    const size_t n = (size_t)s->n;
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    // particle state kept (first n slots), scratch and per-substep tables not
+    BBX_CHECK(regrow(s, s->posA, S1, n));
+    BBX_CHECK(regrow(s, s->velA, S1, n));
+    BBX_CHECK(regrow(s, s->cellA, S1, n));
+    BBX_CHECK(regrow(s, s->home, S1, n));
+    BBX_CHECK(regrow(s, s->force, S1, n));
+    BBX_CHECK(regrow(s, s->fp, S1, n));
+    BBX_CHECK(regrow(s, s->xstar, S1, n));
+    BBX_CHECK(regrow(s, s->xq, S1, n));
+    BBX_CHECK(regrow(s, s->rho, S1, n));
+    BBX_CHECK(regrow(s, s->pres, S1, n));
+    BBX_CHECK(regrow(s, s->rho_star, S1, n));
+    BBX_CHECK(regrow(s, s->pwarm, S1, n));
+    BBX_CHECK(regrow(s, s->posB, S1, 0));
+    BBX_CHECK(regrow(s, s->velB, S1, 0));
+    BBX_CHECK(regrow(s, s->cellB, S1, 0));
+    BBX_CHECK(regrow(s, s->key, S1, 0));
+    BBX_CHECK(regrow(s, s->newc, S1, 0));
+    BBX_CHECK(regrow(s, s->rank_tmp, S1, 0));
+    BBX_CHECK(regrow(s, s->tmp_idx, S1, 0));
+    BBX_CHECK(regrow(s, s->hash_tmp, S1, 0));
+    BBX_CHECK(regrow(s, s->nbr16, S1 * kBucket, 0));
+    BBX_CHECK(regrow(s, s->nbr32, S1 * kBucket, 0));
+    BBX_CHECK(regrow(s, s->nbr_cnt, S1, 0));
+    // features are feature-major with the row stride: copy row by row
+    {
+        float *q = nullptr;
+        BBX_CHECK(dalloc(&q, S1 * BBX_NFEAT));
+        for (int f = 0; f < BBX_NFEAT && n; f++)
+            BBX_HIP_CHECK(hipMemcpyAsync(q + (size_t)f * S1, s->feat + (size_t)f * S0, sizeof(float) * n,
+                                         hipMemcpyDeviceToDevice, s->stream));
+        BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+        dfree(s->feat);
+        s->feat = q;
+    }
+    s->ntiles = (int)((S1 + kBlock - 1) / kBlock);
+    BBX_CHECK(regrow(s, s->tile_tab, (size_t)s->ntiles * kTabIntsHost, 0));
+    BBX_HIP_CHECK(hipMemset(s->nbr_cnt, 0, sizeof(int) * S1));
+    BBX_HIP_CHECK(hipMemset(s->tile_tab, 0, sizeof(int) * (size_t)s->ntiles * kTabIntsHost));
+    // slots beyond n are in no chain
+    std::vector<int> none(S1 - n, (int)s->grid.total);
+    BBX_HIP_CHECK(hipMemcpy(s->cellA + n, none.data(), sizeof(int) * none.size(), hipMemcpyHostToDevice));
+    BBX_HIP_CHECK(hipMemcpy(s->home + n, none.data(), sizeof(int) * none.size(), hipMemcpyHostToDevice));
+    s->stride = (int)S1;
+    fill_sim_const(s);
+    return BBX_OK;
+}
+
+int bbx_solver_capacity(bbx_solver *s, int *capacity) {
+    BBX_CHECK(check_solver(s));
+    if (!capacity) return fail(BBX_E_INVALID, "null output");
+    *capacity = s->stride;
+    return BBX_OK;
+}
+
+int bbx_solver_append(bbx_solver *s, const double *pos, const double *vel, int count) {
+    BBX_CHECK(check_setup(s));
+    if (count < 0) return fail(BBX_E_INVALID, "negative count");
+    if (count == 0) return BBX_OK;
+    if (!pos) return fail(BBX_E_INVALID, "null positions");
+    if ((long long)s->n + count > 0x7fffffffLL / kBucket) return fail(BBX_E_RANGE, "too many particles");
+    const int n0 = s->n;
+    if (n0 + count > s->stride) BBX_CHECK(bbx_solver_reserve(s, std::max(2 * s->stride, n0 + count)));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    // AppendData (src/core/particle.h:232-240): ids n0.., zero force, and
+    // no previous pressure for the warm start
+    std::vector<float4> hp((size_t)count), hv((size_t)count);
+    for (int k = 0; k < count; k++) {
+        const int id = n0 + k;
+        float idf;
+        std::memcpy(&idf, &id, sizeof(int));
+        hp[k] = make_float4((float)pos[3 * (size_t)k], (float)pos[3 * (size_t)k + 1], (float)pos[3 * (size_t)k + 2], idf);
+        hv[k] = vel ? make_float4((float)vel[3 * (size_t)k], (float)vel[3 * (size_t)k + 1], (float)vel[3 * (size_t)k + 2], 0.f)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    BBX_HIP_CHECK(hipMemcpy(s->posA + n0, hp.data(), sizeof(float4) * count, hipMemcpyHostToDevice));
+    BBX_HIP_CHECK(hipMemcpy(s->velA + n0, hv.data(), sizeof(float4) * count, hipMemcpyHostToDevice));
+    BBX_HIP_CHECK(hipMemset(s->force + n0, 0, sizeof(float4) * count));
+    BBX_HIP_CHECK(hipMemset(s->fp + n0, 0, sizeof(float4) * count));
+    BBX_HIP_CHECK(hipMemset(s->rho + n0, 0, sizeof(float) * count));
+    BBX_HIP_CHECK(hipMemset(s->pres + n0, 0, sizeof(float) * count));
+    s->n = n0 + count;
+    fill_sim_const(s);
+    // DistributeByParticleList (src/core/grid.h:358-387): each new particle
+    // goes to the tail of its cell's chain; buckets come with the next substep
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(distribute(s, false, false, false, n0));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return BBX_OK;
+}
+
 int bbx_solver_save(bbx_solver *s, const char *path, int flags) {
     BBX_CHECK(check_solver(s));
     const size_t n = (size_t)s->n;

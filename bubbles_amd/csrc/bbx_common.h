@@ -236,7 +236,9 @@ void fill_sim_const(bbx_solver *s);
 
 int launch_fill_int(bbx_solver *s, int *p, int value, size_t n);
 int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure);
+// append_from >= 0: the append pass after new particles were placed in
+// slots append_from..n-1 (old chains kept, new particles at their tails)
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from = -1);
 int run_density_only(bbx_solver *s);
 // features: 0 none, 1 warm-start features, 2 features + network inference +
 // the warm k = 0 initialisation in the same kernel

@@ -127,9 +127,15 @@ int launch_scan(bbx_solver *s, const int *in, int *out, size_t n) {
 // seen from the new cell, old slot); a particle whose cell moved by more than
 // one in any axis, or that was in no chain, stays in no chain (the reference
 // only looks at the 27 neighbours' chains).
+// Append mode (append_from >= 0, after new particles were placed in slots
+// append_from..n-1): every old particle keeps its chain and place (key = old
+// slot), every new one joins the chain of the cell it hashes into, after the
+// old members in the order added (DistributeByParticleList + AddToChain,
+// src/core/grid.h:358-387,128-140).
 __global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__restrict__ pos,
                                                  const int *__restrict__ cell_old,
                                                  const int *__restrict__ flag, int force_full,
+                                                 int append_from,
                                                  int *__restrict__ newc, int *__restrict__ hcell,
                                                  unsigned long long *__restrict__ key,
                                                  int *__restrict__ count,
@@ -142,10 +148,16 @@ __global__ void __launch_bounds__(kBlock) k_hash(SimConst c, const float4 *__res
     const float4 p = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     int u[3], bad = 0;
     int cid = hash_cell(c.g, p, u, &bad);
-    bool full = force_full || (*flag != 0);
+    bool full = append_from < 0 && (force_full || (*flag != 0));
     int nc;
     unsigned long long k;
-    if (full) {
+    if (append_from >= 0) {
+        k = (unsigned long long)s;
+        if (s < append_from)
+            nc = valid ? cell_old[s] : none;
+        else
+            nc = bad ? none : cid;
+    } else if (full) {
         nc = bad ? none : cid;
         k = (unsigned long long)(unsigned)pid_of(p);
     } else {
@@ -196,12 +208,15 @@ __global__ void __launch_bounds__(kBlock) k_scatter(SimConst c, const int *__res
                                                     const int *__restrict__ count,
                                                     int *__restrict__ tmp_idx,
                                                     int *__restrict__ flag, int force_full,
+                                                    int append,
                                                     int *__restrict__ counters) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0) {
-        counters[CNT_FULL] = (force_full || *flag) ? 1 : 0;
         counters[CNT_LOST] = count[c.g.total];
-        *flag = 0;  // ResetHigherLevel
+        if (!append) {  // an append leaves the higher-level flag to the next substep
+            counters[CNT_FULL] = (force_full || *flag) ? 1 : 0;
+            *flag = 0;  // ResetHigherLevel
+        }
     }
     if (s >= c.n) return;
     tmp_idx[start[newc[s]] + rank_tmp[s]] = s;
@@ -561,7 +576,7 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // UpdateGridDistribution: full or incremental chain update, then buckets.
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure) {
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from) {
     SimConst &c = s->sc;
     const int n = s->n;
     const size_t cells = (size_t)c.g.total + 1;
@@ -570,7 +585,7 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
         {
             BBX_PROF_BEGIN("grid_hash");
             k_hash<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->posA, s->cellA, s->flag,
-                                                         force_full ? 1 : 0, s->newc,
+                                                         force_full ? 1 : 0, append_from, s->newc,
                                                          s->hash_tmp, s->key, s->cell_count,
                                                          s->rank_tmp, s->counters);
             BBX_HIP_CHECK(hipGetLastError());
@@ -586,7 +601,8 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
             k_scatter<<<nblocks(n), kBlock, 0, s->stream>>>(c, s->newc, s->rank_tmp,
                                                             s->cell_start, s->cell_count,
                                                             s->tmp_idx, s->flag,
-                                                            force_full ? 1 : 0, s->counters);
+                                                            force_full ? 1 : 0, append_from >= 0 ? 1 : 0,
+                                                            s->counters);
             BBX_HIP_CHECK(hipGetLastError());
             BBX_PROF_END();
         }

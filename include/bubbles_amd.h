@@ -231,6 +231,21 @@ int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m);
 int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,
                        float *targets, bbx_stats *stats);
 
+/* ---- growing particle sets ----
+ * ContinuousParticleSetBuilder3 (src/core/grid.h:1246-1450).  Capacity is
+ * the number of particles the device buffers hold (a multiple of 64).
+ * reserve keeps the state of the current particles; append grows the
+ * capacity itself when needed. */
+int bbx_solver_reserve(bbx_solver *s, int capacity);
+int bbx_solver_capacity(bbx_solver *s, int *capacity);
+/* Commit + DistributeByParticleList (src/core/grid.h:1410-1425,358-387):
+ * the `count` particles (pos 3*count doubles, vel NULL = zero) get ids
+ * n .. n+count-1 and join the tail of the chain of the cell they hash into,
+ * in the order given, after the chain's current members; a particle outside
+ * the grid is in no chain.  Buckets and densities are rebuilt by the next
+ * substep.  Needs bbx_solver_setup first. */
+int bbx_solver_append(bbx_solver *s, const double *pos, const double *vel, int count);
+
 /* ---- frame I/O in the reference's serializer format ----
  * Flag bits as SERIALIZER_* (src/third/serializer.h:7-17). */
 #define BBX_SER_POSITION 0x01

@@ -633,6 +633,49 @@ void bbo_sim_set_chains(bbo_sim *s, const int *cell_start, const int *members) {
     s->frame_index = 1;
 }
 
+/* ContinuousParticleSetBuilder::Commit with a mapped grid
+ * (src/core/grid.h:1410-1425): AppendData (src/core/particle.h:232-240)
+ * gives the new particles ids n.., then DistributeByParticleList
+ * (src/core/grid.h:358-387) appends each, in order, to the tail of the
+ * active chain of the cell it hashes into (AddToChain, :128-140) and builds
+ * its bucket. */
+static void *grow(void *p, size_t elem, size_t n) {
+    void *q = realloc(p, elem * (n > 0 ? n : 1));
+    return q;
+}
+void bbo_sim_append(bbo_sim *s, int count, const double *pos, const double *vel) {
+    if (count <= 0) return;
+    const int n0 = s->n, n1 = n0 + count;
+    const size_t nv = (size_t)n1;
+    for (int b = 0; b < 2; b++) s->next[b] = (int *)grow(s->next[b], sizeof(int), nv);
+    s->bucket_ids = (int *)grow(s->bucket_ids, sizeof(int) * BBO_BUCKET_SIZE, nv);
+    s->bucket_cnt = (int *)grow(s->bucket_cnt, sizeof(int), nv);
+    v3 **vs[] = {&s->pos, &s->vel, &s->force, &s->inter, &s->tpos, &s->tvel, &s->fp, &s->smooth};
+    for (size_t k = 0; k < sizeof vs / sizeof vs[0]; k++) {
+        *vs[k] = (v3 *)grow(*vs[k], sizeof(v3), nv);
+        memset(*vs[k] + n0, 0, sizeof(v3) * (size_t)count);
+    }
+    double **ds[] = {&s->density, &s->pressure, &s->derr, &s->dpred};
+    for (size_t k = 0; k < sizeof ds / sizeof ds[0]; k++) {
+        *ds[k] = (double *)grow(*ds[k], sizeof(double), nv);
+        memset(*ds[k] + n0, 0, sizeof(double) * (size_t)count);
+    }
+    memset(s->bucket_cnt + n0, 0, sizeof(int) * (size_t)count);
+    for (int k = 0; k < count; k++) {
+        s->pos[n0 + k] = mk(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+        if (vel) s->vel[n0 + k] = mk(vel[3 * k], vel[3 * k + 1], vel[3 * k + 2]);
+    }
+    s->n = n1;
+    for (int i = n0; i < n1; i++) {
+        double p[3] = {s->pos[i].x, s->pos[i].y, s->pos[i].z};
+        long long h = bbo_grid_hash(&s->grid, p, NULL);
+        if (h < 0) continue; /* outside the grid: the reference asserts */
+        cell_t *c = &s->cells[h];
+        add_to_chain(s, c, chain_sel(c), i);
+    }
+    parallel_for(s->threads, n0, n1, bucket_range, s);
+}
+
 void bbo_sim_get_buckets(const bbo_sim *s, int *counts, int *ids) {
     memcpy(counts, s->bucket_cnt, sizeof(int) * (size_t)s->n);
     if (ids) memcpy(ids, s->bucket_ids, sizeof(int) * BBO_BUCKET_SIZE * (size_t)s->n);

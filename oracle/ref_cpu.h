@@ -119,6 +119,8 @@ void bbo_sim_destroy(bbo_sim *s);
 /* PciSphSolver3::Setup / SphSolver3::Setup: mass, delta denominator, full
  * distribution (ascending id chains), frame_index = 1 */
 void bbo_sim_setup(bbo_sim *s);
+/* ContinuousParticleSetBuilder::Commit + DistributeByParticleList */
+void bbo_sim_append(bbo_sim *s, int count, const double *pos, const double *vel);
 void bbo_sim_info(const bbo_sim *s, double *mass, double *kernel_radius, double *radius,
                   double *delta_denom, bbo_grid_desc *grid);
 void bbo_sim_set_threads(bbo_sim *s, int threads);

@@ -84,6 +84,7 @@ def lib():
         L.bbo_sim_create.argtypes = [C.POINTER(Params), i, dp, dp]
         L.bbo_sim_destroy.argtypes = [vp]
         L.bbo_sim_setup.argtypes = [vp]
+        L.bbo_sim_append.argtypes = [vp, i, dp, dp]
         L.bbo_sim_info.argtypes = [vp, dp, dp, dp, dp, C.POINTER(GridDesc)]
         L.bbo_sim_set_threads.argtypes = [vp, i]
         L.bbo_sim_get.argtypes = [vp, C.c_char_p, dp]
@@ -213,6 +214,14 @@ class Sim:
         if getattr(self, "h", None):
             lib().bbo_sim_destroy(self.h)
             self.h = None
+
+    def append(self, positions, velocities=None):
+        """Commit of a continuous builder with a mapped grid: new ids n..,
+        each new particle at the tail of its cell's chain."""
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
+        lib().bbo_sim_append(self.h, p.shape[0], _dp(p), _dp(v) if v is not None else None)
+        self.n += p.shape[0]
 
     def info(self):
         m, kr, r, dd = C.c_double(), C.c_double(), C.c_double(), C.c_double()

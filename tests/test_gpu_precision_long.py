@@ -8,7 +8,7 @@ JSON.  Cases:
 
 * plain REFERENCE_EXACT (one iteration per substep, so both trajectories
   differ by arithmetic alone): 262,144 particles over 1000 substeps and
-  4,194,304 (256x64x256) over 20;
+  4,194,304 (256x64x256, which needs a 4x1x4 tank) over 20;
 * learned warm start, REFERENCE_EXACT, 262,144 over 200 substeps: every
   substep the oracle starts from the GPU's own warm pressures (the network is
   not part of the oracle), so the comparison isolates the FP32 solver;
@@ -16,9 +16,22 @@ JSON.  Cases:
   FP64 runs stop after a different number of corrections (each run on its own
   trajectory).
 
-Measured on MI355X (max relative density / pressure force / position in h):
-1000 substeps at 262k stay below 2e-2 / 2e-2 / 2e-2 with means near 1e-4;
-bounds below are about 3x the measured maxima.
+Measured on MI355X at 262k, REFERENCE_EXACT (density relative, pressure
+force relative to max |f_p|, position in units of h; max / mean):
+
+  substep   density           f_p               position
+  1         1.5e-7 / 2.7e-8   1.4e-4 / 9.6e-7   1.5e-6 / 3.3e-7
+  100       1.5e-3 / 8.6e-6   3.4e-3 / 3.0e-5   1.5e-3 / 1.6e-5
+  300       1.6e-2 / 5.3e-6   6.5e-2 / 4.9e-5   1.7e-2 / 2.3e-5
+  1000      1.6e-1 / 1.0e-4   6.3e-1 / 5.4e-4   5.3e-1 / 7.6e-4
+
+The means stay at round-off while the maxima grow about 10x per 300
+substeps: the splash is chaotic, and particles whose trajectories have
+separated by a fraction of h differ in everything.  Bounds are about 3x the
+measured values.  The learned warm start, fed the same warm pressures, stays
+as close (200 substeps: density 4.9e-3 max, position 6.0e-3 h max).
+TEXTBOOK over 200 substeps: 6.46 iterations (FP32) vs 6.47 (FP64), the
+same count in 91% of the substeps, never more than one apart.
 """
 import dataclasses
 import json
@@ -59,20 +72,24 @@ def _record(key, value):
             json.dump(rec, f, indent=1)
 
 
-def _pair(bbx, oracle, dims):
-    sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+def _pair(bbx, oracle, dims, tank=(2.0, 1.0, 2.0)):
+    sc = scenes.dam_break(*dims, spacing=0.01, tank=tank, seed=2)
     g = bbx.Solver(sc)
     o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position").astype(np.float64)), threads=16)
     return sc, g, o
 
 
 @pytest.mark.timeout(1800)
-@pytest.mark.parametrize("dims,steps,checkpoints", [
-    ((64, 64, 64), 1000, (1, 10, 100, 300, 1000)),
-    ((256, 64, 256), 20, (1, 10, 20)),
+@pytest.mark.parametrize("dims,tank,steps,checkpoints,bound", [
+    ((64, 64, 64), (2.0, 1.0, 2.0), 1000, (1, 10, 100, 300, 1000),
+     {"density_rel_max": 0.5, "density_rel_mean": 3e-4, "fp_rel_max": 2.0, "fp_rel_mean": 2e-3,
+      "pos_over_h_max": 1.5, "pos_over_h_mean": 3e-3}),
+    ((256, 64, 256), (4.0, 1.0, 4.0), 20, (1, 10, 20),
+     {"density_rel_max": 1e-3, "density_rel_mean": 3e-5, "fp_rel_max": 3e-2, "fp_rel_mean": 3e-4,
+      "pos_over_h_max": 1e-3, "pos_over_h_mean": 3e-5}),
 ])
-def test_plain_exact_long(bbx, oracle, dims, steps, checkpoints):
-    sc, g, o = _pair(bbx, oracle, dims)
+def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
+    sc, g, o = _pair(bbx, oracle, dims, tank)
     table = {}
     for k in range(1, steps + 1):
         g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
@@ -82,9 +99,8 @@ def test_plain_exact_long(bbx, oracle, dims, steps, checkpoints):
             print(sc.n, k, table[k], flush=True)
     _record(f"plain_exact_{sc.n}", table)
     last = table[steps]
-    assert last["density_rel_max"] <= 6e-2 and last["density_rel_mean"] <= 1e-3, last
-    assert last["fp_rel_max"] <= 6e-2 and last["fp_rel_mean"] <= 1e-3, last
-    assert last["pos_over_h_max"] <= 6e-2, last
+    for k, b in bound.items():
+        assert last[k] <= b, (k, last)
     assert max(table[1]["density_rel_max"], table[1]["fp_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
 
 
@@ -104,7 +120,8 @@ def test_learned_exact_against_fp64(bbx, oracle):
             print(sc.n, k, table[k], flush=True)
     _record(f"learned_exact_{sc.n}", table)
     last = table[steps]
-    assert last["density_rel_max"] <= 6e-2 and last["pos_over_h_max"] <= 6e-2, last
+    assert last["density_rel_max"] <= 2e-2 and last["density_rel_mean"] <= 2e-5, last
+    assert last["pos_over_h_max"] <= 2e-2 and last["pos_over_h_mean"] <= 5e-5, last
     assert max(table[1]["density_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
 
 
@@ -121,4 +138,4 @@ def test_textbook_iteration_drift(bbx, oracle):
            "max_abs_diff": int(np.abs(a - b).max()),
            "final": _errors(g, o, sc.kernel_radius)}
     _record(f"textbook_drift_{sc.n}", rec)
-    assert abs(a.mean() - b.mean()) <= 0.5, rec
+    assert abs(a.mean() - b.mean()) <= 0.5 and rec["equal_fraction"] >= 0.7 and rec["max_abs_diff"] <= 2, rec
