@@ -62,18 +62,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_dam_break.cpp")
-FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_dam_break")
+FACADE_DIR = os.path.join(ROOT, "tests", "cpp")
+FACADE_BIN = os.path.join(FACADE_DIR, "facade_dam_break")
+EMIT_BIN = os.path.join(FACADE_DIR, "facade_emit")
 
 
 def build_facade_example() -> str:
-    """Compile the C++ facade scene (host code only) against the library."""
+    """Compile the C++ facade scenes (host code only) against the library."""
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, FACADE_SRC, "-o", FACADE_BIN,
-           "-L" + PKG, "-lbubbles_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../bubbles_amd"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"facade example failed to build:\n{res.stdout}\n{res.stderr}")
+    for exe in (FACADE_BIN, EMIT_BIN):
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, exe + ".cpp", "-o", exe,
+               "-L" + PKG, "-lbubbles_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../bubbles_amd"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{exe}.cpp failed to build:\n{res.stdout}\n{res.stderr}")
     return FACADE_BIN
 
 

@@ -28,8 +28,13 @@
 #ifndef BUBBLES_FACADE_H
 #define BUBBLES_FACADE_H
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <functional>
+#include <map>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -335,6 +340,206 @@ inline SphParticleSet3 *SphParticleSet3FromBuilder(ParticleSetBuilder3 *builder)
     if (!builder) throw bubbles_error("Invalid builder pointer for SphParticleSet3FromBuilder");
     SphParticleSet3 *s = new SphParticleSet3();
     s->particleSet = builder->MakeParticleSet();
+    return s;
+}
+
+namespace bubbles_detail {
+/* the chains of a full distribution: every particle in the chain of its
+ * hashed cell, in ascending id (DistributeByParticle,
+ * src/core/grid.h:390-407); points outside the grid are in none */
+inline void hashed_chains(const Grid3 *g, const std::vector<double> &pos, int n, std::vector<int> &start,
+                          std::vector<int> &members) {
+    const unsigned total = g->GetCellCount();
+    start.assign((size_t)total + 1, 0);
+    std::vector<int> id((size_t)n);
+    if (n) bbx_throw_if(bbx_grid_hash_desc(&g->desc, pos.data(), n, id.data(), nullptr), "chains");
+    for (int i = 0; i < n; i++)
+        if (id[i] >= 0) start[(size_t)id[i] + 1]++;
+    for (unsigned c = 0; c < total; c++) start[c + 1] += start[c];
+    members.assign((size_t)start[total], -1);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; i++)
+        if (id[i] >= 0) members[(size_t)fill[id[i]]++] = i;
+}
+
+inline void require_domain(bbx_solver *dev, const Grid3 *g) {
+    bbx_info info;
+    bbx_throw_if(bbx_solver_info(dev, &info), "chains");
+    if (info.grid.total != g->GetCellCount()) throw bubbles_error("grid is not the solver's domain");
+}
+
+/* Cell chains of a particle set as CSR (cell_start total+1, members in
+ * chain order) plus its positions by id: the solver's chains once a solver
+ * owns the particles, else those of a full distribution (what Setup does). */
+inline void chain_snapshot(ParticleSet3 *ps, const Grid3 *g, std::vector<int> &start,
+                           std::vector<int> &members, std::vector<double> &pos) {
+    const int n = ps->GetParticleCount();
+    if (!ps->dev) {
+        pos = ps->pos;
+        hashed_chains(g, pos, n, start, members);
+        return;
+    }
+    require_domain(ps->dev, g);
+    const unsigned total = g->GetCellCount();
+    start.assign((size_t)total + 1, 0);
+    pos.assign(3 * (size_t)n, 0.0);
+    members.assign((size_t)std::max(n, 1), -1);
+    if (n) bbx_throw_if(bbx_solver_download(ps->dev, BBX_FIELD_POSITION, pos.data()), "chains");
+    bbx_throw_if(bbx_grid_get_chains(ps->dev, start.data(), members.data()), "chains");
+    members.resize((size_t)start[total]);
+}
+}  // namespace bubbles_detail
+
+/* ResetAndDistribute (src/core/grid.h:688-697): the chains of a full
+ * distribution.  Before a solver owns the particles there is no chain state
+ * to reset: Setup distributes. */
+inline void ResetAndDistribute(Grid3 *grid, ParticleSet3 *pSet) {
+    if (!grid || !pSet || !pSet->dev) return;
+    bubbles_detail::require_domain(pSet->dev, grid);
+    const int n = pSet->GetParticleCount();
+    std::vector<double> pos(3 * (size_t)n);
+    if (n) bbx_throw_if(bbx_solver_download(pSet->dev, BBX_FIELD_POSITION, pos.data()), "ResetAndDistribute");
+    std::vector<int> start, members;
+    bubbles_detail::hashed_chains(grid, pos, n, start, members);
+    if (members.empty()) members.push_back(-1);
+    bbx_throw_if(bbx_grid_set_chains(pSet->dev, start.data(), members.data()), "ResetAndDistribute");
+    pSet->Invalidate();
+}
+
+static const int kDefaultMaxParticles = 1000000;
+
+/* ContinuousParticleSetBuilder3 (src/core/grid.h:1245-1450): a particle set
+ * that keeps growing while a solver steps it.  AddParticle queues up to
+ * maxNumOfParticles; Commit hands the queue to the set: to the device when a
+ * solver owns it (bbx_solver_append: ids n.., each new particle at the tail
+ * of its cell's chain, DistributeByParticleList), else to the host arrays
+ * Setup will upload.  MapGrid records, for every cell a particle hashes into,
+ * the positions of that cell's chain; MapGridEmit adds those positions back
+ * wherever no particle of the cell's current chain lies closer than d. */
+class ContinuousParticleSetBuilder3 {
+  public:
+    std::vector<vec3f> positions, velocities, forces;
+    ParticleSet3 *particleSet;
+    Float kernelRadius = 2.0 * 0.02;
+    int maxNumOfParticles;
+    int reemitions = 0;
+    bool reemitOnce;
+    bool warned = false;
+    Grid3 *mappedDomain = nullptr;
+    std::set<unsigned int> mappedCellSet;
+    std::map<unsigned int, std::vector<vec3f>> mappedPositions;
+
+    explicit ContinuousParticleSetBuilder3(int maxParticles = kDefaultMaxParticles, bool reemitOnce_ = false)
+        : particleSet(new ParticleSet3()), maxNumOfParticles(std::max(1, maxParticles)), reemitOnce(reemitOnce_) {}
+
+    void SetKernelRadius(Float radius) { kernelRadius = radius; }
+    ParticleSet3 *GetParticleSet() { return particleSet; }
+    int GetParticleCount() { return particleSet->GetParticleCount(); }
+
+    int AddParticle(const vec3f &pos, const vec3f &vel = vec3f(0), const vec3f &force = vec3f(0)) {
+        const int total = particleSet->GetParticleCount() + (int)positions.size();
+        if (total + 1 <= maxNumOfParticles) {
+            positions.push_back(pos);
+            velocities.push_back(vel);
+            forces.push_back(force);
+            return 1;
+        }
+        if (!warned) {
+            std::printf("\nReached maximum builder capacity\n");
+            warned = true;
+        }
+        return 0;
+    }
+
+    void Commit() {
+        if (positions.empty()) return;
+        const int k = (int)positions.size();
+        std::vector<double> p(3 * (size_t)k), v(3 * (size_t)k);
+        for (int i = 0; i < k; i++)
+            for (int a = 0; a < 3; a++) {
+                p[3 * (size_t)i + a] = positions[(size_t)i][a];
+                v[3 * (size_t)i + a] = velocities[(size_t)i][a];
+            }
+        ParticleSet3 *ps = particleSet;
+        if (ps->dev) {
+            // the reference sizes the set for maxNumOfParticles up front
+            int cap = 0;
+            bbx_throw_if(bbx_solver_capacity(ps->dev, &cap), "Commit");
+            if (cap < maxNumOfParticles) bbx_throw_if(bbx_solver_reserve(ps->dev, maxNumOfParticles), "Commit");
+            bbx_throw_if(bbx_solver_append(ps->dev, p.data(), v.data(), k), "Commit");
+            ps->Invalidate();
+        } else {
+            ps->pos.insert(ps->pos.end(), p.begin(), p.end());
+            ps->vel.insert(ps->vel.end(), v.begin(), v.end());
+        }
+        ps->count += k;
+        positions.clear();
+        velocities.clear();
+        forces.clear();
+    }
+
+    void MapGrid(Grid3 *grid) {
+        if (!grid) throw bubbles_error("MapGrid: null grid");
+        mappedDomain = grid;
+        std::vector<int> start, members;
+        std::vector<double> pos;
+        bubbles_detail::chain_snapshot(particleSet, grid, start, members, pos);
+        const int n = particleSet->GetParticleCount();
+        for (int i = 0; i < n; i++) {
+            const unsigned h =
+                grid->GetLinearHashedPosition(vec3f(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]));
+            mappedCellSet.insert(h);
+            if (start[h + 1] == start[h])
+                throw bubbles_error("Called for MapGrid but grid is inconsistent, missing first distribution?");
+            std::vector<vec3f> cellPos;
+            for (int k = start[h]; k < start[h + 1]; k++) {
+                const size_t j = (size_t)members[(size_t)k];
+                cellPos.push_back(vec3f(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]));
+            }
+            mappedPositions[h] = cellPos;
+        }
+    }
+
+    void MapGridEmit(const std::function<vec3f(const vec3f &)> &velocity, Float d = 0.02) {
+        if (reemitions > 0 && reemitOnce) return;
+        if (warned) return;
+        if (!mappedDomain) throw bubbles_error("MapGridEmit: call MapGrid first");
+        std::vector<int> start, members;
+        std::vector<double> pos;
+        bubbles_detail::chain_snapshot(particleSet, mappedDomain, start, members, pos);
+        int numNewParticles = 0;
+        for (unsigned h : mappedCellSet) {
+            const std::vector<vec3f> &cellPos = mappedPositions[h];
+            const int size = start[h + 1] - start[h];
+            if (size >= BBX_BUCKET_SIZE) continue;
+            const int toInsert = std::min(BBX_BUCKET_SIZE - size, (int)cellPos.size());
+            for (int i = 0; i < toInsert; i++) {
+                const vec3f pi = cellPos[(size_t)i];
+                bool canAdd = true;
+                for (int k = start[h]; k < start[h + 1] && canAdd; k++) {
+                    const size_t j = (size_t)members[(size_t)k];
+                    canAdd = (vec3f(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]) - pi).Length() >= d;
+                }
+                if (!canAdd) continue;
+                if (AddParticle(pi, velocity(pi))) {
+                    numNewParticles++;
+                } else {
+                    Commit();
+                    return;
+                }
+            }
+        }
+        if (numNewParticles > 0) Commit();
+        reemitions += 1;
+    }
+};
+
+/* SphParticleSet3FromContinuousBuilder (src/core/particle.cpp:29-35): the
+ * builder keeps the particle set and grows it on every Commit. */
+inline SphParticleSet3 *SphParticleSet3FromContinuousBuilder(ContinuousParticleSetBuilder3 *builder) {
+    if (!builder) throw bubbles_error("Invalid builder pointer for SphParticleSet3FromContinuousBuilder");
+    SphParticleSet3 *s = new SphParticleSet3();
+    s->particleSet = builder->GetParticleSet();
     return s;
 }
 

@@ -2,6 +2,7 @@
 the C ABI: a scene written in the reference's style gives bit-identical
 results to the same scene run through the ABI, and its Setup scalars and
 grid equal the oracle's."""
+import dataclasses
 import json
 import os
 import subprocess
@@ -56,3 +57,87 @@ def test_facade_matches_abi_and_oracle(bbx, oracle):
     printed = np.array([[float("%g" % v) for v in row] for row in pos])
     np.testing.assert_allclose(r["position"], printed, rtol=1e-15, atol=0)
     np.testing.assert_allclose(r["density"], [float("%g" % v) for v in g.get("density")], rtol=1e-15)
+
+
+def _emit_scene():
+    """The emitter block of tests/cpp/facade_emit.cpp, in its AddParticle
+    order, with the velocity field (0.5 x, -1, 0)."""
+    s = 0.01
+    pts = [(-0.03 + x * s, 0.05 + y * s, -0.03 + z * s)
+           for z in range(6) for y in range(4) for x in range(6)]
+    pos = np.array(pts, dtype=np.float64)
+    return dataclasses.replace(facade_scene(), positions=pos, velocities=_emit_velocity(pos))
+
+
+def _emit_velocity(p):
+    p = np.asarray(p, dtype=np.float64).reshape(-1, 3)
+    return np.stack([0.5 * p[:, 0], -np.ones(len(p)), np.zeros(len(p))], axis=1)
+
+
+def _map_grid(g, pos):
+    """MapGrid (src/core/grid.h:1286-1321) right after a full distribution:
+    for every cell a particle hashes into, the positions of the cell's
+    particles in ascending id."""
+    mapped = {}
+    for i, h in enumerate(g.hash(pos)):
+        mapped.setdefault(int(h), []).append(pos[i])
+    return mapped
+
+
+def _map_grid_emit(g, mapped, d, cap, state):
+    """MapGridEmit (src/core/grid.h:1366-1406) with AddParticle's capacity
+    rule and Commit (:1408-1425), restated over the ABI."""
+    if state["warned"]:
+        return
+    start, members = g.chains()
+    pos = g.get("position")
+    new = []
+
+    def commit():
+        if new:
+            p = np.array(new)
+            g.append(p, _emit_velocity(p))
+
+    for h in sorted(mapped):
+        size = int(start[h + 1] - start[h])
+        if size >= 100:  # MaximumParticlesPerBucket
+            continue
+        chain = pos[members[start[h]:start[h + 1]]]
+        for pi in mapped[h][:min(100 - size, len(mapped[h]))]:
+            dd = chain - pi
+            dist = np.sqrt((dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2])
+            if np.any(dist < d):
+                continue
+            if g.n + len(new) + 1 <= cap:
+                new.append(pi)
+            else:
+                state["warned"] = True
+                commit()
+                return
+    commit()
+
+
+@pytest.mark.gpu
+def test_facade_continuous_emission_matches_abi(bbx):
+    frames, cap = 16, 576
+    out_path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"bbx_emit_{os.getpid()}.bin")
+    res = subprocess.run([hipbuild.EMIT_BIN, str(frames), str(cap), out_path],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    fpos = np.fromfile(out_path, dtype=np.float64).reshape(-1, 3)
+    os.remove(out_path)
+    sc = _emit_scene()
+    g = bbx.Solver(sc)
+    mapped = _map_grid(g, sc.positions)
+    assert out["mapped"] == len(mapped)
+    state, counts = {"warned": False}, []
+    for _ in range(frames):
+        g.pcisph_advance(1.0 / 240.0, 8, 0.01, bbx.MODE_TEXTBOOK)
+        _map_grid_emit(g, mapped, 0.01, cap, state)
+        counts.append(g.n)
+    assert out["counts"] == counts
+    assert out["warned"] == int(state["warned"])
+    # the scene exercises emission and the capacity limit
+    assert counts[-1] == cap and len(set(counts)) >= 3, counts
+    np.testing.assert_array_equal(fpos, g.get("position"))

@@ -25,6 +25,10 @@ force relative to max |f_p|, position in units of h; max / mean):
   300       1.6e-2 / 5.3e-6   6.5e-2 / 4.9e-5   1.7e-2 / 2.3e-5
   1000      1.6e-1 / 1.0e-4   6.3e-1 / 5.4e-4   5.3e-1 / 7.6e-4
 
+4,194,304 particles, 20 substeps: density 3.4e-4 / 9.7e-6, f_p 7.4e-3 /
+5.9e-5, position 3.0e-4 / 9.7e-6 h (the 1M case of the short study reads
+3.2e-4, 4.3e-3 and 3.0e-4 h at the same substep).
+
 The means stay at round-off while the maxima grow about 10x per 300
 substeps: the splash is chaotic, and particles whose trajectories have
 separated by a fraction of h differ in everything.  Bounds are about 3x the
@@ -101,7 +105,11 @@ def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
     last = table[steps]
     for k, b in bound.items():
         assert last[k] <= b, (k, last)
-    assert max(table[1]["density_rel_max"], table[1]["fp_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
+    # one substep: arithmetic alone (the f_p maximum is a tail statistic over
+    # all particles and grows with the count: 1.4e-4 at 262k, 1.7e-4 at 4M)
+    first = table[1]
+    assert first["density_rel_max"] <= 1e-6 and first["pos_over_h_max"] <= 1e-5, first
+    assert first["fp_rel_max"] <= 5e-4, first
 
 
 @pytest.mark.timeout(1800)
