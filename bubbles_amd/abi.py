@@ -65,6 +65,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BbxCollider(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reverse_orientation", C.c_int), ("active", C.c_int), ("pad", C.c_int),
+                ("friction", C.c_double), ("to_world", C.c_double * 12), ("to_object", C.c_double * 12),
+                ("size", C.c_double * 3), ("radius", C.c_double), ("linear_velocity", C.c_double * 3),
+                ("angular_velocity", C.c_double * 3), ("bounds_min", C.c_double * 3),
+                ("bounds_max", C.c_double * 3), ("sdf", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -138,6 +146,14 @@ def load():
         "bbx_solver_append": (i, [vp, dp, dp, i]),
         "bbx_serializer_read": (i, [C.c_char_p, i, ip, ip, dp, dp, dp, dp, ip]),
         "bbx_serializer_parse_number": (i, [C.c_char_p, dp]),
+        "bbx_sdf_create": (i, [ip, dp, dp, dp, C.POINTER(vp)]),
+        "bbx_sdf_from_mesh": (i, [dp, i, ip, i, d, d, C.POINTER(vp)]),
+        "bbx_sdf_describe": (i, [vp, ip, dp, dp]),
+        "bbx_sdf_download": (i, [vp, dp]),
+        "bbx_sdf_sample": (i, [vp, dp, i, dp, dp]),
+        "bbx_sdf_destroy": (None, [vp]),
+        "bbx_solver_set_colliders": (i, [vp, C.POINTER(BbxCollider), i]),
+        "bbx_colliders_resolve": (i, [C.POINTER(BbxCollider), i, d, d, dp, dp, i, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -220,6 +236,105 @@ def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+class Sdf:
+    """A FieldGrid3f on the device (bbx_sdf)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_grid(cls, grid):
+        """Upload a shapes.SdfGrid."""
+        h = C.c_void_p()
+        nodes = np.ascontiguousarray(grid.nodes, dtype=np.int32)
+        org = np.ascontiguousarray(grid.origin, dtype=np.float64)
+        spc = np.ascontiguousarray(grid.spacing, dtype=np.float64)
+        vals = grid.flat()
+        check(load().bbx_sdf_create(_ip(nodes), _dp(org), _dp(spc), _dp(vals), C.byref(h)), "bbx_sdf_create")
+        return cls(h)
+
+    @classmethod
+    def from_mesh(cls, vertices, triangles, dx=0.01, margin=0.1):
+        """GenerateShapeSDF of a closed triangle mesh, built on the device."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(load().bbx_sdf_from_mesh(_dp(v), v.shape[0], _ip(t), t.shape[0], float(dx), float(margin),
+                                       C.byref(h)), "bbx_sdf_from_mesh")
+        return cls(h)
+
+    def describe(self):
+        nodes = np.zeros(3, dtype=np.int32)
+        org, spc = np.zeros(3), np.zeros(3)
+        check(load().bbx_sdf_describe(self.h, _ip(nodes), _dp(org), _dp(spc)), "bbx_sdf_describe")
+        return tuple(int(x) for x in nodes), tuple(org), tuple(spc)
+
+    def download(self):
+        from bubbles_amd.shapes import SdfGrid
+
+        nodes, org, spc = self.describe()
+        vals = np.zeros(int(np.prod(nodes)))
+        check(load().bbx_sdf_download(self.h, _dp(vals)), "bbx_sdf_download")
+        return SdfGrid(nodes, org, spc, vals)
+
+    def sample(self, points):
+        """(values, gradients) of FieldGrid::Sample / ::Gradient on the device."""
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        vals, grads = np.zeros(p.shape[0]), np.zeros((p.shape[0], 3))
+        check(load().bbx_sdf_sample(self.h, _dp(p), p.shape[0], _dp(vals), _dp(grads)), "bbx_sdf_sample")
+        return vals, grads
+
+    def close(self):
+        if self.h:
+            load().bbx_sdf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def collider_structs(colliders):
+    """bbx_collider array for a list of shapes.Collider; host SDF grids are
+    uploaded (the returned keep-alive list holds the device copies)."""
+    arr = (BbxCollider * max(1, len(colliders)))()
+    keep = []
+    for k, c in enumerate(colliders):
+        b = arr[k]
+        b.kind, b.reverse_orientation, b.active = int(c.kind), int(bool(c.reverse_orientation)), int(bool(c.active))
+        b.friction = float(c.friction)
+        tw = np.asarray(c.to_world, dtype=np.float64)
+        to = np.linalg.inv(tw)
+        b.to_world[:] = list(tw[:3, :].reshape(-1))
+        b.to_object[:] = list(to[:3, :].reshape(-1))
+        b.size[:] = list(c.size)
+        b.radius = float(c.radius)
+        b.linear_velocity[:] = list(c.linear_velocity)
+        b.angular_velocity[:] = list(c.angular_velocity)
+        b.bounds_min[:] = list(c.bounds_min)
+        b.bounds_max[:] = list(c.bounds_max)
+        if c.sdf is not None:
+            dev = c.sdf if isinstance(c.sdf, Sdf) else Sdf.from_grid(c.sdf)
+            keep.append(dev)
+            b.sdf = dev.h
+    return arr, keep
+
+
+def colliders_resolve(colliders, radius, restitution, positions, velocities):
+    """ColliderSet3::ResolveCollision on the device for every point:
+    (positions, velocities, collided)."""
+    arr, keep = collider_structs(colliders)
+    p = np.array(positions, dtype=np.float64).reshape(-1, 3)
+    v = np.array(velocities, dtype=np.float64).reshape(-1, 3)
+    hit = np.zeros(p.shape[0], dtype=np.int32)
+    check(load().bbx_colliders_resolve(arr, len(colliders), float(radius), float(restitution), _dp(p), _dp(v),
+                                       p.shape[0], _ip(hit)), "bbx_colliders_resolve")
+    del keep
+    return p, v, hit
+
+
 def params_for(scene, **over) -> Params:
     p = Params()
     check(load().bbx_default_params(C.byref(p)), "bbx_default_params")
@@ -292,6 +407,12 @@ class Solver:
         check(load().bbx_solver_append(self.h, _dp(p), _dp(v) if v is not None else None, p.shape[0]),
               "bbx_solver_append")
         self.n += p.shape[0]
+
+    def set_colliders(self, colliders):
+        """Replace the tank by a collider list (empty: back to the tank)."""
+        arr, keep = collider_structs(colliders)
+        check(load().bbx_solver_set_colliders(self.h, arr, len(colliders)), "bbx_solver_set_colliders")
+        self._collider_keep = keep  # device SDFs must outlive the list
 
     def save(self, path, flags=SER_POSITION | SER_VELOCITY | SER_DENSITY):
         """Append this solver's particles as one serializer block."""

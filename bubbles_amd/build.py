@@ -19,7 +19,8 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libbubbles_amd.so")
 
-SOURCES = ["bbx_abi.hip", "bbx_grid.hip", "bbx_sph.hip", "bbx_mlp.hip", "bbx_stream.hip", "bbx_io.hip"]
+SOURCES = ["bbx_abi.hip", "bbx_grid.hip", "bbx_sph.hip", "bbx_mlp.hip", "bbx_stream.hip", "bbx_io.hip",
+           "bbx_shapes.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

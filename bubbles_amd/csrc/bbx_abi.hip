@@ -316,6 +316,8 @@ void fill_sim_const(bbx_solver *s) {
     // (0 .. kTileCap2) lower them to time and test the global-memory paths
     c.tile_cap = kTileCapHost;
     c.tile_cap2 = kTileCap2Host;
+    c.ncoll = s->ncoll;
+    c.coll = s->coll;
     if (const char *e = std::getenv("BBX_TILE_CAP")) {
         int v = std::atoi(e);
         c.tile_cap = v < 0 ? 0 : (v > kTileCapHost ? kTileCapHost : v);
@@ -384,6 +386,8 @@ static void free_all(bbx_solver *s) {
     dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->xq);
     dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
     dfree(s->loop); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
+    dfree(s->coll);
+    s->ncoll = 0;
     if (s->h_loop) (void)hipHostFree(s->h_loop);
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     if (s->h_max_force) (void)hipHostFree(s->h_max_force);

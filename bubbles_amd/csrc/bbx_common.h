@@ -65,6 +65,22 @@ struct GridGeom {
 };
 
 // Per-solver scalars passed to kernels by value.
+// One collider on the device, FP32 (built on the host from bbx_collider).
+struct DevCollider {
+    int kind, reverse, active, pad;
+    float friction;
+    float m[12], mi[12];     // ObjectToWorld / WorldToObject, rows 0..2
+    float hx, hy, hz;        // box half sizes
+    float radius;            // sphere
+    float lin[3], ang[3];    // sphere velocity (VelocityAt)
+    float bmin[3], bmax[3];  // mesh bounds
+    const float *field;      // SDF nodes, x fastest
+    int nx, ny, nz, pad2;
+    float ox, oy, oz;        // SDF origin
+    float sx, sy, sz;        // SDF spacing
+};
+constexpr int kMaxColliders = 64;
+
 struct SimConst {
     int n;             // particles
     int stride;        // neighbour-list row stride (>= n, multiple of 64)
@@ -89,6 +105,8 @@ struct SimConst {
     int tile_cap;      // tiles with more staged candidates read global memory
     int tile_cap2;     // the same for kernels that stage two arrays
     int row_max;       // bucket build: rows up to this many candidates use the mask path (<= 64)
+    int ncoll;         // 0: the tank (reversed box above); else the collider list
+    const DevCollider *coll;
 };
 
 // Per-call step constants.
@@ -224,6 +242,9 @@ struct bbx_solver {
 
     bbx_mlp *mlp = nullptr;
     bbx::Profiler prof;
+    // collider list (bbx_solver_set_colliders); none: the params' tank
+    bbx::DevCollider *coll = nullptr;
+    int ncoll = 0;
 };
 
 // ---------------------------------------------------------------------------

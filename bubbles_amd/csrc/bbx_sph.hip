@@ -15,6 +15,7 @@
 //   pressure force + integrate       PredictPressureForceFor + AccumulateForcesFor + TimeIntegrationFor
 //
 // Arithmetic is FP32; density and pressure-force sums use Kahan compensation.
+#include "bbx_collider.h"
 #include "bbx_mlp_device.h"
 #include "bbx_tile.h"
 
@@ -43,6 +44,15 @@ struct NbrTab {
     const int *__restrict__ tab;
 };
 
+// The tank alone takes the inlined reversed-box path; a collider list goes
+// through the collider-set rule (same result for the tank as its only entry).
+__device__ __forceinline__ void collide_any(const SimConst &c, float e, float3 &p, float3 &v) {
+    if (c.ncoll == 0)
+        collide(c, e, p, v);
+    else
+        resolve_colliders(c.coll, c.ncoll, c.radius, e, p, v);
+}
+
 // PredictVelocityAndPositionFor (src/equations/pcisph_equations3.cpp:3-28):
 // v* = v + dt/m (f + f_p), x* = x + dt v*, collider with restitution 0.
 __device__ __forceinline__ float4 predict_position(const SimConst &c, const StepConst &st,
@@ -51,7 +61,7 @@ __device__ __forceinline__ float4 predict_position(const SimConst &c, const Step
     float3 tv = make_float3(v.x + st.dt_over_m * fx, v.y + st.dt_over_m * fy,
                             v.z + st.dt_over_m * fz);
     float3 tp = make_float3(x.x + st.dt * tv.x, x.y + st.dt * tv.y, x.z + st.dt * tv.z);
-    collide(c, 0.f, tp, tv);
+    collide_any(c, 0.f, tp, tv);
     return make_float4(tp.x, tp.y, tp.z, 0.f);
 }
 
@@ -63,7 +73,7 @@ __device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float
     const float inv_m = 1.f / c.mass;
     v = make_float3(vi.x + dt * (Fx * inv_m), vi.y + dt * (Fy * inv_m), vi.z + dt * (Fz * inv_m));
     p = make_float3(xi.x + dt * v.x, xi.y + dt * v.y, xi.z + dt * v.z);
-    collide(c, 0.6f, p, v);
+    collide_any(c, 0.6f, p, v);
     if (!inside_box(p.x, p.y, p.z, c.bmin, c.bmax)) {
         p.x = fminf(fmaxf(p.x, c.bmin[0] + c.radius), c.bmax[0] - c.radius);
         p.y = fminf(fmaxf(p.y, c.bmin[1] + c.radius), c.bmax[1] - c.radius);

@@ -246,6 +246,63 @@ int bbx_solver_capacity(bbx_solver *s, int *capacity);
  * substep.  Needs bbx_solver_setup first. */
 int bbx_solver_append(bbx_solver *s, const double *pos, const double *vel, int count);
 
+/* ---- colliders ----
+ * ColliderSet3 / Collider3 / Shape (src/core/collider.cpp:3-44,109-264,
+ * src/shapes/box.cpp:74-172, src/shapes/sphere.cpp:47-70,
+ * src/shapes/bvh.cpp:212-242,666-706, src/core/shape.cpp:206-305,356-488,
+ * src/core/grid.h:1093-1153).  A solver without a collider list collides
+ * with the reversed box of its params (the tank); a list replaces it and is
+ * resolved per particle as ColliderSet3::ResolveCollision does: the active
+ * collider with the smallest |ClosestDistance| handles the particle. */
+#define BBX_SHAPE_BOX 0     /* MakeBox: size = edge lengths */
+#define BBX_SHAPE_SPHERE 1  /* MakeSphere: radius */
+#define BBX_SHAPE_MESH 2    /* MakeMesh: collides through its SDF, bounds early-out */
+#define BBX_SHAPE_SDF 3     /* a shape given only by an SDF grid */
+
+typedef struct bbx_sdf bbx_sdf;  /* FieldGrid3f, vertex centred, on the device */
+
+typedef struct bbx_collider {
+    int kind;                    /* BBX_SHAPE_* */
+    int reverse_orientation;     /* Shape::reverseOrientation */
+    int active;                  /* Collider3::isActive */
+    int pad;
+    double friction;             /* Collider3::frictionCoefficient */
+    double to_world[12];         /* ObjectToWorld, rows 0..2 of the 4x4 matrix */
+    double to_object[12];        /* WorldToObject (its inverse) */
+    double size[3];              /* box edge lengths */
+    double radius;               /* sphere radius */
+    double linear_velocity[3];   /* Shape::SetVelocities (sphere VelocityAt) */
+    double angular_velocity[3];
+    double bounds_min[3];        /* mesh: bounds of its vertices (OptmizedClosestPointCheck) */
+    double bounds_max[3];
+    const bbx_sdf *sdf;          /* MESH and SDF kinds */
+} bbx_collider;
+
+/* A FieldGrid3f from values: nodes[k] points per axis (x fastest), node
+ * (i,j,k) at origin + spacing * (i,j,k). */
+int bbx_sdf_create(const int nodes[3], const double origin[3], const double spacing[3],
+                   const double *values, bbx_sdf **out);
+/* GenerateShapeSDF for a mesh (src/core/shape.cpp:469-488,411-424): grid
+ * over the mesh bounds grown by margin * extent, spacing dx adjusted to the
+ * x extent; each node holds the distance to the closest triangle
+ * (DistanceTriangle, at least 1e-5), negative inside the mesh.  Triangles
+ * are 3 vertex indices each.  Built on the device. */
+int bbx_sdf_from_mesh(const double *vertices, int nvertices, const int *triangles, int ntriangles,
+                      double dx, double margin, bbx_sdf **out);
+int bbx_sdf_describe(const bbx_sdf *sdf, int nodes[3], double origin[3], double spacing[3]);
+int bbx_sdf_download(const bbx_sdf *sdf, double *values);
+/* FieldGrid::Sample and ::Gradient on the device (gradients may be NULL) */
+int bbx_sdf_sample(const bbx_sdf *sdf, const double *points, int n, double *values, double *gradients);
+void bbx_sdf_destroy(bbx_sdf *sdf);
+/* Replace the solver's colliders (count 0: back to the params' tank).  The
+ * SDFs must outlive their use by the solver. */
+int bbx_solver_set_colliders(bbx_solver *s, const bbx_collider *colliders, int count);
+/* ColliderSet3::ResolveCollision(radius, restitution, &p, &v) on the device
+ * for n points (in place); collided[i] = 1 where the particle was moved
+ * (collided may be NULL). */
+int bbx_colliders_resolve(const bbx_collider *colliders, int count, double radius, double restitution,
+                          double *pos, double *vel, int n, int *collided);
+
 /* ---- frame I/O in the reference's serializer format ----
  * Flag bits as SERIALIZER_* (src/third/serializer.h:7-17). */
 #define BBX_SER_POSITION 0x01

@@ -354,6 +354,9 @@ struct bbo_sim {
     int threads;
     /* tank (reversed box) */
     v3 tank_c, tank_h;
+    /* collider list replacing the tank (bbo_sim_set_colliders) */
+    bbo_collider *coll;
+    int ncoll;
     /* per-call parameters passed to range workers */
     double arg_dt, arg_delta;
     int arg_first, arg_record;
@@ -428,6 +431,7 @@ bbo_sim *bbo_sim_create(const bbo_params *p, int n, const double *pos, const dou
 
 void bbo_sim_destroy(bbo_sim *s) {
     if (!s) return;
+    free(s->coll);
     free(s->nbr); free(s->nbr_cnt); free(s->cells);
     free(s->next[0]); free(s->next[1]);
     free(s->bucket_ids); free(s->bucket_cnt);
@@ -866,7 +870,7 @@ static void box_closest_point(const bbo_sim *s, v3 point, v3 *qpoint, v3 *qnorma
 }
 
 /* ColliderSet3::ResolveCollision with the single container collider */
-static void resolve_collision(const bbo_sim *s, double radius, double e, v3 *pos, v3 *vel) {
+static void tank_collision(const bbo_sim *s, double radius, double e, v3 *pos, v3 *vel) {
     /* nearest collider by |ClosestDistance|: only one, so it always wins
      * unless the distance is not < Infinity (FLT_MAX) */
     double d = absf(box_closest_distance(s, *pos));
@@ -895,6 +899,361 @@ static void resolve_collision(const bbo_sim *s, double radius, double e, v3 *pos
         *vel = vadd(vadd(reln, relt), colv);
     }
     *pos = target;
+}
+
+/* ---- general colliders ---- */
+/* Transform::Point / ::Normal (src/core/transform.h:433-468), affine rows */
+static inline v3 cx_point(const double *m, v3 p) {
+    return mk(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+static inline v3 cx_normal(const double *mi, v3 n) {
+    return mk(mi[0] * n.x + mi[4] * n.y + mi[8] * n.z, mi[1] * n.x + mi[5] * n.y + mi[9] * n.z,
+              mi[2] * n.x + mi[6] * n.y + mi[10] * n.z);
+}
+static inline v3 cx_v3(const double *a) { return mk(a[0], a[1], a[2]); }
+static inline v3 vcross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* BoxClosestPoint (src/shapes/box.cpp:111-172); the distance is
+ * BoxClosestDistance's (:74-109), which picks the same closest point */
+static void cbox_query(const bbo_collider *c, v3 point, v3 *qp, v3 *qn, double *qd) {
+    v3 pl = cx_point(c->to_object, point);
+    double hx = c->size[0] / 2.0, hy = c->size[1] / 2.0, hz = c->size[2] / 2.0;
+    v3 lo = mk(-hx, -hy, -hz), hi = mk(hx, hy, hz);
+    v3 closest, normal = mk(1, 0, 0);
+    int neg = 0;
+    if (inside_box(pl, lo, hi)) {
+        closest = plane_closest(pl, hi, mk(1, 0, 0));
+        double d2 = vlen2(vsub(closest, pl));
+        for (int i = 1; i < 6; i++) {
+            v3 n = mk(PLANE_N[i][0], PLANE_N[i][1], PLANE_N[i][2]);
+            v3 local = plane_closest(pl, i < 3 ? hi : lo, n);
+            double ld2 = vlen2(vsub(local, pl));
+            if (ld2 < d2) { closest = local; normal = n; d2 = ld2; }
+        }
+        neg = 1;
+    } else {
+        closest = mk(clampr(pl.x, lo.x, hi.x), clampr(pl.y, lo.y, hi.y), clampr(pl.z, lo.z, hi.z));
+        v3 cov = vsub(pl, closest);
+        double maxc = vdot(cov, normal);
+        for (int i = 1; i < 6; i++) {
+            v3 n = mk(PLANE_N[i][0], PLANE_N[i][1], PLANE_N[i][2]);
+            double cosine = vdot(n, cov);
+            if (cosine > maxc) { normal = n; maxc = cosine; }
+        }
+    }
+    double distance = vdist(closest, pl);
+    if (neg) distance *= -1;
+    if (c->reverse_orientation) normal = vmul(normal, -1);
+    *qp = cx_point(c->to_world, closest);
+    *qn = cx_normal(c->to_object, normal);
+    *qd = distance;
+}
+
+/* SphereClosestDistance / SphereClosestPoint (src/shapes/sphere.cpp:47-70),
+ * VelocityAt (src/core/shape.cpp:300-305) */
+static double csphere_distance(const bbo_collider *c, v3 point) {
+    v3 pl = cx_point(c->to_object, point);
+    return vdist(pl, mk(0, 0, 0)) - c->radius;
+}
+static void csphere_query(const bbo_collider *c, v3 point, v3 *qp, v3 *qn, double *qd, v3 *qv) {
+    double d = csphere_distance(c, point);
+    v3 pl = cx_point(c->to_object, point);
+    v3 n = mk(0, 1, 0);
+    if (!(pl.x == 0 && pl.y == 0 && pl.z == 0)) n = vdiv(pl, vlen(pl));
+    v3 pn = vmul(n, c->radius);
+    if (c->reverse_orientation) n = vmul(n, -1);
+    v3 p = cx_point(c->to_world, pn);
+    *qp = p;
+    *qn = cx_normal(c->to_object, n);
+    *qd = d;
+    v3 r = vsub(p, mk(c->to_world[3], c->to_world[7], c->to_world[11]));
+    *qv = vadd(cx_v3(c->linear_velocity), vcross(cx_v3(c->angular_velocity), r));
+}
+
+/* GetBarycentric (src/core/sampling.h:7-31) */
+static void barycentric(double x, int nodes, int *i, double *f) {
+    double s = floor(x);
+    int high = nodes - 1;
+    if (high == 0) { *i = 0; *f = 0; }
+    else if (s < 0) { *i = 0; *f = 0; }
+    else if (s > (double)(high - 1)) { *i = high - 1; *f = 1; }
+    else { *i = (int)s; *f = x - s; }
+}
+static inline double lerpd(double a, double b, double t) { return (1 - t) * a + t * b; }
+
+/* FieldGrid::Sample (src/core/grid.h:1093-1131) with Trilerp (src/core/sampling.h:33-44) */
+double bbo_sdf_sample(const bbo_sdf *g, const double p[3]) {
+    int ii[3], jj[3];
+    double w[3];
+    for (int k = 0; k < 3; k++) {
+        double nrm = (p[k] - g->origin[k]) / g->spacing[k];
+        barycentric(nrm, g->nodes[k], &ii[k], &w[k]);
+        jj[k] = ii[k] + 1 < g->nodes[k] - 1 ? ii[k] + 1 : g->nodes[k] - 1;
+    }
+    const int sy = g->nodes[0], sz = g->nodes[0] * g->nodes[1];
+    const double *f = g->values;
+#define SDFV(a, b, c) f[(a) + (b) * sy + (c) * sz]
+    double f000 = SDFV(ii[0], ii[1], ii[2]), f100 = SDFV(jj[0], ii[1], ii[2]);
+    double f010 = SDFV(ii[0], jj[1], ii[2]), f110 = SDFV(jj[0], jj[1], ii[2]);
+    double f001 = SDFV(ii[0], ii[1], jj[2]), f101 = SDFV(jj[0], ii[1], jj[2]);
+    double f011 = SDFV(ii[0], jj[1], jj[2]), f111 = SDFV(jj[0], jj[1], jj[2]);
+#undef SDFV
+    double b0 = lerpd(lerpd(f000, f100, w[0]), lerpd(f010, f110, w[0]), w[1]);
+    double b1 = lerpd(lerpd(f001, f101, w[0]), lerpd(f011, f111, w[0]), w[1]);
+    return lerpd(b0, b1, w[2]);
+}
+
+/* FieldGrid::Gradient (src/core/grid.h:1133-1153): central differences */
+void bbo_sdf_gradient(const bbo_sdf *g, const double p[3], double out[3]) {
+    for (int i = 0; i < 3; i++) {
+        double a[3] = {p[0], p[1], p[2]}, b[3] = {p[0], p[1], p[2]};
+        a[i] += g->spacing[i];
+        b[i] -= g->spacing[i];
+        double inv = 1.0 / (2.0 * g->spacing[i]);
+        out[i] = (bbo_sdf_sample(g, a) - bbo_sdf_sample(g, b)) * inv;
+    }
+}
+static double sdf_at(const bbo_sdf *g, v3 p) { double q[3] = {p.x, p.y, p.z}; return bbo_sdf_sample(g, q); }
+static v3 sdf_grad(const bbo_sdf *g, v3 p) {
+    double q[3] = {p.x, p.y, p.z}, o[3];
+    bbo_sdf_gradient(g, q, o);
+    return mk(o[0], o[1], o[2]);
+}
+
+/* ClosestPointBySDF (src/shapes/bvh.cpp:666-706) */
+static void csdf_query(const bbo_collider *c, v3 pworld, v3 *qp, v3 *qn, double *qd) {
+    v3 tn = mk(0, 1, 0);
+    v3 point = cx_point(c->to_object, pworld);
+    v3 tp = point;
+    int has_gradient = 0;
+    for (int i = 0; i < 5; i++) {
+        double sdf = sdf_at(&c->sdf, tp);
+        if (absf(sdf) < 0.001) break;
+        tn = sdf_grad(&c->sdf, tp);
+        double len = vlen(tn);
+        if (len > 0) tn = vdiv(tn, len);
+        tp = vsub(tp, vmul(tn, sdf));
+        has_gradient = 1;
+    }
+    if (!has_gradient) {
+        tn = sdf_grad(&c->sdf, tp);
+        double len = vlen(tn);
+        if (len > 0) tn = vdiv(tn, len);
+    }
+    *qp = cx_point(c->to_world, tp);
+    *qn = cx_normal(c->to_object, tn);
+    *qd = vdist(point, tp);
+}
+
+/* Shape::ClosestDistance (src/core/shape.cpp:206-228); a mesh takes its SDF
+ * sample where the reference walks its BVH */
+double bbo_collider_closest_distance(const bbo_collider *c, const double p[3]) {
+    v3 q = mk(p[0], p[1], p[2]), cp, cn;
+    double d;
+    switch (c->kind) {
+        case BBO_SHAPE_BOX: cbox_query(c, q, &cp, &cn, &d); return d;
+        case BBO_SHAPE_SPHERE: return csphere_distance(c, q);
+        default: return bbo_sdf_sample(&c->sdf, p);
+    }
+}
+/* Shape::IsInside (src/core/shape.cpp:275-282) */
+int bbo_collider_is_inside(const bbo_collider *c, const double p[3]) {
+    if (c->kind == BBO_SHAPE_MESH || c->kind == BBO_SHAPE_SDF) return bbo_sdf_sample(&c->sdf, p) < 0;
+    return (c->reverse_orientation != 0) == !(bbo_collider_closest_distance(c, p) < 0);
+}
+void bbo_collider_closest_point(const bbo_collider *c, const double p[3], double point[3], double normal[3],
+                                double *signed_distance, double velocity[3]) {
+    v3 q = mk(p[0], p[1], p[2]), cp, cn, cv = mk(0, 0, 0);
+    double d;
+    if (c->kind == BBO_SHAPE_BOX) cbox_query(c, q, &cp, &cn, &d);
+    else if (c->kind == BBO_SHAPE_SPHERE) csphere_query(c, q, &cp, &cn, &d, &cv);
+    else csdf_query(c, q, &cp, &cn, &d);
+    point[0] = cp.x; point[1] = cp.y; point[2] = cp.z;
+    normal[0] = cn.x; normal[1] = cn.y; normal[2] = cn.z;
+    velocity[0] = cv.x; velocity[1] = cv.y; velocity[2] = cv.z;
+    *signed_distance = d;
+}
+
+static int in_bounds(const bbo_collider *c, v3 p) {
+    return inside_box(p, cx_v3(c->bounds_min), cx_v3(c->bounds_max));
+}
+
+/* CollisionHandle with Collider3::IsPenetrating / OptmizedClosestPointCheck
+ * (src/core/collider.cpp:3-44,113-131) */
+static int collision_handle(const bbo_collider *c, double radius, double e, v3 *pos, v3 *vel) {
+    if (c->kind == BBO_SHAPE_MESH && !in_bounds(c, *pos)) return 0;
+    double p[3] = {pos->x, pos->y, pos->z}, qp[3], qn[3], qv[3], qd;
+    bbo_collider_closest_point(c, p, qp, qn, &qd, qv);
+    if (!(bbo_collider_is_inside(c, p) || absf(qd) < radius)) return 0;
+    v3 n = cx_v3(qn), colv = cx_v3(qv);
+    v3 target = vadd(cx_v3(qp), vmul(n, radius));
+    v3 rel = vsub(*vel, colv);
+    double ndr = vdot(n, rel);
+    v3 reln = vmul(n, ndr);
+    v3 relt = vsub(rel, reln);
+    if (ndr < 0) {
+        v3 drn = vmul(reln, (-1.0 - e));
+        reln = vmul(reln, -e);
+        if (vlen2(relt) > 0) {
+            double scale = vlen(drn) / vlen(relt);
+            double fscale = 1.0 - c->friction * scale;
+            relt = vmul(relt, fmaxr(0, fscale));
+        }
+        *vel = vadd(vadd(reln, relt), colv);
+    }
+    *pos = target;
+    return 1;
+}
+
+/* ColliderSet3::ResolveCollision (src/core/collider.cpp:238-264) */
+static int resolve_set(const bbo_collider *cs, int count, double radius, double e, v3 *pos, v3 *vel) {
+    int target = -1;
+    double mind = R_INFINITY;
+    for (int i = 0; i < count; i++) {
+        const bbo_collider *c = &cs[i];
+        if (!c->active) continue;
+        if (c->kind == BBO_SHAPE_MESH && !in_bounds(c, *pos)) continue;
+        double p[3] = {pos->x, pos->y, pos->z};
+        double d = absf(bbo_collider_closest_distance(c, p));
+        if (d < mind) { target = i; mind = d; }
+    }
+    if (target < 0) return 0;
+    return collision_handle(&cs[target], radius, e, pos, vel);
+}
+
+void bbo_colliders_resolve(const bbo_collider *cs, int count, double radius, double restitution, double *pos,
+                           double *vel, int n, int *collided) {
+    for (int i = 0; i < n; i++) {
+        v3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        v3 v = mk(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+        int h = resolve_set(cs, count, radius, restitution, &p, &v);
+        pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+        vel[3 * i] = v.x; vel[3 * i + 1] = v.y; vel[3 * i + 2] = v.z;
+        if (collided) collided[i] = h;
+    }
+}
+
+void bbo_sim_set_colliders(bbo_sim *s, const bbo_collider *cs, int count) {
+    free(s->coll);
+    s->coll = NULL;
+    s->ncoll = 0;
+    if (count <= 0) return;
+    s->coll = (bbo_collider *)malloc(sizeof(bbo_collider) * (size_t)count);
+    memcpy(s->coll, cs, sizeof(bbo_collider) * (size_t)count);
+    s->ncoll = count;
+}
+
+static void resolve_collision(const bbo_sim *s, double radius, double e, v3 *pos, v3 *vel) {
+    if (s->ncoll > 0) resolve_set(s->coll, s->ncoll, radius, e, pos, vel);
+    else tank_collision(s, radius, e, pos, vel);
+}
+
+/* ---- mesh SDF: SetNodeSDFKernel / GenerateShapeSDF
+ *      (src/core/shape.cpp:411-424,478-505) ---- */
+static inline double sgnd(double a) { int t = a < 0 ? -1 : 0; return a > 0 ? 1 : t; }
+/* DistanceTriangle (src/shapes/bvh.cpp:212-242), squared */
+static double tri_dist2(const double *t, double px, double py, double pz) {
+    const double ax = t[0], ay = t[1], az = t[2], bx = t[3], by = t[4], bz = t[5];
+    const double cx = t[6], cy = t[7], cz = t[8];
+    const double bax = bx - ax, bay = by - ay, baz = bz - az, pax = px - ax, pay = py - ay, paz = pz - az;
+    const double cbx = cx - bx, cby = cy - by, cbz = cz - bz, pbx = px - bx, pby = py - by, pbz = pz - bz;
+    const double acx = ax - cx, acy = ay - cy, acz = az - cz, pcx = px - cx, pcy = py - cy, pcz = pz - cz;
+    const double nx = bay * acz - baz * acy, ny = baz * acx - bax * acz, nz = bax * acy - bay * acx;
+    const double s0 = (bay * nz - baz * ny) * pax + (baz * nx - bax * nz) * pay + (bax * ny - bay * nx) * paz;
+    const double s1 = (cby * nz - cbz * ny) * pbx + (cbz * nx - cbx * nz) * pby + (cbx * ny - cby * nx) * pbz;
+    const double s2 = (acy * nz - acz * ny) * pcx + (acz * nx - acx * nz) * pcy + (acx * ny - acy * nx) * pcz;
+    if (sgnd(s0) + sgnd(s1) + sgnd(s2) < 2.0) {
+        double t0 = clampr((bax * pax + bay * pay + baz * paz) / (bax * bax + bay * bay + baz * baz), 0.0001, 1.0);
+        double t1 = clampr((cbx * pbx + cby * pby + cbz * pbz) / (cbx * cbx + cby * cby + cbz * cbz), 0.0001, 1.0);
+        double t2 = clampr((acx * pcx + acy * pcy + acz * pcz) / (acx * acx + acy * acy + acz * acz), 0.0001, 1.0);
+        double ex = bax * t0 - pax, ey = bay * t0 - pay, ez = baz * t0 - paz;
+        double fx = cbx * t1 - pbx, fy = cby * t1 - pby, fz = cbz * t1 - pbz;
+        double gx = acx * t2 - pcx, gy = acy * t2 - pcy, gz = acz * t2 - pcz;
+        double d0 = ex * ex + ey * ey + ez * ez, d1 = fx * fx + fy * fy + fz * fz, d2 = gx * gx + gy * gy + gz * gz;
+        return fminr(fminr(d0, d1), d2);
+    }
+    double np = nx * pax + ny * pay + nz * paz;
+    return np * np / (nx * nx + ny * ny + nz * nz);
+}
+double bbo_triangle_distance(const double a[3], const double b[3], const double c[3], const double p[3]) {
+    double t[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+    return sqrt(tri_dist2(t, p[0], p[1], p[2]));
+}
+/* signed solid angle of a triangle seen from p (Van Oosterom-Strackee) */
+static double tri_solid_angle(const double *t, double px, double py, double pz) {
+    const double ax = t[0] - px, ay = t[1] - py, az = t[2] - pz;
+    const double bx = t[3] - px, by = t[4] - py, bz = t[5] - pz;
+    const double cx = t[6] - px, cy = t[7] - py, cz = t[8] - pz;
+    const double la = sqrt(ax * ax + ay * ay + az * az);
+    const double lb = sqrt(bx * bx + by * by + bz * bz);
+    const double lc = sqrt(cx * cx + cy * cy + cz * cz);
+    const double det = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
+    const double den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
+                       (bx * cx + by * cy + bz * cz) * la;
+    return 2.0 * atan2(det, den);
+}
+
+typedef struct {
+    const double *tri;
+    int ntri, nx, ny;
+    double o[3], dx;
+    double *values;
+} sdf_ctx;
+static void sdf_range(void *vctx, int i0, int i1) {
+    sdf_ctx *c = (sdf_ctx *)vctx;
+    for (int node = i0; node < i1; node++) {
+        int i = node % c->nx, j = (node / c->nx) % c->ny, k = node / (c->nx * c->ny);
+        double px = c->o[0] + c->dx * (double)i, py = c->o[1] + c->dx * (double)j, pz = c->o[2] + c->dx * (double)k;
+        double best = DBL_MAX, wind = 0;
+        for (int t = 0; t < c->ntri; t++) {
+            best = fminr(best, tri_dist2(&c->tri[9 * t], px, py, pz));
+            wind += tri_solid_angle(&c->tri[9 * t], px, py, pz);
+        }
+        int interior = absf(wind) > 2.0 * R_PI;
+        double psd = fmaxr(sqrt(best), 0.00001);
+        c->values[node] = interior ? -psd : psd;
+    }
+}
+
+int bbo_sdf_from_mesh(const double *vertices, int nvertices, const int *triangles, int ntriangles, double dx,
+                      double margin, int threads, int nodes[3], double origin[3], double spacing[3],
+                      double *values) {
+    if (nvertices < 3 || ntriangles < 1 || !(dx > 0)) return -1;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double *tri = (double *)malloc(sizeof(double) * 9 * (size_t)ntriangles);
+    for (int t = 0; t < ntriangles; t++)
+        for (int c = 0; c < 3; c++) {
+            int v = triangles[3 * t + c];
+            if (v < 0 || v >= nvertices) { free(tri); return -1; }
+            for (int k = 0; k < 3; k++) {
+                double x = vertices[3 * v + k];
+                tri[9 * t + 3 * c + k] = x;
+                lo[k] = fminr(lo[k], x);
+                hi[k] = fmaxr(hi[k], x);
+            }
+        }
+    double pmin[3], pmax[3];
+    for (int k = 0; k < 3; k++) {
+        double scale = hi[k] - lo[k];
+        pmin[k] = lo[k] - margin * scale;
+        pmax[k] = hi[k] + margin * scale;
+    }
+    double width = pmax[0] - pmin[0], height = pmax[1] - pmin[1], depth = pmax[2] - pmin[2];
+    int res = (int)ceil(width / dx);
+    double sp = width / (double)res;
+    int resy = (int)ceil(res * height / width);
+    int resz = (int)ceil(res * depth / width);
+    nodes[0] = res + 1; nodes[1] = resy + 1; nodes[2] = resz + 1;
+    for (int k = 0; k < 3; k++) { origin[k] = pmin[k]; spacing[k] = sp; }
+    if (values) {
+        sdf_ctx c = {tri, ntriangles, nodes[0], nodes[1], {pmin[0], pmin[1], pmin[2]}, sp, values};
+        parallel_for(threads, 0, nodes[0] * nodes[1] * nodes[2], sdf_range, &c);
+    }
+    free(tri);
+    return 0;
 }
 
 /* ---- PCISPH: src/equations/pcisph_equations3.cpp ---- */

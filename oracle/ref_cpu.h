@@ -18,6 +18,8 @@
  *                                           src/solvers/pcisph_solver3.cpp:103-189
  *   box collider (reversed tank) .......... src/core/collider.cpp:3-44,238-264,
  *                                           src/shapes/box.cpp:74-172, src/core/shape.cpp:275-282
+ *   collider sets (boxes, spheres, SDFs) .. src/core/collider.cpp:109-264, src/shapes/sphere.cpp:47-70,
+ *                                           src/shapes/bvh.cpp:212-242,666-706, src/core/grid.h:1093-1153
  * Data structures follow the reference (linked-list cell chains with two node
  * arrays, 100-slot buckets, AoS doubles) so the CPU timing is representative.
  */
@@ -167,6 +169,48 @@ int bbo_pcisph_advance(bbo_sim *s, double interval, int max_it, double tol, int 
 /* SPH (WCSPH) substep and Advance (scale 1) */
 void bbo_sph_substep(bbo_sim *s, double dt, int sph_mode, bbo_stats *st);
 int bbo_sph_advance(bbo_sim *s, double interval, int sph_mode);
+
+/* ---- colliders: ColliderSet3 / Collider3 / Shape (src/core/collider.cpp:3-44,
+ * 109-264; src/shapes/box.cpp:74-172; src/shapes/sphere.cpp:47-70;
+ * src/shapes/bvh.cpp:212-242,666-706; src/core/shape.cpp:206-305,411-424,
+ * 478-505; src/core/grid.h:1093-1153) ---- */
+enum { BBO_SHAPE_BOX = 0, BBO_SHAPE_SPHERE = 1, BBO_SHAPE_MESH = 2, BBO_SHAPE_SDF = 3 };
+typedef struct bbo_sdf {          /* FieldGrid3f, vertex centred */
+    int nodes[3];
+    double origin[3], spacing[3];
+    const double *values;         /* x fastest; owned by the caller */
+} bbo_sdf;
+typedef struct bbo_collider {
+    int kind, reverse_orientation, active, pad;
+    double friction;
+    double to_world[12], to_object[12];   /* rows 0..2 of ObjectToWorld / WorldToObject */
+    double size[3];
+    double radius;
+    double linear_velocity[3], angular_velocity[3];
+    double bounds_min[3], bounds_max[3];  /* mesh bounds */
+    bbo_sdf sdf;
+} bbo_collider;
+/* DistanceTriangle (src/shapes/bvh.cpp:212-242) */
+double bbo_triangle_distance(const double a[3], const double b[3], const double c[3], const double p[3]);
+/* GenerateShapeSDF for a mesh: with values NULL only the grid is described;
+ * else values gets every node (distance >= 1e-5, negative inside by the
+ * winding number). */
+int bbo_sdf_from_mesh(const double *vertices, int nvertices, const int *triangles, int ntriangles, double dx,
+                      double margin, int threads, int nodes[3], double origin[3], double spacing[3],
+                      double *values);
+double bbo_sdf_sample(const bbo_sdf *g, const double p[3]);
+void bbo_sdf_gradient(const bbo_sdf *g, const double p[3], double out[3]);
+/* Shape::ClosestPoint / ::ClosestDistance / ::IsInside */
+void bbo_collider_closest_point(const bbo_collider *c, const double p[3], double point[3], double normal[3],
+                                double *signed_distance, double velocity[3]);
+double bbo_collider_closest_distance(const bbo_collider *c, const double p[3]);
+int bbo_collider_is_inside(const bbo_collider *c, const double p[3]);
+/* ColliderSet3::ResolveCollision for n points (in place) */
+void bbo_colliders_resolve(const bbo_collider *cs, int count, double radius, double restitution, double *pos,
+                           double *vel, int n, int *collided);
+/* the simulation's collider list (count 0: the params' tank); the list is
+ * copied, SDF values are not */
+void bbo_sim_set_colliders(bbo_sim *s, const bbo_collider *cs, int count);
 
 /* learned warm-start features (FP64), 16 per particle, computed from the
  * state after density + non-pressure force of a substep; see features.h */
