@@ -9,8 +9,12 @@
  *   ParticleSet3, SphParticleSet3 ............. src/core/particle.h:154-401,508-609
  *   ParticleSetBuilder3, SphParticleSet3FromBuilder  src/core/particle.h:611-710, src/core/particle.cpp:21-27
  *   Grid3, UtilBuildGridForDomain .............. src/core/grid.h:197-600, src/core/util.cpp:269-287
- *   Transform (translation), MakeBox, Shape .... src/shapes/box.cpp:6-10
- *   ColliderSetBuilder3, ColliderSet3 .......... src/core/collider.h
+ *   Transform, Translate/Scale/RotateX|Y|Z ..... src/core/transform.h
+ *   Shape, MakeBox, MakeSphere, MakeMesh ....... src/core/shape.h:125-290, src/shapes/box.cpp:6-10,
+ *                                                src/shapes/sphere.cpp:5-9, src/shapes/bvh.cpp:51-61
+ *   ParsedMesh, LoadObj, GenerateShapeSDF ...... src/core/shape.h:12-22, src/third/obj_loader.cpp:434-441,
+ *                                                src/core/shape.cpp:478-505
+ *   Collider3, ColliderSetBuilder3, ColliderSet3  src/core/collider.h, src/core/collider.cpp:300-353
  *   SphSolverData3, DefaultSphSolverData3 ...... src/core/sph_solver.h:31-50, src/solvers/sph_solver3.cpp:124-151
  *   PciSphSolver3 .............................. src/core/pcisph_solver.h:56-83
  *   SphSolver3 ................................. src/core/sph_solver.h:76-96
@@ -20,7 +24,7 @@
  *     accessors download lazily (cached until the next step) and setters
  *     upload immediately;
  *   - errors throw bubbles_error instead of printing and exiting;
- *   - containers are reversed axis-aligned boxes placed by a translation;
+ *   - colliders are boxes, spheres and meshes (meshes through their SDF);
  *   - PciSphSolver3 exposes the predict-correct loop settings (the reference
  *     hard-codes maxIt 5, tol 0.01 and, in effect, one iteration).
  * The device path is FP32 (+ Kahan); Float stays double at this API.
@@ -31,6 +35,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -83,45 +89,268 @@ struct Bounds3f {
     vec3f Center() const { return (pMin + pMax) * 0.5; }
 };
 
-/* Translation-only transform (the reference's container transform in every
- * PCISPH scene is Transform() or Translate(...)). */
+/* Transform (src/core/transform.h): a 4x4 affine matrix and its inverse */
 struct Transform {
-    vec3f t;
-    Transform() {}
-    explicit Transform(const vec3f &tr) : t(tr) {}
+    Float m[4][4], mInv[4][4];
+    Transform() {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) m[i][j] = mInv[i][j] = i == j ? 1.0 : 0.0;
+    }
+    Transform(const Float a[4][4], const Float ai[4][4]) {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                m[i][j] = a[i][j];
+                mInv[i][j] = ai[i][j];
+            }
+    }
+    Transform operator*(const Transform &t) const {
+        Float r[4][4], ri[4][4];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                r[i][j] = ri[i][j] = 0;
+                for (int k = 0; k < 4; k++) {
+                    r[i][j] += m[i][k] * t.m[k][j];
+                    ri[i][j] += t.mInv[i][k] * mInv[k][j];
+                }
+            }
+        return Transform(r, ri);
+    }
+    vec3f Point(const vec3f &p) const {
+        return vec3f(m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3],
+                     m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3],
+                     m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3]);
+    }
+    /* the rows 0..2 of m and mInv, as bbx_collider wants them */
+    void Rows(double *to_world, double *to_object) const {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 4; j++) {
+                to_world[4 * i + j] = m[i][j];
+                to_object[4 * i + j] = mInv[i][j];
+            }
+    }
+    bool IsTranslation() const {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                if (m[i][j] != (i == j ? 1.0 : 0.0)) return false;
+        return true;
+    }
 };
-inline Transform Translate(Float x, Float y, Float z) { return Transform(vec3f(x, y, z)); }
-inline Transform Translate(const vec3f &v) { return Transform(v); }
+inline Transform Inverse(const Transform &t) { return Transform(t.mInv, t.m); }
+inline Transform Translate(Float x, Float y, Float z) {
+    Transform t;
+    t.m[0][3] = x; t.m[1][3] = y; t.m[2][3] = z;
+    t.mInv[0][3] = -x; t.mInv[1][3] = -y; t.mInv[2][3] = -z;
+    return t;
+}
+inline Transform Translate(const vec3f &v) { return Translate(v.x, v.y, v.z); }
+inline Transform Scale(Float x, Float y, Float z) {
+    Transform t;
+    t.m[0][0] = x; t.m[1][1] = y; t.m[2][2] = z;
+    t.mInv[0][0] = 1 / x; t.mInv[1][1] = 1 / y; t.mInv[2][2] = 1 / z;
+    return t;
+}
+inline Transform Scale(Float s) { return Scale(s, s, s); }
+namespace bubbles_detail {
+inline Transform rotation(int a, int b, Float degrees) {
+    const Float th = degrees * (3.14159265358979323846 / 180.0), c = std::cos(th), s = std::sin(th);
+    Transform t;
+    t.m[a][a] = c; t.m[a][b] = -s; t.m[b][a] = s; t.m[b][b] = c;
+    t.mInv[a][a] = c; t.mInv[a][b] = s; t.mInv[b][a] = -s; t.mInv[b][b] = c;
+    return t;
+}
+}  // namespace bubbles_detail
+inline Transform RotateX(Float deg) { return bubbles_detail::rotation(1, 2, deg); }
+inline Transform RotateY(Float deg) { return bubbles_detail::rotation(2, 0, deg); }
+inline Transform RotateZ(Float deg) { return bubbles_detail::rotation(0, 1, deg); }
 
+/* ParsedMesh (src/core/shape.h:12-22): vertices and triangle indices */
+struct ParsedMesh {
+    std::vector<vec3f> p;
+    std::vector<int> indices;  // 3 per triangle
+    int nTriangles = 0, nVertices = 0;
+    char name[256] = {0};
+};
+
+/* LoadObj (src/third/obj_loader.cpp:434-441): the first object's vertices
+ * ("v x y z") and faces ("f a b c ...", fan-triangulated, 1-based or
+ * negative indices, "a/b/c" tokens use a) of a Wavefront OBJ file. */
+inline ParsedMesh *LoadObj(const char *path) {
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) throw bubbles_error(std::string("LoadObj: cannot open ") + path);
+    ParsedMesh *mesh = new ParsedMesh();
+    char line[4096];
+    while (std::fgets(line, sizeof(line), fp)) {
+        const char *c = line;
+        while (*c == ' ' || *c == '\t') c++;
+        if (c[0] == 'v' && (c[1] == ' ' || c[1] == '\t')) {
+            double x = 0, y = 0, z = 0;
+            if (std::sscanf(c + 2, "%lf %lf %lf", &x, &y, &z) == 3) mesh->p.push_back(vec3f(x, y, z));
+        } else if (c[0] == 'f' && (c[1] == ' ' || c[1] == '\t')) {
+            std::vector<int> face;
+            const char *q = c + 2;
+            while (*q) {
+                while (*q == ' ' || *q == '\t') q++;
+                if (!*q || *q == '\n' || *q == '\r') break;
+                char *end = nullptr;
+                long v = std::strtol(q, &end, 10);
+                if (end == q) break;
+                const int nv = (int)mesh->p.size();
+                face.push_back(v < 0 ? nv + (int)v : (int)v - 1);
+                q = end;
+                while (*q && *q != ' ' && *q != '\t' && *q != '\n' && *q != '\r') q++;
+            }
+            for (size_t k = 2; k < face.size(); k++) {
+                mesh->indices.push_back(face[0]);
+                mesh->indices.push_back(face[k - 1]);
+                mesh->indices.push_back(face[k]);
+            }
+        }
+    }
+    std::fclose(fp);
+    mesh->nVertices = (int)mesh->p.size();
+    mesh->nTriangles = (int)mesh->indices.size() / 3;
+    const char *slash = std::strrchr(path, '/');
+    std::snprintf(mesh->name, sizeof(mesh->name), "%s", slash ? slash + 1 : path);
+    for (int i : mesh->indices)
+        if (i < 0 || i >= mesh->nVertices) {
+            delete mesh;
+            throw bubbles_error(std::string("LoadObj: face index out of range in ") + path);
+        }
+    return mesh;
+}
+
+enum ShapeType { ShapeBox = 0, ShapeSphere = 1, ShapeMesh = 2 };
+
+/* Shape (src/core/shape.h:125-271): box, sphere or mesh */
 struct Shape {
-    vec3f center, size;
+    ShapeType type = ShapeBox;
+    Transform ObjectToWorld, WorldToObject;
     bool reverseOrientation = false;
-    /* Shape::BoxGetBounds (src/shapes/box.cpp:59-64) */
+    Float sizex = 0, sizey = 0, sizez = 0, radius = 0;
+    vec3f linearVelocity, angularVelocity;
+    ParsedMesh *mesh = nullptr;
+    bbx_sdf *grid = nullptr;  // mesh SDF on the device
+    Bounds3f meshBounds;
+    /* GetBounds (src/core/shape.cpp:158-180): transformed local bounds */
     Bounds3f GetBounds() const {
-        vec3f h(size.x / 2.0, size.y / 2.0, size.z / 2.0);
-        return Bounds3f(vec3f(-h.x, -h.y, -h.z) + center, h + center);
+        if (type == ShapeMesh) return meshBounds;
+        const vec3f h = type == ShapeBox ? vec3f(sizex / 2.0, sizey / 2.0, sizez / 2.0) : vec3f(radius);
+        Bounds3f b(ObjectToWorld.Point(vec3f(-h.x, -h.y, -h.z)), ObjectToWorld.Point(vec3f(-h.x, -h.y, -h.z)));
+        for (int c = 0; c < 8; c++) {
+            const vec3f q = ObjectToWorld.Point(vec3f(c & 1 ? h.x : -h.x, c & 2 ? h.y : -h.y, c & 4 ? h.z : -h.z));
+            b = Bounds3f(vec3f(std::fmin(b.pMin.x, q.x), std::fmin(b.pMin.y, q.y), std::fmin(b.pMin.z, q.z)),
+                         vec3f(std::fmax(b.pMax.x, q.x), std::fmax(b.pMax.y, q.y), std::fmax(b.pMax.z, q.z)));
+        }
+        return b;
+    }
+    /* Update (src/core/shape.cpp:294-298): a new placement for the next step */
+    void Update(const Transform &toWorld) {
+        ObjectToWorld = toWorld;
+        WorldToObject = Inverse(toWorld);
+    }
+    void SetVelocities(const vec3f &vel, const vec3f &angular) {
+        linearVelocity = vel;
+        angularVelocity = angular;
     }
 };
 
 /* MakeBox (src/shapes/box.cpp:6-10) */
 inline Shape *MakeBox(const Transform &toWorld, const vec3f &size, bool reverseOrientation = false) {
     Shape *s = new Shape();
-    s->center = toWorld.t;
-    s->size = size;
+    s->type = ShapeBox;
+    s->Update(toWorld);
+    s->sizex = size.x; s->sizey = size.y; s->sizez = size.z;
     s->reverseOrientation = reverseOrientation;
     return s;
 }
 
+/* MakeSphere (src/shapes/sphere.cpp:5-9) */
+inline Shape *MakeSphere(const Transform &toWorld, Float radius, bool reverseOrientation = false) {
+    Shape *s = new Shape();
+    s->type = ShapeSphere;
+    s->Update(toWorld);
+    s->radius = radius;
+    s->reverseOrientation = reverseOrientation;
+    return s;
+}
+
+/* MakeMesh (src/shapes/bvh.cpp:51-61): the transform is applied to the
+ * vertices, the shape itself stays at the identity */
+inline Shape *MakeMesh(ParsedMesh *mesh, const Transform &toWorld, bool reverseOrientation = false) {
+    if (!mesh || mesh->nTriangles < 1) throw bubbles_error("MakeMesh: empty mesh");
+    for (auto &v : mesh->p) v = toWorld.Point(v);
+    Shape *s = new Shape();
+    s->type = ShapeMesh;
+    s->mesh = mesh;
+    s->reverseOrientation = reverseOrientation;
+    vec3f lo(1e300), hi(-1e300);
+    for (int i : mesh->indices) {
+        const vec3f &v = mesh->p[(size_t)i];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::fmin(lo[k], v[k]);
+            hi[k] = std::fmax(hi[k], v[k]);
+        }
+    }
+    s->meshBounds = Bounds3f(lo, hi);
+    return s;
+}
+inline Shape *MakeMesh(const char *path, const Transform &toWorld, bool reverseOrientation = false) {
+    return MakeMesh(LoadObj(path), toWorld, reverseOrientation);
+}
+
+/* GenerateShapeSDF (src/core/shape.cpp:478-505) for meshes, on the device */
+inline void GenerateShapeSDF(Shape *shape, Float dx = 0.01, Float margin = 0.1) {
+    if (!shape || shape->type != ShapeMesh || !shape->mesh)
+        throw bubbles_error("GenerateShapeSDF: only mesh shapes carry an SDF here");
+    std::vector<double> v(3 * shape->mesh->p.size());
+    for (size_t i = 0; i < shape->mesh->p.size(); i++)
+        for (int k = 0; k < 3; k++) v[3 * i + k] = shape->mesh->p[i][k];
+    if (shape->grid) bbx_sdf_destroy(shape->grid);
+    shape->grid = nullptr;
+    bbx_throw_if(bbx_sdf_from_mesh(v.data(), shape->mesh->nVertices, shape->mesh->indices.data(),
+                                   shape->mesh->nTriangles, dx, margin, &shape->grid),
+                 "GenerateShapeSDF");
+}
+
+/* Collider3 (src/core/collider.h:62-80) */
+struct Collider3 {
+    Shape *shape = nullptr;
+    Float frictionCoefficient = 0;
+    bool isActive = true;
+    void SetActive(bool active) { isActive = active; }
+    bool IsActive() const { return isActive; }
+};
+
+/* MakeCollider3 (src/core/collider.cpp:345-353): a mesh gets its SDF */
+inline Collider3 *MakeCollider3(Shape *shape, Float frictionCoef = 0) {
+    if (!shape) throw bubbles_error("MakeCollider3: null shape");
+    if (shape->type == ShapeMesh && !shape->grid) GenerateShapeSDF(shape);
+    Collider3 *c = new Collider3();
+    c->shape = shape;
+    c->frictionCoefficient = frictionCoef;
+    return c;
+}
+
 struct ColliderSet3 {
-    std::vector<Shape *> shapes;
+    std::vector<Collider3 *> colliders;
+    int nColiders = 0;
+    void SetActive(int which, bool active) {
+        if (which >= 0 && which < (int)colliders.size()) colliders[(size_t)which]->SetActive(active);
+    }
+    bool IsActive(int which) const {
+        return which >= 0 && which < (int)colliders.size() && colliders[(size_t)which]->IsActive();
+    }
 };
 
 struct ColliderSetBuilder3 {
-    std::vector<Shape *> shapes;
-    void AddCollider3(Shape *shape) { shapes.push_back(shape); }
+    std::vector<Collider3 *> colliders;
+    void AddCollider3(Collider3 *collider) { colliders.push_back(collider); }
+    void AddCollider3(Shape *shape, Float frictionCoef = 0) { colliders.push_back(MakeCollider3(shape, frictionCoef)); }
     ColliderSet3 *GetColliderSet() {
+        if (colliders.empty()) throw bubbles_error("No colliders given for {GetColliderSet}");
         ColliderSet3 *c = new ColliderSet3();
-        c->shapes = shapes;
+        c->colliders = colliders;
+        c->nColiders = (int)colliders.size();
         return c;
     }
 };
@@ -594,15 +823,56 @@ inline bbx_solver *make_device(int kind, SphSolverData3 *d, Float targetDensity,
     return s;
 }
 
-/* the container must be one reversed box whose bounds define the grid domain */
-inline void check_colliders(ColliderSet3 *c, Grid3 *g) {
-    if (!c) return;
-    if (c->shapes.size() != 1 || !c->shapes[0]->reverseOrientation)
-        throw bubbles_error("SetColliders: one reversed box container is supported");
-    const Shape *s = c->shapes[0];
-    for (int k = 0; k < 3; k++)
-        if (g && (s->center[k] != g->prm.tank_center[k] || s->size[k] != g->prm.tank_size[k]))
-            throw bubbles_error("SetColliders: container must be the grid's domain box");
+/* true when the set is the grid's domain as one reversed, frictionless box:
+ * the solver's built-in tank, which needs no collider list */
+inline bool is_plain_tank(const ColliderSet3 *c, const Grid3 *g) {
+    if (!c || !g || c->colliders.size() != 1) return false;
+    const Collider3 *k = c->colliders[0];
+    const Shape *s = k->shape;
+    if (!k->isActive || k->frictionCoefficient != 0 || !s || s->type != ShapeBox || !s->reverseOrientation ||
+        !s->ObjectToWorld.IsTranslation())
+        return false;
+    const Float size[3] = {s->sizex, s->sizey, s->sizez};
+    for (int i = 0; i < 3; i++)
+        if (s->ObjectToWorld.m[i][3] != g->prm.tank_center[i] || size[i] != g->prm.tank_size[i]) return false;
+    return true;
+}
+
+/* ColliderSet3 -> the device's collider list (ColliderSet3::ResolveCollision
+ * semantics, src/core/collider.cpp:238-264); called at Setup and before
+ * every step, so Shape::Update, SetVelocities and SetActive take effect */
+inline void push_colliders(bbx_solver *dev, const ColliderSet3 *c, const Grid3 *g, bool &listInUse) {
+    if (!c || is_plain_tank(c, g)) {
+        if (listInUse) bbx_throw_if(bbx_solver_set_colliders(dev, nullptr, 0), "SetColliders");
+        listInUse = false;
+        return;
+    }
+    std::vector<bbx_collider> list(c->colliders.size());
+    for (size_t i = 0; i < list.size(); i++) {
+        const Collider3 *k = c->colliders[i];
+        const Shape *s = k->shape;
+        bbx_collider &b = list[i];
+        b = bbx_collider();
+        b.kind = s->type == ShapeBox ? BBX_SHAPE_BOX : (s->type == ShapeSphere ? BBX_SHAPE_SPHERE : BBX_SHAPE_MESH);
+        b.reverse_orientation = s->reverseOrientation ? 1 : 0;
+        b.active = k->isActive ? 1 : 0;
+        b.friction = k->frictionCoefficient;
+        s->ObjectToWorld.Rows(b.to_world, b.to_object);
+        b.size[0] = s->sizex; b.size[1] = s->sizey; b.size[2] = s->sizez;
+        b.radius = s->radius;
+        for (int d = 0; d < 3; d++) {
+            b.linear_velocity[d] = s->linearVelocity[d];
+            b.angular_velocity[d] = s->angularVelocity[d];
+            b.bounds_min[d] = s->meshBounds.pMin[d];
+            b.bounds_max[d] = s->meshBounds.pMax[d];
+        }
+        if (s->type == ShapeMesh) {
+            if (!s->grid) throw bubbles_error("SetColliders: mesh collider without an SDF");
+            b.sdf = s->grid;
+        }
+    }
+    bbx_throw_if(bbx_solver_set_colliders(dev, list.data(), (int)list.size()), "SetColliders");
+    listInUse = true;
 }
 }  // namespace bubbles_detail
 
@@ -623,9 +893,9 @@ class PciSphSolver3 {
     void Setup(Float targetDensity, Float targetSpacing, Float relativeRadius, Grid3 *domain,
                SphParticleSet3 *pSet) {
         if (!data_) throw bubbles_error("Invalid call to {PciSphSolver3::Setup}");
-        pending_collider_check();
         dev_ = bubbles_detail::make_device(BBX_PCISPH, data_, targetDensity, targetSpacing, relativeRadius,
                                            domain, pSet);
+        colliders();
         bbx_info info;
         bbx_throw_if(bbx_solver_info(dev_, &info), "Setup");
         deltaDenom = info.delta_denom;
@@ -635,7 +905,7 @@ class PciSphSolver3 {
     void SetColliders(ColliderSet3 *colliders) {
         if (!data_) throw bubbles_error("Invalid solverData for {SetColliders}");
         data_->collider = colliders;
-        pending_collider_check();
+        if (dev_) this->colliders();
     }
     ColliderSet3 *GetColliders() { return data_ ? data_->collider : nullptr; }
     void SetViscosityCoefficient(Float v) {
@@ -649,6 +919,7 @@ class PciSphSolver3 {
     /* PciSphSolver3::Advance (src/solvers/pcisph_solver3.cpp:67-89) */
     void Advance(Float timeIntervalInSeconds) {
         require();
+        colliders();
         int n = 0;
         bbx_throw_if(bbx_pcisph_advance(dev_, timeIntervalInSeconds, maxIterations, maxErrorDensity,
                                         loopMode, warmStart, &n),
@@ -659,6 +930,7 @@ class PciSphSolver3 {
     /* one fixed-dt substep (AdvanceTimeStep); returns its statistics */
     bbx_stats AdvanceSubstep(Float dt) {
         require();
+        colliders();
         bbx_stats st;
         bbx_throw_if(bbx_pcisph_substep(dev_, dt, maxIterations, maxErrorDensity, loopMode, warmStart, &st),
                      "AdvanceSubstep");
@@ -680,12 +952,11 @@ class PciSphSolver3 {
   private:
     SphSolverData3 *data_ = nullptr;
     bbx_solver *dev_ = nullptr;
+    bool listInUse_ = false;
     void require() {
         if (!dev_) throw bubbles_error("PciSphSolver3: call Setup first");
     }
-    void pending_collider_check() {
-        if (data_ && data_->collider) bubbles_detail::check_colliders(data_->collider, data_->domain);
-    }
+    void colliders() { bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_); }
 };
 
 /* SphSolver3 (src/core/sph_solver.h:76-96), race-free two-phase integration */
@@ -699,14 +970,14 @@ class SphSolver3 {
     void Setup(Float targetDensity, Float targetSpacing, Float relativeRadius, Grid3 *domain,
                SphParticleSet3 *pSet) {
         if (!data_) throw bubbles_error("Invalid SphSolver3 initialization");
-        if (data_->collider) bubbles_detail::check_colliders(data_->collider, domain);
         dev_ = bubbles_detail::make_device(BBX_SPH, data_, targetDensity, targetSpacing, relativeRadius,
                                            domain, pSet);
+        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
     }
     void SetColliders(ColliderSet3 *c) {
         if (!data_) throw bubbles_error("Invalid solverData for {SetColliders}");
         data_->collider = c;
-        if (data_->domain) bubbles_detail::check_colliders(c, data_->domain);
+        if (dev_) bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
     }
     ColliderSet3 *GetColliders() { return data_ ? data_->collider : nullptr; }
     SphSolverData3 *GetSphSolverData() { return data_; }
@@ -722,6 +993,7 @@ class SphSolver3 {
     SphParticleSet3 *GetSphParticleSet() { return data_ ? data_->sphpSet : nullptr; }
     void Advance(Float timeIntervalInSeconds) {
         if (!dev_) throw bubbles_error("SphSolver3: call Setup first");
+        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
         int n = 0;
         bbx_throw_if(bbx_sph_advance(dev_, timeIntervalInSeconds, &n), "Advance");
         lastSubsteps = n;
@@ -733,6 +1005,7 @@ class SphSolver3 {
   private:
     SphSolverData3 *data_ = nullptr;
     bbx_solver *dev_ = nullptr;
+    bool listInUse_ = false;
 };
 
 /* ---- serializer (src/third/serializer.h, src/third/serializer.cpp) ---- */

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from bubbles_amd import build as hipbuild
-from bubbles_amd import scenes
+from bubbles_amd import scenes, shapes
 
 
 def facade_scene():
@@ -141,3 +141,53 @@ def test_facade_continuous_emission_matches_abi(bbx):
     # the scene exercises emission and the capacity limit
     assert counts[-1] == cap and len(set(counts)) >= 3, counts
     np.testing.assert_array_equal(fpos, g.get("position"))
+
+
+def _write_cube_obj(path):
+    """The obstacle mesh of tests/cpp/facade_obstacles.cpp as an OBJ file
+    (shortest round-trip decimals, so both sides read the same doubles)."""
+    v, t = shapes.cube_mesh(0.06)
+    v = v + np.array([0.06, -0.16, 0.06])
+    with open(path, "w") as f:
+        f.write("# cube obstacle\n")
+        for p in v:
+            f.write("v %r %r %r\n" % (float(p[0]), float(p[1]), float(p[2])))
+        for a, b, c in t + 1:
+            f.write("f %d %d %d\n" % (a, b, c))
+    return v, t
+
+
+@pytest.mark.gpu
+def test_facade_obstacles_match_abi(bbx, tmp_path):
+    obj, out_path = tmp_path / "block.obj", tmp_path / "pos.bin"
+    v, t = _write_cube_obj(obj)
+    frames = 12
+    res = subprocess.run([hipbuild.OBSTACLES_BIN, str(frames), str(obj), str(out_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    fpos = np.fromfile(out_path, dtype=np.float64).reshape(-1, 3)
+    assert out["mesh_triangles"] == 12
+    np.testing.assert_array_equal(out["mesh_bounds"], list(v.min(axis=0)) + list(v.max(axis=0)))
+    s = 0.01
+    pos = np.array([(x * s + -0.08, y * s + -0.02, z * s + -0.08)
+                    for z in range(16) for y in range(12) for x in range(16)])
+    sc = dataclasses.replace(scenes.dam_break(16, 12, 16, spacing=0.01, tank=(0.5, 0.4, 0.5), jitter=0.0),
+                             positions=pos, velocities=np.tile([0.0, -1.5, 0.0], (len(pos), 1)))
+    assert out["n"] == sc.n
+    ball = shapes.sphere(shapes.translate(0.0, -0.12, 0.0), 0.05)
+    block = shapes.mesh(v, bbx.Sdf.from_mesh(v, t, dx=0.01, margin=0.1))  # GenerateShapeSDF defaults
+    tank = shapes.box(shapes.identity(), (0.5, 0.4, 0.5), reverse_orientation=True)
+    g = bbx.Solver(sc)
+    subs = 0
+    for f in range(frames):
+        paddle = shapes.box(shapes.translate(-0.06, -0.15, -0.06) @ shapes.rotate_z(25 + 5 * f), (0.1, 0.03, 0.1),
+                            friction=0.1)
+        ball.active = f < 8
+        g.set_colliders([ball, paddle, block, tank])
+        subs += g.pcisph_advance(1.0 / 240.0, 8, 0.01, bbx.MODE_TEXTBOOK)
+    assert out["substeps"] == subs
+    # the facade composes and inverts transforms in its own order: the FP64
+    # matrices may differ in the last bit before they are rounded to FP32
+    np.testing.assert_allclose(fpos, g.get("position"), atol=1e-6)
+    assert out["min_y"] < -0.09  # the column went down among the obstacles
