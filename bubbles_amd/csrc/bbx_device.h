@@ -86,6 +86,11 @@ __device__ __forceinline__ float poly6_w(const SimConst &c, float d2) {
 // spiky cutoff (src/core/kernel.cpp:181-226): zero when |r-h| < 1e-8 or r > h
 __device__ __forceinline__ bool spiky_in(const SimConst &c, float r) { return cutoff_in(r - c.h); }
 
+// v if keep, else +0 (also for a NaN or infinite v), without a branch
+__device__ __forceinline__ float keep_if(bool keep, float v) {
+    return __uint_as_float(__float_as_uint(v) & (0u - (unsigned)keep));
+}
+
 // Hardware reciprocal, square root and reciprocal square root (about 1 ulp,
 // one instruction each) for the gather sums; the FP64 bucket rule and the
 // hash never use them.

@@ -181,28 +181,30 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
                 }
                 const float dx = xi.x - xjx, dy = xi.y - xjy, dz = xi.z - xjz;
                 const float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
-                const bool in = live && spiky_in(c, r);
-                const float x = in ? 1.f - r * c.inv_h : 0.f;
+                // masks instead of branches: every pair's arithmetic runs, a
+                // dead pair contributes exact zeros, and the neighbour's LDS
+                // reads stay unconditional (so they are issued early)
+                const bool in = live & spiky_in(c, r);
+                const float x = keep_if(in, 1.f - r * c.inv_h);
                 const float irj = fast_rcp(rj);
-                // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j; dead entries add exact zeros
+                // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j
                 const float s = coef * (c.spiky_lap * x) * irj;
                 const float dvx = vjx - vi.x, dvy = vjy - vi.y, dvz = vjz - vi.z;
                 fx += dvx * s;
                 fy += dvy * s;
                 fz += dvz * s;
-                if (kFeat && live && j != self) {
-                    const float u = r * c.inv_h;
+                if constexpr (kFeat) {
+                    const bool fl = live & (j != self);
+                    const float u = keep_if(fl, r * c.inv_h);
                     m1 += u;
                     m2 += u * u;
                     m3 += u * u * u;
-                    cx -= dx * c.inv_h;
-                    cy -= dy * c.inv_h;
-                    cz -= dz * c.inv_h;
-                    if (r > 0.f && x > 0.f) {
-                        // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
-                        const float g = -c.spiky_grad * x * x * fast_rcp(r);
-                        divv += c.mass * irj * g * (dvx * dx + dvy * dy + dvz * dz);
-                    }
+                    cx -= keep_if(fl, dx * c.inv_h);
+                    cy -= keep_if(fl, dy * c.inv_h);
+                    cz -= keep_if(fl, dz * c.inv_h);
+                    // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
+                    const float g = -c.spiky_grad * x * x * fast_rcp(r);
+                    divv += keep_if(fl & (r > 0.f) & (x > 0.f), c.mass * irj * g * (dvx * dx + dvy * dy + dvz * dz));
                 }
             };
             // quads half, half + 2, half + 4, ... of the bucket, two in flight
@@ -513,8 +515,12 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                 const float r = d2 * ir;
                 const bool in = live && spiky_in(c, r) && r >= 1e-8f;
                 const float x = 1.f - r * c.inv_h;
-                // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r
-                const float g = in ? kg * (qi + qj) * x * x * ir : 0.f;  // dead: exact zero
+                // gradW(r, dir) = -dW(r) dir = spiky_grad x^2 (x_j - x_i)/r;
+                // a dead pair adds an exact zero.  Masking the bits (rather
+                // than ?:) keeps the pair's arithmetic unconditional, so the
+                // compiler issues the neighbour's LDS read as one 16-byte
+                // load and does not branch per pair.
+                const float g = keep_if(in, kg * (qi + qj) * x * x * ir);
                 kahan_add(tx, ex, g * dx);
                 kahan_add(ty, ey, g * dy);
                 kahan_add(tz, ez, g * dz);
