@@ -589,7 +589,8 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     float tx = 0.f, ty = 0.f, tz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
     const int cnt = nb.cnt[i];
     for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j, bool live) {
-        if (!live || j == self) return;
+        // masks, not early returns: the pair's loads and arithmetic are
+        // unconditional and a skipped pair adds exact zeros
         float4 xj, vj;
         if (in_lds) {
             xj = sx[j];
@@ -599,17 +600,16 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
             xj = xq(g);
             vj = vr(g);
         }
-        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (!spiky_in(c, r)) return;
-        float x = 1.f - r * c.inv_h;
-        if (r > 0.f && !(fabsf(r) < 1e-8f)) {
-            float g = c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r;
-            kahan_add(tx, ex, g * dx);
-            kahan_add(ty, ey, g * dy);
-            kahan_add(tz, ez, g * dz);
-        }
-        float s = coef * c.spiky_lap * x / vj.w;
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        const bool in = live & (j != self) & spiky_in(c, r);
+        const float x = 1.f - r * c.inv_h;
+        const bool pg = in & (r > 0.f) & !(fabsf(r) < 1e-8f);
+        const float g = keep_if(pg, c.mass2 * (qi + xj.w) * c.spiky_grad * x * x / r);
+        kahan_add(tx, ex, g * dx);
+        kahan_add(ty, ey, g * dy);
+        kahan_add(tz, ez, g * dz);
+        const float s = keep_if(in, coef * c.spiky_lap * x / vj.w);
         fx += (vj.x - vi.x) * s;
         fy += (vj.y - vi.y) * s;
         fz += (vj.z - vi.z) * s;
@@ -670,7 +670,6 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     const float spiky_w = c.spiky_lap * c.h * c.h / 6.f;  // 15 / (pi h^3)
     const int cnt = nb.cnt[i];
     for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j, bool live) {
-        if (!live) return;
         float4 xj, vj;
         if (in_lds) {
             xj = sx[j];
@@ -680,14 +679,11 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
             xj = pos[g];
             vj = vr(g);
         }
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        float w = 0.f;
-        if (spiky_in(c, r)) {
-            float x = 1.f - r * c.inv_h;
-            w = spiky_w * x * x * x;
-        }
-        float wj = c.mass / vj.w * w;
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float x = 1.f - r * c.inv_h;
+        const float w = keep_if(live & spiky_in(c, r), spiky_w * x * x * x);
+        const float wj = keep_if(live, c.mass / vj.w * w);
         ws += wj;
         sx_ += wj * vj.x;
         sy_ += wj * vj.y;

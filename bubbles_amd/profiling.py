@@ -49,7 +49,27 @@ def short_name(kernel: str) -> str:
     return SHORT.get(base, base)
 
 
-def stats_markdown(path: str, command: str) -> str:
+def loop_kernel_split(trace_csv: str, threshold_us: float = 15.0):
+    """Per loop kernel: (working launches, their mean us, early exits, their
+    mean us) from a kernel trace.  A launch that finds an earlier iteration
+    integrated exits at entry in a few microseconds; a working launch over
+    the 1M benchmark takes tens."""
+    dur = defaultdict(list)
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            name = short_name(r["Kernel_Name"])
+            if name in ("predict_pressure", "pressure_force"):
+                dur[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = {}
+    for name, d in dur.items():
+        work = [x for x in d if x >= threshold_us]
+        skip = [x for x in d if x < threshold_us]
+        out[name] = (len(work), statistics.mean(work) if work else 0.0, len(skip),
+                     statistics.mean(skip) if skip else 0.0)
+    return out
+
+
+def stats_markdown(path: str, command: str, trace_csv: str = None) -> str:
     rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
@@ -64,6 +84,11 @@ def stats_markdown(path: str, command: str) -> str:
     lines.append("Loop kernels (`predict_pressure`, `pressure_force`) average over all of "
                  "their launches, including those that exit at entry once an earlier "
                  "iteration has integrated.")
+    if trace_csv:
+        lines += ["", "The same launches split by duration (from the kernel trace, threshold 15 us):", "",
+                  "| kernel | working launches | avg (us) | early exits | avg (us) |", "|---|---|---|---|---|"]
+        for name, (nw, aw, ns, as_) in sorted(loop_kernel_split(trace_csv).items()):
+            lines.append(f"| {name} | {nw} | {aw:.2f} | {ns} | {as_:.2f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -182,6 +207,7 @@ def main(argv=None):
     st.add_argument("csv")
     st.add_argument("--command", required=True)
     st.add_argument("--out", required=True)
+    st.add_argument("--trace", default=None, help="kernel trace CSV: split loop-kernel launches")
     tr = sub.add_parser("traffic")
     tr.add_argument("fetch_csv")
     tr.add_argument("write_csv")
@@ -199,7 +225,7 @@ def main(argv=None):
             f.write(sq_markdown(sq_summary(args.csv), args.command))
     elif args.cmd == "stats":
         with open(args.out, "w") as f:
-            f.write(stats_markdown(args.csv, args.command))
+            f.write(stats_markdown(args.csv, args.command, args.trace))
     else:
         rec = traffic(args.fetch_csv, args.write_csv, args.n)
         rec["command"] = args.command

@@ -729,6 +729,52 @@ class ContinuousParticleSetBuilder3 {
         }
     }
 
+    /* MapGridEmitToOther (src/core/grid.h:1323-1364): this builder's mapped
+     * positions refill the same cells of the other builder's grid.  As in
+     * the reference, the other grid's chain members are looked up by id in
+     * this builder's particle set. */
+    void MapGridEmitToOther(ContinuousParticleSetBuilder3 *other,
+                            const std::function<vec3f(const vec3f &)> &velocity, Float d = 0.02) {
+        if (reemitions > 0 && reemitOnce) return;
+        if (!other || !other->mappedDomain) throw bubbles_error("MapGridEmitToOther: the other builder has no mapped grid");
+        std::vector<int> start, members;
+        std::vector<double> otherPos, pos;
+        bubbles_detail::chain_snapshot(other->particleSet, other->mappedDomain, start, members, otherPos);
+        const int n = particleSet->GetParticleCount();
+        if (particleSet->dev) {
+            pos.assign(3 * (size_t)n, 0.0);
+            if (n) bbx_throw_if(bbx_solver_download(particleSet->dev, BBX_FIELD_POSITION, pos.data()), "MapGridEmitToOther");
+        } else {
+            pos = particleSet->pos;
+        }
+        int numNewParticles = 0;
+        for (unsigned h : mappedCellSet) {
+            const std::vector<vec3f> &cellPos = mappedPositions[h];
+            const int size = start[h + 1] - start[h];
+            if (size >= BBX_BUCKET_SIZE) continue;
+            const int toInsert = std::min(BBX_BUCKET_SIZE - size, (int)cellPos.size());
+            for (int i = 0; i < toInsert; i++) {
+                const vec3f pi = cellPos[(size_t)i];
+                bool canAdd = true;
+                for (int k = start[h]; k < start[h + 1] && canAdd; k++) {
+                    const int id = members[(size_t)k];
+                    if (id >= n) throw bubbles_error("MapGridEmitToOther: chain id beyond this builder's set");
+                    const size_t j = (size_t)id;
+                    canAdd = (vec3f(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]) - pi).Length() >= d;
+                }
+                if (!canAdd) continue;
+                if (other->AddParticle(pi, velocity(pi))) {
+                    numNewParticles++;
+                } else {
+                    other->Commit();
+                    return;
+                }
+            }
+        }
+        if (numNewParticles > 0) other->Commit();
+        reemitions += 1;
+    }
+
     void MapGridEmit(const std::function<vec3f(const vec3f &)> &velocity, Float d = 0.02) {
         if (reemitions > 0 && reemitOnce) return;
         if (warned) return;

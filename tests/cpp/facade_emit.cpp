@@ -50,7 +50,12 @@ int main(int argc, char **argv) {
         std::vector<int> counts;
         for (int f = 0; f < frames; f++) {
             solver.Advance(1.0 / 240.0);
-            pBuilder.MapGridEmit(velocityField, spacing);  // may print the capacity warning
+            // odd frames through MapGridEmitToOther with the builder as its own
+            // target (the same refill, without the early exit once warned)
+            if (f % 2 == 0)
+                pBuilder.MapGridEmit(velocityField, spacing);  // may print the capacity warning
+            else
+                pBuilder.MapGridEmitToOther(&pBuilder, velocityField, spacing);
             counts.push_back(pSet->GetParticleCount());
         }
         std::printf("\n{\"mapped\": %zu, \"counts\": [", pBuilder.mappedCellSet.size());
