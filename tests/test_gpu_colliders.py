@@ -151,3 +151,56 @@ def test_pcisph_with_obstacles_matches_oracle(bbx, oracle):
         o.set("velocity", gv)
         deepest = min(deepest, float(gp[:, 1].min()))
     assert deepest < -0.12  # the column went down among the obstacles
+
+
+def test_sph_with_obstacles_matches_oracle(bbx, oracle):
+    """WCSPH (SphSolver3) with a sphere, a tilted box and the tank: the same
+    collider rule in the integration pass."""
+    sc = scenes.sph_dam_break()
+    keep = sc.positions[:, 1] > -0.4  # the column's upper part, falling onto the obstacles
+    sc = dataclasses.replace(sc, positions=sc.positions[keep].copy(), velocities=sc.velocities[keep].copy())
+    sc = dataclasses.replace(sc, positions=sc.positions[:4096].copy(), velocities=np.tile([0.0, -2.0, 0.0], (4096, 1)))
+    ball = shapes.sphere(shapes.translate(-0.3, -0.42, -0.3), 0.06)
+    slab = shapes.box(shapes.translate(-0.1, -0.44, -0.1) @ shapes.rotate_x(15), (0.2, 0.04, 0.2), friction=0.2)
+    tank = shapes.box(shapes.translate(*sc.tank_center), sc.tank_size, reverse_orientation=True)
+    g = bbx.Solver(sc, kind=bbx.SPH)
+    g.set_colliders([ball, slab, tank])
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")))
+    o.set_colliders([ball, slab, tank])
+    dt = 1.6e-4
+    for k in range(40):
+        g.sph_substep(dt)
+        o.sph_substep(dt, oracle.SPH_TWO_PHASE)
+        gp, op = g.get("position"), o.get("position")
+        perr = np.abs(gp - op).max(axis=1)
+        assert (perr > 5e-5).sum() <= 2 and perr.max() < 0.02, (k, np.sort(perr)[-3:])
+        o.set("position", gp)
+        o.set("velocity", g.get("velocity"))
+    assert g.get("position")[:, 1].min() < -0.4
+
+
+def test_append_into_a_scene_with_obstacles(bbx, oracle):
+    """A growing set among obstacles: appended particles fall onto a sphere."""
+    sc = _obstacle_scene()
+    ball = shapes.sphere(shapes.translate(0.0, -0.13, 0.0), 0.05)
+    tank = shapes.box(shapes.translate(*sc.tank_center), sc.tank_size, reverse_orientation=True)
+    g = bbx.Solver(sc)
+    g.set_colliders([ball, tank])
+    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")), threads=16)
+    o.set_colliders([ball, tank])
+    rng = np.random.default_rng(9)
+    for k in range(30):
+        if k % 10 == 0:
+            p = (rng.uniform([-0.05, 0.1, -0.05], [0.05, 0.15, 0.05], size=(200, 3))
+                 .astype(np.float32).astype(np.float64))
+            v = np.tile([0.0, -2.0, 0.0], (200, 1))
+            g.append(p, v)
+            o.append(p, v)
+        g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT)
+        o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT)
+        gp, op = g.get("position"), o.get("position")
+        perr = np.abs(gp - op).max(axis=1)
+        assert (perr > 5e-5).sum() <= 2 and perr.max() < 0.02, (k, np.sort(perr)[-3:])
+        o.set("position", gp)
+        o.set("velocity", g.get("velocity"))
+    assert g.n == sc.n + 600
