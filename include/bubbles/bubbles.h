@@ -887,10 +887,12 @@ inline bool is_plain_tank(const ColliderSet3 *c, const Grid3 *g) {
 /* ColliderSet3 -> the device's collider list (ColliderSet3::ResolveCollision
  * semantics, src/core/collider.cpp:238-264); called at Setup and before
  * every step, so Shape::Update, SetVelocities and SetActive take effect */
-inline void push_colliders(bbx_solver *dev, const ColliderSet3 *c, const Grid3 *g, bool &listInUse) {
+inline void push_colliders(bbx_solver *dev, const ColliderSet3 *c, const Grid3 *g, bool &listInUse,
+                           std::vector<bbx_collider> &last) {
     if (!c || is_plain_tank(c, g)) {
         if (listInUse) bbx_throw_if(bbx_solver_set_colliders(dev, nullptr, 0), "SetColliders");
         listInUse = false;
+        last.clear();
         return;
     }
     std::vector<bbx_collider> list(c->colliders.size());
@@ -898,7 +900,7 @@ inline void push_colliders(bbx_solver *dev, const ColliderSet3 *c, const Grid3 *
         const Collider3 *k = c->colliders[i];
         const Shape *s = k->shape;
         bbx_collider &b = list[i];
-        b = bbx_collider();
+        std::memset(&b, 0, sizeof(b));  // padding included, for the comparison below
         b.kind = s->type == ShapeBox ? BBX_SHAPE_BOX : (s->type == ShapeSphere ? BBX_SHAPE_SPHERE : BBX_SHAPE_MESH);
         b.reverse_orientation = s->reverseOrientation ? 1 : 0;
         b.active = k->isActive ? 1 : 0;
@@ -917,8 +919,14 @@ inline void push_colliders(bbx_solver *dev, const ColliderSet3 *c, const Grid3 *
             b.sdf = s->grid;
         }
     }
+    // unchanged since the last upload: nothing to do (the upload waits for
+    // the solver's stream)
+    if (listInUse && list.size() == last.size() &&
+        std::memcmp(list.data(), last.data(), sizeof(bbx_collider) * list.size()) == 0)
+        return;
     bbx_throw_if(bbx_solver_set_colliders(dev, list.data(), (int)list.size()), "SetColliders");
     listInUse = true;
+    last = list;
 }
 }  // namespace bubbles_detail
 
@@ -1002,7 +1010,8 @@ class PciSphSolver3 {
     void require() {
         if (!dev_) throw bubbles_error("PciSphSolver3: call Setup first");
     }
-    void colliders() { bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_); }
+    std::vector<bbx_collider> lastList_;
+    void colliders() { bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_, lastList_); }
 };
 
 /* SphSolver3 (src/core/sph_solver.h:76-96), race-free two-phase integration */
@@ -1018,12 +1027,12 @@ class SphSolver3 {
         if (!data_) throw bubbles_error("Invalid SphSolver3 initialization");
         dev_ = bubbles_detail::make_device(BBX_SPH, data_, targetDensity, targetSpacing, relativeRadius,
                                            domain, pSet);
-        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
+        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_, lastList_);
     }
     void SetColliders(ColliderSet3 *c) {
         if (!data_) throw bubbles_error("Invalid solverData for {SetColliders}");
         data_->collider = c;
-        if (dev_) bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
+        if (dev_) bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_, lastList_);
     }
     ColliderSet3 *GetColliders() { return data_ ? data_->collider : nullptr; }
     SphSolverData3 *GetSphSolverData() { return data_; }
@@ -1039,7 +1048,7 @@ class SphSolver3 {
     SphParticleSet3 *GetSphParticleSet() { return data_ ? data_->sphpSet : nullptr; }
     void Advance(Float timeIntervalInSeconds) {
         if (!dev_) throw bubbles_error("SphSolver3: call Setup first");
-        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_);
+        bubbles_detail::push_colliders(dev_, data_->collider, data_->domain, listInUse_, lastList_);
         int n = 0;
         bbx_throw_if(bbx_sph_advance(dev_, timeIntervalInSeconds, &n), "Advance");
         lastSubsteps = n;
@@ -1052,6 +1061,7 @@ class SphSolver3 {
     SphSolverData3 *data_ = nullptr;
     bbx_solver *dev_ = nullptr;
     bool listInUse_ = false;
+    std::vector<bbx_collider> lastList_;
 };
 
 /* ---- serializer (src/third/serializer.h, src/third/serializer.cpp) ---- */
