@@ -7,9 +7,11 @@ BBX_LONG_PRECISION=1, and BBX_PRECISION_OUT=<file> to write the tables as
 JSON.  Cases:
 
 * plain REFERENCE_EXACT (one iteration per substep, so both trajectories
-  differ by arithmetic alone): 262,144 particles over 1000 substeps and
-  4,194,304 (256x64x256, which needs a 4x1x4 tank) over 20;
-* learned warm start, REFERENCE_EXACT, 262,144 over 200 substeps: every
+  differ by arithmetic alone): 262,144, 1,048,576 and 4,194,304 particles
+  (256x64x256, which needs a 4x1x4 tank) over 1000 substeps, and the 4M
+  case over 20;
+* learned warm start, REFERENCE_EXACT, 262,144 over 200 substeps and
+  1,048,576 over 1000: every
   substep the oracle starts from the GPU's own warm pressures (the network is
   not part of the oracle), so the comparison isolates the FP32 solver;
 * TEXTBOOK iteration drift, 262,144 over 200 substeps: how often the FP32 and
@@ -24,6 +26,14 @@ force relative to max |f_p|, position in units of h; max / mean):
   100       1.5e-3 / 8.6e-6   3.4e-3 / 3.0e-5   1.5e-3 / 1.6e-5
   300       1.6e-2 / 5.3e-6   6.5e-2 / 4.9e-5   1.7e-2 / 2.3e-5
   1000      1.6e-1 / 1.0e-4   6.3e-1 / 5.4e-4   5.3e-1 / 7.6e-4
+
+After 1000 substeps (max / mean):
+
+  N           density           f_p               position
+  262,144     1.6e-1 / 1.0e-4   6.3e-1 / 5.4e-4   5.3e-1 / 7.6e-4 h
+  1,048,576   1.4e-1 / 6.9e-5   1.3e0  / 3.1e-4   7.0e-1 / 5.2e-4 h
+  4,194,304   8.9e-2 / 2.6e-5   5.0e-1 / 1.0e-4   4.5e-1 / 2.1e-4 h
+  1M learned  3.3e0  / 1.2e-4   1.3e0  / 4.2e-4   1.4e0  / 8.8e-4 h
 
 4,194,304 particles, 20 substeps: density 3.4e-4 / 9.7e-6, f_p 7.4e-3 /
 5.9e-5, position 3.0e-4 / 9.7e-6 h (the 1M case of the short study reads
@@ -83,14 +93,23 @@ def _pair(bbx, oracle, dims, tank=(2.0, 1.0, 2.0)):
     return sc, g, o
 
 
+_EVERY_100 = (1, 10) + tuple(range(100, 1001, 100))
+
+
 @pytest.mark.timeout(1800)
 @pytest.mark.parametrize("dims,tank,steps,checkpoints,bound", [
-    ((64, 64, 64), (2.0, 1.0, 2.0), 1000, (1, 10, 100, 300, 1000),
-     {"density_rel_max": 0.5, "density_rel_mean": 3e-4, "fp_rel_max": 2.0, "fp_rel_mean": 2e-3,
-      "pos_over_h_max": 1.5, "pos_over_h_mean": 3e-3}),
-    ((256, 64, 256), (4.0, 1.0, 4.0), 20, (1, 10, 20),
-     {"density_rel_max": 1e-3, "density_rel_mean": 3e-5, "fp_rel_max": 3e-2, "fp_rel_mean": 3e-4,
-      "pos_over_h_max": 1e-3, "pos_over_h_mean": 3e-5}),
+    pytest.param((64, 64, 64), (2.0, 1.0, 2.0), 1000, (1, 10, 100, 300, 1000),
+                 {"density_rel_max": 0.5, "density_rel_mean": 3e-4, "fp_rel_max": 2.0, "fp_rel_mean": 2e-3,
+                  "pos_over_h_max": 1.5, "pos_over_h_mean": 3e-3}, id="262k-1000"),
+    pytest.param((256, 64, 256), (4.0, 1.0, 4.0), 20, (1, 10, 20),
+                 {"density_rel_max": 1e-3, "density_rel_mean": 3e-5, "fp_rel_max": 3e-2, "fp_rel_mean": 3e-4,
+                  "pos_over_h_max": 1e-3, "pos_over_h_mean": 3e-5}, id="4M-20"),
+    pytest.param((128, 64, 128), (2.0, 1.0, 2.0), 1000, _EVERY_100,
+                 {"density_rel_max": 1.0, "density_rel_mean": 2e-4, "fp_rel_max": 4.0, "fp_rel_mean": 1e-3,
+                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1.5e-3}, id="1M-1000"),
+    pytest.param((256, 64, 256), (4.0, 1.0, 4.0), 1000, _EVERY_100,
+                 {"density_rel_max": 1.0, "density_rel_mean": 1e-4, "fp_rel_max": 4.0, "fp_rel_mean": 5e-4,
+                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1e-3}, id="4M-1000"),
 ])
 def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
     sc, g, o = _pair(bbx, oracle, dims, tank)
@@ -101,7 +120,7 @@ def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
         if k in checkpoints:
             table[k] = _errors(g, o, sc.kernel_radius)
             print(sc.n, k, table[k], flush=True)
-    _record(f"plain_exact_{sc.n}", table)
+    _record(f"plain_exact_{sc.n}_{steps}", table)
     last = table[steps]
     for k, b in bound.items():
         assert last[k] <= b, (k, last)
@@ -113,23 +132,33 @@ def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
 
 
 @pytest.mark.timeout(1800)
-def test_learned_exact_against_fp64(bbx, oracle):
+@pytest.mark.parametrize("dims,steps", [pytest.param((64, 64, 64), 200, id="262k-200"),
+                                        pytest.param((128, 64, 128), 1000, id="1M-1000")])
+def test_learned_exact_against_fp64(bbx, oracle, dims, steps):
     from bubbles_amd import learned
 
-    sc, g, o = _pair(bbx, oracle, (64, 64, 64))
+    sc, g, o = _pair(bbx, oracle, dims)
     g.attach_mlp(learned.load())
-    table, steps = {}, 200
+    table = {}
+    checkpoints = (1, 10, 100, 200) if steps == 200 else _EVERY_100
     for k in range(1, steps + 1):
         g.pcisph_substep(sc.dt, 5, 0.01, bbx.MODE_REFERENCE_EXACT, bbx.WARM_MLP)
         o.pcisph_substep(sc.dt, 5, 0.01, oracle.MODE_REFERENCE_EXACT,
                          warm=g.get("warm_pressure").astype(np.float64))
-        if k in (1, 10, 100, 200):
+        if k in checkpoints:
             table[k] = _errors(g, o, sc.kernel_radius)
             print(sc.n, k, table[k], flush=True)
-    _record(f"learned_exact_{sc.n}", table)
+    _record(f"learned_exact_{sc.n}_{steps}", table)
     last = table[steps]
-    assert last["density_rel_max"] <= 2e-2 and last["density_rel_mean"] <= 2e-5, last
-    assert last["pos_over_h_max"] <= 2e-2 and last["pos_over_h_mean"] <= 5e-5, last
+    if steps == 200:
+        assert last["density_rel_max"] <= 2e-2 and last["density_rel_mean"] <= 2e-5, last
+        assert last["pos_over_h_max"] <= 2e-2 and last["pos_over_h_mean"] <= 5e-5, last
+    else:
+        # the maximum is a splash particle: alone in one run and among
+        # neighbours in the other, its density differs by a multiple of the
+        # oracle's (3.3 measured); the means are what the horizon bounds
+        assert last["density_rel_max"] <= 10.0 and last["density_rel_mean"] <= 4e-4, last
+        assert last["pos_over_h_max"] <= 4.5 and last["pos_over_h_mean"] <= 3e-3, last
     assert max(table[1]["density_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
 
 
