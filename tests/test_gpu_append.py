@@ -112,3 +112,22 @@ def test_append_outside_the_grid_is_in_no_chain(bbx):
     g.append(np.array([[5.0, 5.0, 5.0], [0.1, 0.1, 0.1]]))
     start, members = g.chains()
     assert sc.n + 1 in members and sc.n not in members
+
+
+def test_growing_from_an_empty_set_equals_starting_full(bbx):
+    """A solver created empty and filled by one append steps exactly like a
+    solver created with the same particles: both chains are in ascending id
+    and the state is the same."""
+    sc = scenes.dam_break(12, 8, 12, spacing=0.01, tank=(0.3, 0.2, 0.3), seed=6)
+    empty = dataclasses.replace(sc, positions=np.zeros((0, 3)), velocities=np.zeros((0, 3)))
+    a = bbx.Solver(empty)
+    st = a.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)  # nothing to do
+    a.append(sc.positions, sc.velocities)
+    b = bbx.Solver(sc)
+    _same_chains(a, b)
+    for _ in range(5):
+        sa = a.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+        sb = b.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+        assert sa.iterations == sb.iterations
+    np.testing.assert_array_equal(a.get("position"), b.get("position"))
+    np.testing.assert_array_equal(a.get("velocity"), b.get("velocity"))

@@ -189,3 +189,60 @@ def test_tiny_particle_counts(bbx, oracle, n):
         o = oracle.Sim(gpu_positions_scene(sc, pos.astype(np.float32).astype(np.float64)))
         o.pcisph_substep(sc.dt, 3, 0.01, oracle.MODE_TEXTBOOK)
         np.testing.assert_allclose(g.get("position"), o.get("position"), rtol=0, atol=1e-6)
+
+
+def test_bucket_cap_truncates_at_100_like_the_oracle(bbx, oracle):
+    """150 particles within the radius of each other: every bucket keeps the
+    first 100 in cell/chain order and the overflow count is 150 * 50
+    (src/core/particle.h:45-51)."""
+    import dataclasses
+
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-0.004, 0.004, size=(150, 3)).astype(np.float32).astype(np.float64)
+    base = scenes.dam_break(2, 2, 2)
+    sc = dataclasses.replace(base, positions=pts, velocities=np.zeros_like(pts))
+    g = bbx.Solver(sc)
+    o = oracle.Sim(sc)
+    gst, ost = g.distribute(), o.distribute()
+    assert gst.bucket_overflows == ost.bucket_overflows == 150 * 50
+    gc, gi = g.buckets()
+    oc, oi = o.buckets()
+    assert np.all(gc == 100) and np.array_equal(gc, oc)
+    np.testing.assert_array_equal(gi[:, :100], oi[:, :100])
+
+
+def test_incremental_chain_order_hand_derived_on_device(bbx):
+    """The oracle's hand-derived three-particle case on the device: a cell's
+    new chain visits its neighbours in (y, x, z) order, a two-cell jump
+    without the flag loses the particle, and the flag restores ascending
+    ids (src/core/grid.h:449-494)."""
+    import dataclasses
+
+    base = scenes.dam_break(2, 2, 2)
+    # 0.09 rather than the oracle test's 0.1: x = 0.1 lies on a cell face,
+    # where the FP32 position (0.100000001) hashes one cell further
+    sc = dataclasses.replace(base, positions=np.array([[-0.2, 0.0, 0.0], [0.09, 0.0, 0.0], [0.05, 0.0, 0.0]]),
+                             velocities=np.zeros((3, 3)), target_spacing=0.1, relative_radius=2.0,
+                             tank_size=(0.6, 0.2, 0.2), grid_spacing=0.2, grid_scale=1.0)
+    g = bbx.Solver(sc)
+    assert list(g.info()["grid"].usizes) == [3, 1, 1]
+    start, members = g.chains()
+    assert start.tolist() == [0, 1, 3, 3] and members.tolist() == [0, 1, 2]
+    g.set("position", np.array([[-0.05, 0.0, 0.0], [-0.15, 0.0, 0.0], [0.05, 0.0, 0.0]]))
+    g.distribute()
+    start, members = g.chains()
+    assert start.tolist() == [0, 1, 3, 3] and members.tolist() == [1, 0, 2]
+    g.set("position", np.array([[0.0, 0.0, 0.0], [-0.01, 0.0, 0.0], [0.25, 0.0, 0.0]]))
+    g.distribute()
+    start, members = g.chains()
+    assert start.tolist() == [0, 0, 2, 3] and members.tolist() == [1, 0, 2]
+    g.set("position", np.array([[0.0, 0.0, 0.0], [-0.01, 0.0, 0.0], [-0.25, 0.0, 0.0]]))
+    st = g.distribute()
+    assert st.lost_particles == 1
+    start, members = g.chains()
+    assert members.tolist() == [1, 0]
+    g.flag = 1
+    st = g.distribute()
+    assert st.full_redistribution == 1 and st.lost_particles == 0
+    start, members = g.chains()
+    assert start.tolist() == [0, 1, 3, 3] and members.tolist() == [2, 0, 1]
