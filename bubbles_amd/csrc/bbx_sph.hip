@@ -334,38 +334,43 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
 // into the error slot of iteration k.  Does nothing once an earlier
 // iteration integrated.  LDS: predicted positions.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepConst st, double delta,
-                                                             NbrTab nb,
-                                                             const float4 *__restrict__ xstar,
-                                                             const float4 *__restrict__ pos,
-                                                             float *__restrict__ pres,
-                                                             float *__restrict__ rho_star,
-                                                             float4 *__restrict__ xq,
-                                                             LoopState *__restrict__ loop, int k) {
+// Two lanes per particle (quads half, half + 2, ...), 512 threads per tile:
+// twice the waves of one lane per particle at the same LDS, to hide the
+// gather's latency.
+constexpr int kPpThreads = 2 * kBlock;
+__global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, StepConst st, double delta,
+                                                                 NbrTab nb,
+                                                                 const float4 *__restrict__ xstar,
+                                                                 const float4 *__restrict__ pos,
+                                                                 float *__restrict__ pres,
+                                                                 float *__restrict__ rho_star,
+                                                                 float4 *__restrict__ xq,
+                                                                 LoopState *__restrict__ loop, int k) {
     // predicted positions, copied by 16-byte LDS DMA (the w lane is the
     // unused fourth word of x*)
     __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
-    __shared__ unsigned long long wmax[kBlock / 64];
+    __shared__ unsigned long long wmax[kPpThreads / 64];
     if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
     const int tile = tile_of_block();
     tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
     __syncthreads();
     const bool staged = tt.staged != 0;
-    const int i = tile * kBlock + threadIdx.x;
+    const int pl = threadIdx.x >> 1, half = threadIdx.x & 1;
+    const int i = tile * kBlock + pl;
     const bool valid = i < c.n;
-    // the particle's own data and first bucket quads do not depend on the
-    // staging: request them first so their latency overlaps it
-    NbrQuads<kNbrDepth> first{};
+    // the particle's own data and this lane's first quads do not depend on
+    // the staging: request them first so their latency overlaps it
+    NbrQuads<2> first{};
     int cnt = 0;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), pi = xi;
     float pold = 0.f;
     if (valid) {
-        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
+        first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
         cnt = nb.cnt[i];
         xi = xstar[i];
         pi = pos[i];
-        if (!st.first) pold = pres[i];
+        if (!st.first && half == 0) pold = pres[i];
     }
     if (staged) {
         tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
@@ -373,11 +378,11 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
     }
     __syncthreads();
     unsigned long long ebits = 0ull;
-    if (valid) {
+    if (valid) {  // both lanes of a particle share validity
         float sum = 0.f, comp = 0.f;
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
-            for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
+            for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
                 float3 xj;
                 if (kS) {
                     const float4 a = sx[j];
@@ -389,9 +394,13 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
                 float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
                 kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
-            });
+            }, half);
         });
-        float density = sum * c.mass;
+        // the pair's compensated partial sums; a + b == b + a, so both lanes
+        // hold the same total
+        const float mine = sum - comp;
+        const float total = mine + __shfl_xor(mine, 1, 64);
+        float density = total * c.mass;
         // FP64 error and pressure increment (delta is FP64 on the host)
         double err = (double)density - (double)c.rho0;
         double dp = delta * err;
@@ -399,22 +408,24 @@ __global__ void __launch_bounds__(kBlock) k_predict_pressure(SimConst c, StepCon
             dp *= (double)c.nps;
             err *= (double)c.nps;
         }
-        float p = pold + (float)dp;
-        pres[i] = p;
-        rho_star[i] = density;
-        float d2 = density * density;
-        xq[i] = make_float4(pi.x, pi.y, pi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
-        // |err| with a cleared sign bit: -0.0 must not win the unsigned max
-        // (the reference's AbsfMax keeps the first of equal squares)
-        const double ae = fabs(err);
-        ebits = (unsigned long long)__double_as_longlong(ae);
+        if (half == 0) {
+            float p = pold + (float)dp;
+            pres[i] = p;
+            rho_star[i] = density;
+            float d2 = density * density;
+            xq[i] = make_float4(pi.x, pi.y, pi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
+            // |err| with a cleared sign bit: -0.0 must not win the unsigned max
+            // (the reference's AbsfMax keeps the first of equal squares)
+            const double ae = fabs(err);
+            ebits = (unsigned long long)__double_as_longlong(ae);
+        }
     }
     ebits = wave_max_u64(ebits);
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = ebits;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long m = wmax[0];
-        for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
+        for (int w = 1; w < kPpThreads / 64; w++) m = wmax[w] > m ? wmax[w] : m;
         if (m) atomicMax(&loop->slot[k][blockIdx.x & (kSlots - 1)], m);
     }
 }
@@ -750,7 +761,7 @@ int run_predict_pressure(bbx_solver *s, const StepConst &st, int k) {
     if (k < 0 || k >= kMaxIt) return fail(BBX_E_INVALID, "iteration index out of range");
     // the pressure increment uses the host's FP64 delta (s->delta64)
     BBX_PROF_BEGIN_IT("predict_pressure", k);
-    k_predict_pressure<<<nblocks(s->n), kBlock, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
+    k_predict_pressure<<<nblocks(s->n), kPpThreads, 0, s->stream>>>(s->sc, st, s->delta64, nbr_tab(s), s->xstar,
                                                                 s->posA, s->pres, s->rho_star, s->xq,
                                                                 s->loop, k);
     BBX_HIP_CHECK(hipGetLastError());
