@@ -67,12 +67,13 @@ FACADE_DIR = os.path.join(ROOT, "tests", "cpp")
 FACADE_BIN = os.path.join(FACADE_DIR, "facade_dam_break")
 EMIT_BIN = os.path.join(FACADE_DIR, "facade_emit")
 OBSTACLES_BIN = os.path.join(FACADE_DIR, "facade_obstacles")
+LOADOBJ_BIN = os.path.join(FACADE_DIR, "facade_loadobj")
 
 
 def build_facade_example() -> str:
     """Compile the C++ facade scenes (host code only) against the library."""
     cxx = shutil.which("g++") or "g++"
-    for exe in (FACADE_BIN, EMIT_BIN, OBSTACLES_BIN):
+    for exe in (FACADE_BIN, EMIT_BIN, OBSTACLES_BIN, LOADOBJ_BIN):
         cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, exe + ".cpp", "-o", exe,
                "-L" + PKG, "-lbubbles_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../bubbles_amd"]
         res = subprocess.run(cmd, capture_output=True, text=True)
