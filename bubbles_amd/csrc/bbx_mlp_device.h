@@ -91,6 +91,52 @@ __device__ __forceinline__ void load_infer_operands(const float *__restrict__ P,
     }
 }
 
+// The BF16 weights of both layers staged once per workgroup in LDS.  Rows
+// are padded by four entries (8 bytes), so the sixteen rows that a wave
+// reads at once start in different banks.
+constexpr int kW1Row = kF + 4;                          // 20
+constexpr int kW2Row = kH + 4;                          // 68
+constexpr int kLdsWeights = kH * kW1Row + kH * kW2Row;  // BF16 entries
+
+// Called by all nthreads threads of the workgroup; the caller synchronises
+// before the operands are read.  P is 16-byte aligned (its own allocation)
+// and so are both weight blocks.
+__device__ __forceinline__ void stage_infer_weights(const float *__restrict__ P, unsigned short *lw, int tid,
+                                                    int nthreads) {
+    for (int k = tid; k < kH * kF / 4; k += nthreads) {
+        const int row = k / (kF / 4), c4 = k % (kF / 4);
+        const float4 v = *reinterpret_cast<const float4 *>(P + OFF_W1 + 4 * k);
+        *reinterpret_cast<bf16x4 *>(lw + row * kW1Row + 4 * c4) = pack4(v.x, v.y, v.z, v.w);
+    }
+    unsigned short *lw2 = lw + kH * kW1Row;
+    for (int k = tid; k < kH * kH / 4; k += nthreads) {
+        const int row = k / (kH / 4), c4 = k % (kH / 4);
+        const float4 v = *reinterpret_cast<const float4 *>(P + OFF_W2 + 4 * k);
+        *reinterpret_cast<bf16x4 *>(lw2 + row * kW2Row + 4 * c4) = pack4(v.x, v.y, v.z, v.w);
+    }
+}
+
+// load_infer_operands with the weights read from the staged LDS copy (the
+// same BF16 values: one rounding of the same FP32 parameters)
+__device__ __forceinline__ void load_infer_operands_lds(const float *__restrict__ P, const unsigned short *lw,
+                                                        int lane, bf16x4 (&w1)[4], bf16x4 (&w2)[4][4],
+                                                        float (&mean)[4], float (&istd)[4]) {
+    const int r = lane & 15, q = lane >> 4;
+    const unsigned short *lw2 = lw + kH * kW1Row;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        w1[m] = *reinterpret_cast<const bf16x4 *>(lw + (16 * m + r) * kW1Row + 4 * q);
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++)
+            w2[m][kc] = *reinterpret_cast<const bf16x4 *>(lw2 + (16 * m + r) * kW2Row + 16 * kc + 4 * q);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        mean[v] = P[OFF_MEAN + 4 * q + v];
+        istd[v] = P[OFF_ISTD + 4 * q + v];
+    }
+}
+
 // Inference forward of T 16-sample tiles at once (the MFMA and GELU chains
 // of one tile overlap those of the others).  x[t] = raw features 4q..4q+3
 // of sample lane%16 of tile t; sb1, sb2, sw3 = layer biases and output

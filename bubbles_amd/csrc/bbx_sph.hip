@@ -278,7 +278,11 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
         float *lf = pool;               // 256 padded feature rows
         float *lp = lf + kBlock * kRow; // 256 warm-start pressures
         float *lb = pool + 5 * kTileCap2;  // b1 | b2 | w3
-        static_assert(kBlock * (kRow + 1) <= 5 * kTileCap2 && 3 * kH <= 3 * kTileCap2, "mode-3 LDS layout");
+        // BF16 weights of both layers (16-byte aligned: 5 kTileCap2 + 3 kH floats in)
+        unsigned short *lw = reinterpret_cast<unsigned short *>(lb + 3 * kH);
+        static_assert(kBlock * (kRow + 1) <= 5 * kTileCap2 &&
+                          3 * kH + kLdsWeights / 2 <= 3 * kTileCap2 && (5 * kTileCap2 + 3 * kH) % 4 == 0,
+                      "mode-3 LDS layout");
         if (half == 0) {
 #pragma unroll
             for (int f = 0; f < BBX_NFEAT; f++) lf[pl * kRow + f] = valid ? fv[f] : 0.f;
@@ -288,12 +292,13 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
             lb[kH + threadIdx.x] = P[OFF_B2 + threadIdx.x];
             lb[2 * kH + threadIdx.x] = P[OFF_W3 + threadIdx.x];
         }
+        stage_infer_weights(P, lw, threadIdx.x, kNpThreads);
         __syncthreads();
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int qq = lane >> 4, jj = lane & 15;
         bf16x4 w1[4], w2[4][4];
         float mean[4], istd[4];
-        load_infer_operands(P, lane, w1, w2, mean, istd);
+        load_infer_operands_lds(P, lw, lane, w1, w2, mean, istd);
         const float b3 = P[OFF_B3], oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA];
         const float pcap = P[OFF_PCAP], resid = P[OFF_RESID];
         float xv[2][4];

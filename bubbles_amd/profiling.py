@@ -147,6 +147,10 @@ def traffic_markdown(rec: dict, command: str, algorithmic: dict | None = None) -
         alg = (algorithmic or {}).get(name, "")
         lines.append(f"| {name} | {k['dispatches']} | {k['fetch_bytes'] / 1e6:.1f} | "
                      f"{k['write_bytes'] / 1e6:.1f} | {k['bytes_per_particle']} | {alg} |")
+    if "buckets_density" in rec["kernels"]:
+        lines += ["", "The algorithmic figure of `buckets_density` leaves out the bucket it writes: "
+                  "2 B more per bucket entry (the benchmark line's `bytes_per_particle` adds them "
+                  "at the measured mean entry count)."]
     return "\n".join(lines) + "\n"
 
 
