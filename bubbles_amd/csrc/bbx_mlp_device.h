@@ -98,29 +98,63 @@ constexpr int kW1Row = kF + 4;                          // 20
 constexpr int kW2Row = kH + 4;                          // 68
 constexpr int kLdsWeights = kH * kW1Row + kH * kW2Row;  // BF16 entries
 
-// Called by all nthreads threads of the workgroup; the caller synchronises
-// before the operands are read.  P is 16-byte aligned (its own allocation)
-// and so are both weight blocks.
-__device__ __forceinline__ void stage_infer_weights(const float *__restrict__ P, unsigned short *lw, int tid,
-                                                    int nthreads) {
-    for (int k = tid; k < kH * kF / 4; k += nthreads) {
-        const int row = k / (kF / 4), c4 = k % (kF / 4);
-        const float4 v = *reinterpret_cast<const float4 *>(P + OFF_W1 + 4 * k);
-        *reinterpret_cast<bf16x4 *>(lw + row * kW1Row + 4 * c4) = pack4(v.x, v.y, v.z, v.w);
+// Staging for a 512-thread workgroup, split in two: the global loads are
+// issued early (before a long loop, to hide their latency) and the LDS
+// stores later.  P is 16-byte aligned (its own allocation), and so are both
+// weight blocks.  Thread t holds W1 float4 t (t < 256) and W2 float4s t and
+// t + 512, the layer biases and output weight of unit t (t < 64), and the
+// feature normalisation of its MFMA lane.
+struct InferPrefetch {
+    float4 w1, w2a, w2b;
+    float b1, b2, w3;
+    float mean[4], istd[4];
+};
+constexpr int kPrefetchThreads = 512;
+static_assert(kH * kF / 4 <= kPrefetchThreads && kH * kH / 4 == 2 * kPrefetchThreads && kH <= kPrefetchThreads,
+              "prefetch split");
+
+__device__ __forceinline__ void prefetch_infer(const float *__restrict__ P, int tid, InferPrefetch &f) {
+    const float4 *w1 = reinterpret_cast<const float4 *>(P + OFF_W1);
+    const float4 *w2 = reinterpret_cast<const float4 *>(P + OFF_W2);
+    f.w1 = tid < kH * kF / 4 ? w1[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+    f.w2a = w2[tid];
+    f.w2b = w2[tid + kPrefetchThreads];
+    f.b1 = f.b2 = f.w3 = 0.f;
+    if (tid < kH) {
+        f.b1 = P[OFF_B1 + tid];
+        f.b2 = P[OFF_B2 + tid];
+        f.w3 = P[OFF_W3 + tid];
     }
-    unsigned short *lw2 = lw + kH * kW1Row;
-    for (int k = tid; k < kH * kH / 4; k += nthreads) {
-        const int row = k / (kH / 4), c4 = k % (kH / 4);
-        const float4 v = *reinterpret_cast<const float4 *>(P + OFF_W2 + 4 * k);
-        *reinterpret_cast<bf16x4 *>(lw2 + row * kW2Row + 4 * c4) = pack4(v.x, v.y, v.z, v.w);
+    const int q = (tid & 63) >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        f.mean[v] = P[OFF_MEAN + 4 * q + v];
+        f.istd[v] = P[OFF_ISTD + 4 * q + v];
     }
 }
 
-// load_infer_operands with the weights read from the staged LDS copy (the
-// same BF16 values: one rounding of the same FP32 parameters)
-__device__ __forceinline__ void load_infer_operands_lds(const float *__restrict__ P, const unsigned short *lw,
-                                                        int lane, bf16x4 (&w1)[4], bf16x4 (&w2)[4][4],
-                                                        float (&mean)[4], float (&istd)[4]) {
+// LDS side: BF16 weights into lw (one rounding of the FP32 parameters), b1 | b2 | w3 into lb
+__device__ __forceinline__ void store_infer_prefetch(const InferPrefetch &f, unsigned short *lw, float *lb,
+                                                     int tid) {
+    if (tid < kH * kF / 4)
+        *reinterpret_cast<bf16x4 *>(lw + (tid / (kF / 4)) * kW1Row + 4 * (tid % (kF / 4))) =
+            pack4(f.w1.x, f.w1.y, f.w1.z, f.w1.w);
+    unsigned short *lw2 = lw + kH * kW1Row;
+    const int ka = tid, kb = tid + kPrefetchThreads;
+    *reinterpret_cast<bf16x4 *>(lw2 + (ka / (kH / 4)) * kW2Row + 4 * (ka % (kH / 4))) =
+        pack4(f.w2a.x, f.w2a.y, f.w2a.z, f.w2a.w);
+    *reinterpret_cast<bf16x4 *>(lw2 + (kb / (kH / 4)) * kW2Row + 4 * (kb % (kH / 4))) =
+        pack4(f.w2b.x, f.w2b.y, f.w2b.z, f.w2b.w);
+    if (tid < kH) {
+        lb[tid] = f.b1;
+        lb[kH + tid] = f.b2;
+        lb[2 * kH + tid] = f.w3;
+    }
+}
+
+// MFMA weight operands of lane `lane` from the staged LDS copy
+__device__ __forceinline__ void load_weight_operands_lds(const unsigned short *lw, int lane, bf16x4 (&w1)[4],
+                                                         bf16x4 (&w2)[4][4]) {
     const int r = lane & 15, q = lane >> 4;
     const unsigned short *lw2 = lw + kH * kW1Row;
 #pragma unroll
@@ -129,11 +163,6 @@ __device__ __forceinline__ void load_infer_operands_lds(const float *__restrict_
 #pragma unroll
         for (int kc = 0; kc < 4; kc++)
             w2[m][kc] = *reinterpret_cast<const bf16x4 *>(lw2 + (16 * m + r) * kW2Row + 16 * kc + 4 * q);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        mean[v] = P[OFF_MEAN + 4 * q + v];
-        istd[v] = P[OFF_ISTD + 4 * q + v];
     }
 }
 

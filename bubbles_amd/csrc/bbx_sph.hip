@@ -149,6 +149,10 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
     }
     __syncthreads();
     if (kMode != 3 && !valid) return;  // the lane pair shares validity; mode 3 keeps the block whole
+    // mode 3: the network's parameters are requested here, so that their
+    // latency hides behind the neighbour loop
+    InferPrefetch net;
+    if constexpr (kMode == 3) prefetch_infer(P, threadIdx.x, net);
     float fv[BBX_NFEAT];
     {
         const int self = staged && valid ? tab_local(tt, i) : i;
@@ -287,18 +291,13 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
             for (int f = 0; f < BBX_NFEAT; f++) lf[pl * kRow + f] = valid ? fv[f] : 0.f;
         }
-        if (threadIdx.x < kH) {
-            lb[threadIdx.x] = P[OFF_B1 + threadIdx.x];
-            lb[kH + threadIdx.x] = P[OFF_B2 + threadIdx.x];
-            lb[2 * kH + threadIdx.x] = P[OFF_W3 + threadIdx.x];
-        }
-        stage_infer_weights(P, lw, threadIdx.x, kNpThreads);
+        static_assert(kNpThreads == kPrefetchThreads, "the prefetch split assumes 512 threads");
+        store_infer_prefetch(net, lw, lb, threadIdx.x);
         __syncthreads();
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int qq = lane >> 4, jj = lane & 15;
         bf16x4 w1[4], w2[4][4];
-        float mean[4], istd[4];
-        load_infer_operands_lds(P, lw, lane, w1, w2, mean, istd);
+        load_weight_operands_lds(lw, lane, w1, w2);
         const float b3 = P[OFF_B3], oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA];
         const float pcap = P[OFF_PCAP], resid = P[OFF_RESID];
         float xv[2][4];
@@ -309,7 +308,7 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
             for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
         }
         float yn[2];
-        forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, mean, istd, qq, yn);
+        forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, net.mean, net.istd, qq, yn);
         if (lane < 16) {
 #pragma unroll
             for (int t = 0; t < 2; t++) {
