@@ -180,33 +180,74 @@ def roofline_for(res, n, stream_gbs):
             "frac_of_measured_copy": round(achieved / stream_gbs, 4) if achieved and stream_gbs else None}
 
 
-def run_cpu_baseline(sc, args):
-    from oracle import ref_cpu
-
-    threads = min(16, os.cpu_count() or 1)
-    sim = ref_cpu.Sim(sc, threads=threads)
-    mode = ref_cpu.MODE_TEXTBOOK if args.mode == "textbook" else ref_cpu.MODE_REFERENCE_EXACT
-    sim.pcisph_substep(sc.dt, args.max_it, args.tol, mode)  # warm-up
-    done, t = 0, 0.0
-    while done < args.cpu_steps and t < args.cpu_budget_s:
-        t0 = time.perf_counter()
-        sim.pcisph_substep(sc.dt, args.max_it, args.tol, mode)
-        t += time.perf_counter() - t0
-        done += 1
-    del sim
-    cpu_model = ""
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
-            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
-        pass
-    return {"value": round(sc.n * done / t, 1), "unit": "particle-steps/s", "cores": threads,
-            "cpu_model": cpu_model,
-            "kind": "port",
-            "sample": f"{done} timed substeps (after 1 warm-up) of the same {sc.n}-particle "
-                      f"{sc.name}, plain PCISPH, FP64 oracle with linked-list cells and 100-slot "
-                      f"buckets, {args.mode} loop, max {args.max_it} iterations, {threads} "
-                      f"threads, {t:.1f} s"}
+        return ""
+
+
+def _cpu_share():
+    """Threads for the all-core leg: the CPUs this process may run on, at
+    most 16 (the host-CPU share of one GPU on a shared box)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def _time_pcisph(ref_cpu, sc, threads, mode, max_it, tol, warm, timed):
+    sim = ref_cpu.Sim(sc, threads=threads)
+    for _ in range(warm):
+        sim.pcisph_substep(sc.dt, max_it, tol, mode)
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        sim.pcisph_substep(sc.dt, max_it, tol, mode)
+    t = time.perf_counter() - t0
+    del sim
+    return sc.n * timed / t, t
+
+
+def _time_sph(ref_cpu, sc, threads, frames):
+    sim = ref_cpu.Sim(sc, threads=threads)
+    t0 = time.perf_counter()
+    steps = sum(sim.sph_advance(1e-3, ref_cpu.SPH_TWO_PHASE) for _ in range(frames))
+    t = time.perf_counter() - t0
+    del sim
+    return sc.n * steps / t, t, steps
+
+
+def run_cpu_baseline(sc, args):
+    """The FP64 oracle (linked-list cells, 100-slot buckets, static thread
+    slicing like the reference's ParallelFor, src/cuda/cutil.h:294-338) on
+    this host: the PCISPH workload on every CPU of the share and on one
+    thread, and the 16,384-particle SPH dam break over 200 Advance(1e-3)
+    frames on the share (and 20 frames on one thread)."""
+    from bubbles_amd import scenes
+    from oracle import ref_cpu
+
+    threads = _cpu_share()
+    mode = ref_cpu.MODE_TEXTBOOK if args.mode == "textbook" else ref_cpu.MODE_REFERENCE_EXACT
+    cap = args.tol_max_it or args.max_it
+    v_all, t_all = _time_pcisph(ref_cpu, sc, threads, mode, cap, args.tol, 5, args.cpu_steps)
+    v_one, t_one = _time_pcisph(ref_cpu, sc, 1, mode, cap, args.tol, 1, args.cpu_steps_1t)
+    c1 = scenes.sph_dam_break()
+    c1_all, c1_t_all, c1_steps = _time_sph(ref_cpu, c1, threads, 200)
+    c1_one, c1_t_one, c1_steps_one = _time_sph(ref_cpu, c1, 1, 20)
+    return {"value": round(v_all, 1), "unit": "particle-steps/s", "cores": threads,
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(), "kind": "port",
+            "one_thread": round(v_one, 1),
+            "sample": f"FP64 oracle, {args.mode} PCISPH capped at {cap} iterations, the same {sc.n}-particle "
+                      f"{sc.name}: 5 warm-up + {args.cpu_steps} timed substeps on {threads} threads "
+                      f"({t_all:.1f} s), 1 warm-up + {args.cpu_steps_1t} on 1 thread ({t_one:.1f} s)",
+            "c1_sph_dam_break": {"n": c1.n, "frames": 200, "substeps": c1_steps,
+                                 "particle_steps_per_s": round(c1_all, 1), "threads": threads,
+                                 "seconds": round(c1_t_all, 2),
+                                 "one_thread": {"frames": 20, "substeps": c1_steps_one,
+                                                "particle_steps_per_s": round(c1_one, 1),
+                                                "seconds": round(c1_t_one, 2)}}}
 
 
 def summary(res):
@@ -235,8 +276,10 @@ def main():
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=100)
-    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="timed oracle substeps on all CPUs of the share (after 5 warm-up)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=2,
+                    help="timed oracle substeps on one thread (after 1 warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

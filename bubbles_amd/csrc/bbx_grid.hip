@@ -225,6 +225,13 @@ __global__ void __launch_bounds__(kBlock) k_scatter(SimConst c, const int *__res
 // ---------------------------------------------------------------------------
 // 4. rank inside the cell by key and move the particle state
 // ---------------------------------------------------------------------------
+// Keys are unique inside a cell (particle id, or (direction, old slot), or
+// old slot), so rank = number of smaller keys in the cell is a permutation
+// of the cell's segment at every occupancy.  A workgroup whose particles all
+// lie in one crowded cell stages the cell's keys through LDS a workgroup-
+// width chunk at a time (every lane then reads the same LDS word: a
+// broadcast); any other workgroup compares against global memory.
+constexpr int kRankLds = 512;  // cells above this use the LDS chunks when uniform
 __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__restrict__ tmp_idx,
                                                       const int *__restrict__ newc,
                                                       const int *__restrict__ hcell,
@@ -237,21 +244,41 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
                                                       float4 *__restrict__ vel_out,
                                                       int *__restrict__ cell_out,
                                                       int *__restrict__ home_out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.n) return;
-    int s = tmp_idx[t];
-    int cc = newc[s];
-    unsigned long long k = key[s];
-    int b = start[cc];
-    int e = b + count[cc];
-    int dest;
-    if (e - b <= 4096) {
-        int rank = 0;
+    __shared__ unsigned long long skey[kBlock];
+    __shared__ int cell0;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < c.n;
+    const int s = valid ? tmp_idx[t] : 0;
+    const int cc = valid ? newc[s] : -1;
+    const unsigned long long k = valid ? key[s] : 0ull;
+    const int b = valid ? start[cc] : 0;
+    const int e = valid ? b + count[cc] : 0;
+    // thread 0 is valid in every launched workgroup (the grid is ceil(n / kBlock))
+    if (threadIdx.x == 0) cell0 = cc;
+    __syncthreads();
+    const bool uniform = __syncthreads_and(!valid || cc == cell0) != 0;
+    const int c0 = cell0;
+    const int bb = uniform ? start[c0] : 0;
+    const int ee = uniform ? bb + count[c0] : 0;
+    int rank = 0;
+    if (uniform && ee - bb > kRankLds) {
+        // every valid thread has [b, e) == [bb, ee); the trip count and the
+        // barriers are the same for the whole workgroup
+        for (int q0 = bb; q0 < ee; q0 += kBlock) {
+            const int q = q0 + (int)threadIdx.x;
+            if (q < ee) skey[threadIdx.x] = key[tmp_idx[q]];
+            __syncthreads();
+            if (valid) {
+                const int m = min(kBlock, ee - q0);
+                for (int u = 0; u < m; u++) rank += skey[u] < k ? 1 : 0;
+            }
+            __syncthreads();
+        }
+    } else if (valid) {
         for (int q = b; q < e; q++) rank += key[tmp_idx[q]] < k ? 1 : 0;
-        dest = b + rank;
-    } else {
-        dest = t;  // pathological occupancy: keep arrival order
     }
+    if (!valid) return;
+    const int dest = b + rank;
     pos_out[dest] = pos_in[s];
     vel_out[dest] = vel_in[s];
     cell_out[dest] = cc;

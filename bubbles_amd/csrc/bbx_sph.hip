@@ -450,14 +450,13 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
 __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst st, NbrTab nb,
                                                            const float4 *__restrict__ pos,
                                                            const float4 *__restrict__ vel,
-                                                           const float4 *__restrict__ force,
+                                                           float4 *force,
                                                            const float4 *__restrict__ xq,
                                                            const float *__restrict__ pres,
                                                            float4 *__restrict__ xstar,
                                                            float4 *__restrict__ fp_out,
                                                            float4 *__restrict__ pos_out,
                                                            float4 *__restrict__ vel_out,
-                                                           float4 *__restrict__ force_out,
                                                            int *__restrict__ flag,
                                                            unsigned *__restrict__ max_force,
                                                            LoopState *__restrict__ loop, int k,
@@ -550,7 +549,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             integrate_one(c, st.dt, xi, vi, Fx, Fy, Fz, p, v, flag);
             pos_out[i] = make_float4(p.x, p.y, p.z, xi.w);
             vel_out[i] = make_float4(v.x, v.y, v.z, pres[i]);
-            force_out[i] = make_float4(Fx, Fy, Fz, 0.f);
+            force[i] = make_float4(Fx, Fy, Fz, 0.f);  // the lane read force[i] above
             fp_out[i] = make_float4(fpx, fpy, fpz, 0.f);
             fbits = __float_as_uint(sqrtf(Fx * Fx + Fy * Fy + Fz * Fz));
         }
@@ -779,7 +778,7 @@ int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, in
     BBX_PROF_BEGIN_IT("pressure_force", k < 0 ? -1 : k);
     k_pressure_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(
         s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->xq, s->pres, s->xstar, s->fp, s->posB,
-        s->velB, s->force, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol);
+        s->velB, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol);
     BBX_HIP_CHECK(hipGetLastError());
     BBX_PROF_END();
     return BBX_OK;

@@ -738,41 +738,16 @@ class ContinuousParticleSetBuilder3 {
         if (reemitions > 0 && reemitOnce) return;
         if (!other || !other->mappedDomain) throw bubbles_error("MapGridEmitToOther: the other builder has no mapped grid");
         std::vector<int> start, members;
-        std::vector<double> otherPos, pos;
+        std::vector<double> otherPos, lookup;
         bubbles_detail::chain_snapshot(other->particleSet, other->mappedDomain, start, members, otherPos);
         const int n = particleSet->GetParticleCount();
         if (particleSet->dev) {
-            pos.assign(3 * (size_t)n, 0.0);
-            if (n) bbx_throw_if(bbx_solver_download(particleSet->dev, BBX_FIELD_POSITION, pos.data()), "MapGridEmitToOther");
+            lookup.assign(3 * (size_t)n, 0.0);
+            if (n) bbx_throw_if(bbx_solver_download(particleSet->dev, BBX_FIELD_POSITION, lookup.data()), "MapGridEmitToOther");
         } else {
-            pos = particleSet->pos;
+            lookup = particleSet->pos;
         }
-        int numNewParticles = 0;
-        for (unsigned h : mappedCellSet) {
-            const std::vector<vec3f> &cellPos = mappedPositions[h];
-            const int size = start[h + 1] - start[h];
-            if (size >= BBX_BUCKET_SIZE) continue;
-            const int toInsert = std::min(BBX_BUCKET_SIZE - size, (int)cellPos.size());
-            for (int i = 0; i < toInsert; i++) {
-                const vec3f pi = cellPos[(size_t)i];
-                bool canAdd = true;
-                for (int k = start[h]; k < start[h + 1] && canAdd; k++) {
-                    const int id = members[(size_t)k];
-                    if (id >= n) throw bubbles_error("MapGridEmitToOther: chain id beyond this builder's set");
-                    const size_t j = (size_t)id;
-                    canAdd = (vec3f(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]) - pi).Length() >= d;
-                }
-                if (!canAdd) continue;
-                if (other->AddParticle(pi, velocity(pi))) {
-                    numNewParticles++;
-                } else {
-                    other->Commit();
-                    return;
-                }
-            }
-        }
-        if (numNewParticles > 0) other->Commit();
-        reemitions += 1;
+        if (refill(other, start, members, lookup, velocity, d)) reemitions += 1;
     }
 
     void MapGridEmit(const std::function<vec3f(const vec3f &)> &velocity, Float d = 0.02) {
@@ -782,30 +757,52 @@ class ContinuousParticleSetBuilder3 {
         std::vector<int> start, members;
         std::vector<double> pos;
         bubbles_detail::chain_snapshot(particleSet, mappedDomain, start, members, pos);
-        int numNewParticles = 0;
+        if (refill(this, start, members, pos, velocity, d)) reemitions += 1;
+    }
+
+  private:
+    /* Whether p keeps at least distance d from every particle of one chain
+     * (members[first .. first + len), positions looked up by id). */
+    static bool clear_of_chain(const vec3f &p, const std::vector<int> &members, int first, int len,
+                               const std::vector<double> &lookup, Float d) {
+        for (int k = first; k < first + len; k++) {
+            const size_t j = (size_t)members[(size_t)k];
+            if (3 * j + 2 >= lookup.size()) throw bubbles_error("MapGridEmit: chain id beyond the looked-up set");
+            if ((vec3f(lookup[3 * j], lookup[3 * j + 1], lookup[3 * j + 2]) - p).Length() < d) return false;
+        }
+        return true;
+    }
+
+    /* The refill rule of MapGridEmit and MapGridEmitToOther (src/core/grid.h:
+     * 1323-1395).  The mapped cells are visited in ascending id.  A cell
+     * whose chain in the target grid already holds a bucket's worth
+     * (BBX_BUCKET_SIZE) of particles is passed over; otherwise its first
+     * min(BBX_BUCKET_SIZE - chain length, mapped count) positions are tried
+     * in mapping order, and each one clear of the chain (as it stood when
+     * the call began) is added to the target.  When the target's
+     * AddParticle refuses one (capacity), what was added is committed and
+     * the call ends: returns false.  Otherwise anything added is committed
+     * at the end. */
+    bool refill(ContinuousParticleSetBuilder3 *target, const std::vector<int> &start,
+                const std::vector<int> &members, const std::vector<double> &lookup,
+                const std::function<vec3f(const vec3f &)> &velocity, Float d) {
+        int added = 0;
         for (unsigned h : mappedCellSet) {
-            const std::vector<vec3f> &cellPos = mappedPositions[h];
-            const int size = start[h + 1] - start[h];
-            if (size >= BBX_BUCKET_SIZE) continue;
-            const int toInsert = std::min(BBX_BUCKET_SIZE - size, (int)cellPos.size());
-            for (int i = 0; i < toInsert; i++) {
-                const vec3f pi = cellPos[(size_t)i];
-                bool canAdd = true;
-                for (int k = start[h]; k < start[h + 1] && canAdd; k++) {
-                    const size_t j = (size_t)members[(size_t)k];
-                    canAdd = (vec3f(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]) - pi).Length() >= d;
+            const int first = start[h], len = start[h + 1] - start[h];
+            if (len >= BBX_BUCKET_SIZE) continue;
+            const std::vector<vec3f> &cand = mappedPositions[h];
+            const size_t tries = std::min((size_t)(BBX_BUCKET_SIZE - len), cand.size());
+            for (size_t t = 0; t < tries; t++) {
+                if (!clear_of_chain(cand[t], members, first, len, lookup, d)) continue;
+                if (!target->AddParticle(cand[t], velocity(cand[t]))) {
+                    target->Commit();
+                    return false;
                 }
-                if (!canAdd) continue;
-                if (AddParticle(pi, velocity(pi))) {
-                    numNewParticles++;
-                } else {
-                    Commit();
-                    return;
-                }
+                added++;
             }
         }
-        if (numNewParticles > 0) Commit();
-        reemitions += 1;
+        if (added > 0) target->Commit();
+        return true;
     }
 };
 

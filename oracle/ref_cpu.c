@@ -680,6 +680,94 @@ void bbo_sim_append(bbo_sim *s, int count, const double *pos, const double *vel)
     parallel_for(s->threads, n0, n1, bucket_range, s);
 }
 
+/* ContinuousParticleSetBuilder::MapGrid (src/core/grid.h:1286-1321): every
+ * cell that some particle hashes into, in ascending id (the order a
+ * std::set<unsigned> iterates), each with the positions of the particles of
+ * its active chain, in chain order, at the time of the call.  Returns the
+ * number of mapped cells; cells / offsets (ncells + 1) / positions (3 per
+ * mapped position) may be NULL to count only, *npos gets the total number of
+ * mapped positions. */
+int bbo_map_grid(const bbo_sim *s, int *cells, int *offsets, double *positions, int *npos) {
+    unsigned char *mark = (unsigned char *)calloc(s->grid.total > 0 ? s->grid.total : 1, 1);
+    for (int i = 0; i < s->n; i++) {
+        double p[3] = {s->pos[i].x, s->pos[i].y, s->pos[i].z};
+        long long h = bbo_grid_hash(&s->grid, p, NULL);
+        if (h >= 0) mark[h] = 1;
+    }
+    int nc = 0, np = 0;
+    for (unsigned c = 0; c < s->grid.total; c++) {
+        if (!mark[c]) continue;
+        const cell_t *cell = &s->cells[c];
+        const int b = chain_sel(cell);
+        if (cells) {
+            cells[nc] = (int)c;
+            offsets[nc] = np;
+            int node = cell->head[b];
+            for (int j = 0; j < cell->len[b]; j++) {
+                positions[3 * (np + j)] = s->pos[node].x;
+                positions[3 * (np + j) + 1] = s->pos[node].y;
+                positions[3 * (np + j) + 2] = s->pos[node].z;
+                node = s->next[b][node];
+            }
+        }
+        nc++;
+        np += cell->len[b];
+    }
+    if (offsets) offsets[nc] = np;
+    if (npos) *npos = np;
+    free(mark);
+    return nc;
+}
+
+/* The refill rule of MapGridEmit / MapGridEmitToOther (src/core/grid.h:
+ * 1323-1395) with AddParticle's capacity test (:1397-1408).  Cells are
+ * visited in the mapped order.  A cell whose chain (in this simulation's
+ * grid) holds MaximumParticlesPerBucket (100) or more particles is passed
+ * over.  Otherwise its first min(100 - chain length, mapped count) mapped
+ * positions are tried in order; a position is kept unless a particle of the
+ * cell's chain, as the chain stood when the call began, lies at a distance
+ * below d (the reference's Distance, FP64).  Keeping a position that would
+ * take the particle count past max_particles ends the call with
+ * *capacity_hit = 1; the positions kept before it are still committed.
+ * Returns how many positions were kept, written to out (3 doubles each, at
+ * most out_cap of them). */
+int bbo_map_grid_emit(const bbo_sim *s, int ncells, const int *cells, const int *offsets,
+                      const double *positions, double d, int max_particles, double *out, int out_cap,
+                      int *capacity_hit) {
+    int kept = 0;
+    *capacity_hit = 0;
+    for (int k = 0; k < ncells; k++) {
+        const cell_t *cell = &s->cells[cells[k]];
+        const int b = chain_sel(cell);
+        const int len = cell->len[b];
+        if (len >= BBO_BUCKET_SIZE) continue;
+        const int avail = offsets[k + 1] - offsets[k];
+        const int tries = BBO_BUCKET_SIZE - len < avail ? BBO_BUCKET_SIZE - len : avail;
+        for (int t = 0; t < tries; t++) {
+            const double *q = &positions[3 * (offsets[k] + t)];
+            const v3 pi = mk(q[0], q[1], q[2]);
+            int clear = 1;
+            int node = cell->head[b];
+            for (int j = 0; j < len && clear; j++) {
+                if (vdist(s->pos[node], pi) < d) clear = 0;
+                node = s->next[b][node];
+            }
+            if (!clear) continue;
+            if (s->n + kept + 1 > max_particles) {
+                *capacity_hit = 1;
+                return kept;
+            }
+            if (kept < out_cap) {
+                out[3 * kept] = pi.x;
+                out[3 * kept + 1] = pi.y;
+                out[3 * kept + 2] = pi.z;
+            }
+            kept++;
+        }
+    }
+    return kept;
+}
+
 void bbo_sim_get_buckets(const bbo_sim *s, int *counts, int *ids) {
     memcpy(counts, s->bucket_cnt, sizeof(int) * (size_t)s->n);
     if (ids) memcpy(ids, s->bucket_ids, sizeof(int) * BBO_BUCKET_SIZE * (size_t)s->n);

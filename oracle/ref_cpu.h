@@ -141,6 +141,12 @@ void bbo_sim_get_chains(const bbo_sim *s, int *cell_start, int *members);
 void bbo_sim_set_chains(bbo_sim *s, const int *cell_start, const int *members);
 /* buckets: counts[n], ids[n*100] (row i holds particle i's bucket) */
 void bbo_sim_get_buckets(const bbo_sim *s, int *counts, int *ids);
+/* ContinuousParticleSetBuilder::MapGrid and the MapGridEmit refill rule
+ * (src/core/grid.h:1286-1408); see ref_cpu.c. */
+int bbo_map_grid(const bbo_sim *s, int *cells, int *offsets, double *positions, int *npos);
+int bbo_map_grid_emit(const bbo_sim *s, int ncells, const int *cells, const int *offsets,
+                      const double *positions, double d, int max_particles, double *out, int out_cap,
+                      int *capacity_hit);
 
 /* ---- stages (each one is one reference wrapper) ---- */
 void bbo_stage_distribute(bbo_sim *s, bbo_stats *st);      /* UpdateGridDistributionCPU + ResetHigherLevel */

@@ -110,6 +110,10 @@ def lib():
         L.bbo_sim_get_chains.argtypes = [vp, ip, ip]
         L.bbo_sim_set_chains.argtypes = [vp, ip, ip]
         L.bbo_sim_get_buckets.argtypes = [vp, ip, ip]
+        L.bbo_map_grid.restype = i
+        L.bbo_map_grid.argtypes = [vp, ip, ip, dp, ip]
+        L.bbo_map_grid_emit.restype = i
+        L.bbo_map_grid_emit.argtypes = [vp, i, ip, ip, dp, d, i, dp, i, ip]
         L.bbo_stage_distribute.argtypes = [vp, C.POINTER(Stats)]
         L.bbo_stage_interaction.argtypes = [vp]
         L.bbo_stage_density.argtypes = [vp, i]
@@ -399,6 +403,31 @@ class Sim:
         ids = np.zeros((self.n, BUCKET), dtype=np.int32)
         lib().bbo_sim_get_buckets(self.h, _ip(counts), _ip(ids))
         return counts, ids
+
+    def map_grid(self):
+        """MapGrid: (cells ascending, offsets, positions) -- the mapped
+        positions of cells[k] are positions[offsets[k]:offsets[k + 1]]."""
+        npos = C.c_int()
+        nc = lib().bbo_map_grid(self.h, None, None, None, C.byref(npos))
+        cells = np.zeros(max(nc, 1), dtype=np.int32)
+        offsets = np.zeros(nc + 1, dtype=np.int32)
+        pos = np.zeros((max(npos.value, 1), 3))
+        lib().bbo_map_grid(self.h, _ip(cells), _ip(offsets), _dp(pos), C.byref(npos))
+        return cells[:nc], offsets, pos[:npos.value]
+
+    def map_grid_emit(self, mapped, d, max_particles):
+        """The MapGridEmit refill rule on this state: (positions to commit,
+        capacity_hit)."""
+        cells, offsets, pos = (np.ascontiguousarray(a) for a in mapped)
+        cells = cells.astype(np.int32)
+        offsets = offsets.astype(np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        cap = int(offsets[-1]) if offsets.size else 0
+        out = np.zeros((max(cap, 1), 3))
+        hit = C.c_int()
+        k = lib().bbo_map_grid_emit(self.h, cells.size, _ip(cells), _ip(offsets), _dp(pos), float(d),
+                                    int(max_particles), _dp(out), cap, C.byref(hit))
+        return out[:k].copy(), bool(hit.value)
 
     def distribute(self):
         st = Stats()

@@ -74,53 +74,22 @@ def _emit_velocity(p):
     return np.stack([0.5 * p[:, 0], -np.ones(len(p)), np.zeros(len(p))], axis=1)
 
 
-def _map_grid(g, pos):
-    """MapGrid (src/core/grid.h:1286-1321) right after a full distribution:
-    for every cell a particle hashes into, the positions of the cell's
-    particles in ascending id."""
-    mapped = {}
-    for i, h in enumerate(g.hash(pos)):
-        mapped.setdefault(int(h), []).append(pos[i])
-    return mapped
-
-
-def _map_grid_emit(g, mapped, d, cap, state, check_warned=True):
-    """MapGridEmit (src/core/grid.h:1366-1406) with AddParticle's capacity
-    rule and Commit (:1408-1425), restated over the ABI; MapGridEmitToOther
-    (:1323-1364) with the builder as its own target is the same without the
-    early exit once warned."""
-    if check_warned and state["warned"]:
-        return
-    start, members = g.chains()
+def _oracle_mirror(oracle, sc, g):
+    """An oracle simulation holding the device solver's positions and chains
+    (its MapGrid and refill rule then read the same state)."""
     pos = g.get("position")
-    new = []
-
-    def commit():
-        if new:
-            p = np.array(new)
-            g.append(p, _emit_velocity(p))
-
-    for h in sorted(mapped):
-        size = int(start[h + 1] - start[h])
-        if size >= 100:  # MaximumParticlesPerBucket
-            continue
-        chain = pos[members[start[h]:start[h + 1]]]
-        for pi in mapped[h][:min(100 - size, len(mapped[h]))]:
-            dd = chain - pi
-            dist = np.sqrt((dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2])
-            if np.any(dist < d):
-                continue
-            if g.n + len(new) + 1 <= cap:
-                new.append(pi)
-            else:
-                state["warned"] = True
-                commit()
-                return
-    commit()
+    o = oracle.Sim(dataclasses.replace(sc, positions=pos, velocities=np.zeros_like(pos)), setup=False)
+    o.set_chains(*g.chains())
+    return o
 
 
 @pytest.mark.gpu
-def test_facade_continuous_emission_matches_abi(bbx):
+def test_facade_continuous_emission_matches_abi(bbx, oracle):
+    """The facade's MapGrid / MapGridEmit / MapGridEmitToOther scene against
+    the same frames run through the ABI, with every refill decided by the
+    oracle's emission rule (pinned in tests/test_oracle_emit.py) on the
+    device's own state.  MapGridEmit returns at entry once the builder has
+    warned about its capacity; MapGridEmitToOther (odd frames) does not."""
     frames, cap = 16, 576
     out_path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"bbx_emit_{os.getpid()}.bin")
     res = subprocess.run([hipbuild.EMIT_BIN, str(frames), str(cap), out_path],
@@ -131,15 +100,19 @@ def test_facade_continuous_emission_matches_abi(bbx):
     os.remove(out_path)
     sc = _emit_scene()
     g = bbx.Solver(sc)
-    mapped = _map_grid(g, sc.positions)
-    assert out["mapped"] == len(mapped)
-    state, counts = {"warned": False}, []
-    for _ in range(frames):
+    mapped = _oracle_mirror(oracle, sc, g).map_grid()
+    assert out["mapped"] == len(mapped[0])
+    warned, counts = False, []
+    for f in range(frames):
         g.pcisph_advance(1.0 / 240.0, 8, 0.01, bbx.MODE_TEXTBOOK)
-        _map_grid_emit(g, mapped, 0.01, cap, state, check_warned=(len(counts) % 2 == 0))
+        if not (warned and f % 2 == 0):
+            new, hit = _oracle_mirror(oracle, sc, g).map_grid_emit(mapped, 0.01, cap)
+            warned = warned or hit
+            if len(new):
+                g.append(new, _emit_velocity(new))
         counts.append(g.n)
     assert out["counts"] == counts
-    assert out["warned"] == int(state["warned"])
+    assert out["warned"] == int(warned)
     # the scene exercises emission and the capacity limit
     assert counts[-1] == cap and len(set(counts)) >= 3, counts
     np.testing.assert_array_equal(fpos, g.get("position"))

@@ -246,3 +246,123 @@ def test_incremental_chain_order_hand_derived_on_device(bbx):
     assert st.full_redistribution == 1 and st.lost_particles == 0
     start, members = g.chains()
     assert start.tolist() == [0, 1, 3, 3] and members.tolist() == [2, 0, 1]
+
+
+def test_crowded_cell_order_matches_oracle(bbx, oracle):
+    """6,000 particles inside one cell, in shuffled id order, next to a
+    lattice block: the counting sort ranks a crowded cell exactly (workgroups
+    wholly inside it rank through LDS chunks, the two at its edges through
+    global memory), so the full distribution (ascending id) and an
+    incremental one (F7 concatenation of the 27 neighbours' old chains,
+    src/core/grid.h:389-407,449-494) match the oracle at that occupancy."""
+    import dataclasses
+
+    base = small_dam("sorted")
+    h = base.kernel_radius
+    gd = bbx.Solver(base).info()["grid"]
+    u = np.array(gd.usizes) - 4  # the far corner, away from the lattice block
+    centre = np.array(gd.bounds_min) + (u + 0.5) * np.array(gd.cells_len)
+    rng = np.random.default_rng(17)
+    crowd = centre + rng.uniform(-0.45 * h, 0.45 * h, size=(6000, 3))
+    pos = np.concatenate([base.positions, crowd]).astype(np.float32).astype(np.float64)
+    perm = rng.permutation(pos.shape[0])
+    pos = pos[perm]
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")))
+    start, members = g.chains()
+    assert np.diff(start).max() >= 6000
+    assert_chains_equal((start, members), o.chains())
+    # every particle moves by less than a third of a cell: crowd members
+    # spread into the neighbouring cells and the rank order is rebuilt
+    pre_chain = (start, members)
+    moved = pos + rng.uniform(-0.3 * h, 0.3 * h, size=pos.shape)
+    moved = moved.astype(np.float32).astype(np.float64)
+    g.set("position", moved)
+    assert g.flag == 0
+    gst = g.distribute()
+    o.set_chains(*pre_chain)
+    o.set("position", g.get("position"))
+    o.flag = 0
+    ost = o.distribute()
+    assert gst.full_redistribution == ost.full_redistribution == 0
+    assert gst.lost_particles == ost.lost_particles
+    assert_chains_equal(g.chains(), o.chains())
+    assert_buckets_equal(g.buckets(), o.buckets())
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_dam_break_1m_grid_gate(bbx, oracle, order):
+    """The 1,048,576-particle dam break (128x64x128 at 0.01, 2x1x2 tank):
+    cell chains and buckets bit-exact against the oracle at substeps 0, 1
+    and 50, in sorted and in shuffled particle order.  At each checkpoint
+    the oracle replays the device's pre-distribution chain state and
+    positions through its own distribution and bucket build."""
+    sc = scenes.dam_break(128, 64, 128, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
+    if order == "shuffled":
+        sc = sc.shuffled(2)
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")), threads=16)
+    assert_chains_equal(g.chains(), o.chains())  # Setup: ascending ids
+    step = 0
+    for target in (0, 1, 50):
+        while step < target:
+            g.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+            step += 1
+        pre_chain, pre_pos, flag = g.chains(), g.get("position"), g.flag
+        gst = g.distribute()
+        o.set_chains(*pre_chain)
+        o.set("position", pre_pos)
+        o.flag = flag
+        ost = o.distribute()
+        assert (gst.full_redistribution, gst.lost_particles, gst.bucket_overflows) == \
+            (ost.full_redistribution, ost.lost_particles, ost.bucket_overflows), target
+        assert_chains_equal(g.chains(), o.chains())
+        assert_buckets_equal(g.buckets(), o.buckets())
+        print(f"1M {order} substep {target}: chains and buckets exact", flush=True)
+
+
+def test_device_hash_on_grid_bounds(bbx, oracle):
+    """Particles whose FP32 coordinate lies within 1e-8 of a grid bound take
+    the device hash's ExtremeEpsilon branch (src/core/grid.h:258-298): on the
+    upper bound they are pulled into the last cell instead of falling
+    outside the grid.  One FP32 ulp further out (more than 1e-8 away) they
+    hash outside and join no chain.  The solver's setup distribution (device
+    hash) is compared with the oracle's FP64 hash and chains."""
+    import dataclasses
+
+    base = small_dam("sorted")
+    gd = bbx.Solver(base).info()["grid"]
+    lo, hi = np.array(gd.bounds_min), np.array(gd.bounds_max)
+    rng = np.random.default_rng(23)
+    pts, expect_lost = [], 0
+    for a in range(3):
+        for bound in (lo[a], hi[a]):
+            f = np.float32(bound)
+            assert abs(float(f) - bound) < 1e-8  # this tank's bounds have such FP32 neighbours
+            for k, coord in enumerate((f, np.nextafter(f, np.float32(-np.inf)), np.nextafter(f, np.float32(np.inf)))):
+                for _ in range(16):
+                    p = rng.uniform(lo + 0.05, hi - 0.05)
+                    p[a] = float(coord)
+                    pts.append(p)
+                    outside = float(coord) < lo[a] or float(coord) > hi[a]
+                    if outside and abs(float(coord) - bound) >= 1e-8:
+                        expect_lost += 1
+    pos = np.asarray(pts, dtype=np.float32).astype(np.float64)
+    sc = dataclasses.replace(base, positions=pos, velocities=np.zeros_like(pos))
+    g = bbx.Solver(sc)
+    o = oracle.Sim(gpu_positions_scene(sc, g.get("position")))
+    assert_chains_equal(g.chains(), o.chains())
+    start, members = g.chains()
+    assert members.size == pos.shape[0] - expect_lost and expect_lost == 2 * 3 * 16
+    # every particle exactly on an upper bound is in the last cell of that axis
+    og = o.info()["grid"]
+    on_hi = [i for i in range(pos.shape[0]) if any(pos[i, a] == float(np.float32(hi[a])) for a in range(3))]
+    assert len(on_hi) == 3 * 16
+    for i in on_hi:
+        cid, u = oracle.grid_hash(og, pos[i])
+        a = int(np.argmax([pos[i, b] == float(np.float32(hi[b])) for b in range(3)]))
+        assert cid >= 0 and u[a] == og.usizes[a] - 1
+        cell = int(np.searchsorted(start, np.nonzero(members == i)[0][0], side="right") - 1)
+        assert cell == cid
