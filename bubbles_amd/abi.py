@@ -154,6 +154,7 @@ def load():
         "bbx_sdf_destroy": (None, [vp]),
         "bbx_solver_set_colliders": (i, [vp, C.POINTER(BbxCollider), i]),
         "bbx_colliders_resolve": (i, [C.POINTER(BbxCollider), i, d, d, dp, dp, i, ip]),
+        "bbx_colliders_resolve_noalloc": (i, [C.POINTER(BbxCollider), i, d, d, dp, dp, i, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -331,6 +332,22 @@ def colliders_resolve(colliders, radius, restitution, positions, velocities):
     hit = np.zeros(p.shape[0], dtype=np.int32)
     check(load().bbx_colliders_resolve(arr, len(colliders), float(radius), float(restitution), _dp(p), _dp(v),
                                        p.shape[0], _ip(hit)), "bbx_colliders_resolve")
+    del keep
+    return p, v, hit
+
+
+NOALLOC_MAX_POINTS = 65536  # BBX_NOALLOC_MAX_POINTS
+
+
+def colliders_resolve_noalloc(colliders, radius, restitution, positions, velocities):
+    """The same as colliders_resolve through the entry point that allocates
+    no device memory (at most NOALLOC_MAX_POINTS points per call)."""
+    arr, keep = collider_structs(colliders)
+    p = np.array(positions, dtype=np.float64).reshape(-1, 3)
+    v = np.array(velocities, dtype=np.float64).reshape(-1, 3)
+    hit = np.zeros(p.shape[0], dtype=np.int32)
+    check(load().bbx_colliders_resolve_noalloc(arr, len(colliders), float(radius), float(restitution), _dp(p),
+                                               _dp(v), p.shape[0], _ip(hit)), "bbx_colliders_resolve_noalloc")
     del keep
     return p, v, hit
 

@@ -5,6 +5,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -136,6 +137,63 @@ __global__ void __launch_bounds__(256) k_resolve(const DevCollider *__restrict__
     pos[i] = make_float4(p.x, p.y, p.z, 0.f);
     vel[i] = make_float4(v.x, v.y, v.z, 0.f);
     hit[i] = h ? 1 : 0;
+}
+
+// The same rule over the caller's FP64 arrays as staged in device memory:
+// each coordinate rounds to FP32 exactly as the host conversion of
+// k_resolve's caller does, and the result widens back to FP64.
+__global__ void __launch_bounds__(256) k_resolve_f64(const DevCollider *__restrict__ cs, int nc, float radius,
+                                                     float e, double *__restrict__ pos, double *__restrict__ vel,
+                                                     int *__restrict__ hit, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double *pp = pos + 3 * (size_t)i;
+    double *vv = vel + 3 * (size_t)i;
+    float3 p = make_float3((float)pp[0], (float)pp[1], (float)pp[2]);
+    float3 v = make_float3((float)vv[0], (float)vv[1], (float)vv[2]);
+    const bool h = resolve_colliders(cs, nc, radius, e, p, v);
+    pp[0] = p.x;
+    pp[1] = p.y;
+    pp[2] = p.z;
+    vv[0] = v.x;
+    vv[1] = v.y;
+    vv[2] = v.z;
+    hit[i] = h ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Staging of bbx_colliders_resolve_noalloc: one file-scope device array per
+// GPU, laid out as [collider table | positions | velocities | flags].  Its
+// address is looked up once per device with hipGetSymbolAddress; host code
+// never passes the array's name itself to a kernel or a copy.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocCollBytes = ((sizeof(DevCollider) * kMaxColliders + 255) / 256) * 256;
+constexpr size_t kNoAllocVecBytes = (size_t)BBX_NOALLOC_MAX_POINTS * 3 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocCollBytes + 2 * kNoAllocVecBytes + (size_t)BBX_NOALLOC_MAX_POINTS * sizeof(int);
+static_assert(kNoAllocVecBytes % 256 == 0, "scratch segments stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kNoAllocDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call made with that device current).
+int scratch_for_current_device(unsigned char **out) {
+    int dev = -1;
+    BBX_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(BBX_E_RANGE, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BBX_E_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BBX_OK;
 }
 
 void sdf_fields(const bbx_sdf *s, DevCollider &d) {
@@ -419,6 +477,41 @@ int bbx_colliders_resolve(const bbx_collider *colliders, int count, double radiu
         vel[3 * (size_t)i + 2] = hv[(size_t)i].z;
         if (collided) collided[i] = hh[(size_t)i];
     }
+    return BBX_OK;
+}
+
+int bbx_colliders_resolve_noalloc(const bbx_collider *colliders, int count, double radius, double restitution,
+                                  double *pos, double *vel, int n, int *collided) {
+    if (n < 0 || (n > 0 && (!pos || !vel))) return fail(BBX_E_INVALID, "bad point arrays");
+    if (n > BBX_NOALLOC_MAX_POINTS)
+        return fail(BBX_E_RANGE, "bbx_colliders_resolve_noalloc takes at most " +
+                                     std::to_string(BBX_NOALLOC_MAX_POINTS) + " points per call (got " +
+                                     std::to_string(n) + ")");
+    if (count < 1) return fail(BBX_E_INVALID, "no colliders");
+    if (count > kMaxColliders) return fail(BBX_E_RANGE, "collider count out of range (0..64)");
+    if (!colliders) return fail(BBX_E_INVALID, "null collider list");
+    DevCollider table[kMaxColliders];
+    for (int i = 0; i < count; i++) BBX_CHECK(to_device(colliders[i], table[i]));
+    if (n == 0) return BBX_OK;
+    unsigned char *base = nullptr;
+    BBX_CHECK(scratch_for_current_device(&base));
+    DevCollider *dc = (DevCollider *)base;
+    double *dp = (double *)(base + kNoAllocCollBytes);
+    double *dv = (double *)(base + kNoAllocCollBytes + kNoAllocVecBytes);
+    int *dh = (int *)(base + kNoAllocCollBytes + 2 * kNoAllocVecBytes);
+    const size_t vb = sizeof(double) * 3 * (size_t)n;
+    hipError_t e = hipMemcpy(dc, table, sizeof(DevCollider) * (size_t)count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, pos, vb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dv, vel, vb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_resolve_f64<<<(unsigned)((n + 255) / 256), 256>>>(dc, count, (float)radius, (float)restitution, dp, dv,
+                                                             dh, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(pos, dp, vb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(vel, dv, vb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && collided) e = hipMemcpy(collided, dh, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(BBX_E_HIP, std::string("collider resolve (no-alloc): ") + hipGetErrorString(e));
     return BBX_OK;
 }
 

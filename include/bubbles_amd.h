@@ -302,6 +302,19 @@ int bbx_solver_set_colliders(bbx_solver *s, const bbx_collider *colliders, int c
  * (collided may be NULL). */
 int bbx_colliders_resolve(const bbx_collider *colliders, int count, double radius, double restitution,
                           double *pos, double *vel, int n, int *collided);
+/* bbx_colliders_resolve without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  The staging of
+ * the collider table, positions, velocities and flags lives in one static
+ * device array per GPU (about 3.4 MB, held until the process ends), so a
+ * call takes at most BBX_NOALLOC_MAX_POINTS points (BBX_E_RANGE beyond;
+ * split larger sets).  Results are bit-identical to bbx_colliders_resolve.
+ * Runs on the current device's default stream and returns when done.
+ * Calls that overlap on one GPU (several host threads) share that array
+ * and race: serialise them. */
+#define BBX_NOALLOC_MAX_POINTS 65536
+int bbx_colliders_resolve_noalloc(const bbx_collider *colliders, int count, double radius,
+                                  double restitution, double *pos, double *vel, int n, int *collided);
 
 /* ---- frame I/O in the reference's serializer format ----
  * Flag bits as SERIALIZER_* (src/third/serializer.h:7-17). */
