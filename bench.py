@@ -3,24 +3,26 @@
 
 Workload: the 1,048,576-particle PCISPH dam break (128x64x128 simple-cubic
 block at spacing 0.01, kernel radius 0.02, grid cell = kernel radius, 2x1x2
-tank, dt 5e-4 fixed, TEXTBOOK predict-correct loop to <= 1% max density error,
-at most 8 iterations).  One step = one substep of every particle, including
-all correction iterations; inputs are resident in HBM before timing starts.
+tank, dt 5e-4 fixed, TEXTBOOK predict-correct loop to <= 1% max density
+error).  One step = one substep of every particle, including all correction
+iterations; inputs are resident in HBM before timing starts.
 
-The same scene runs twice: plain PCISPH (pressure starts at 0) and PCISPH
-started from the learned warm-start network (bubbles_amd/data).  `value` is
-the learned run's particle-steps/s; the plain run's figures and the speed-up
-sit beside it in `config`.  A second pair with an iteration cap of 64
-("to_tolerance") shows the same comparison when both runs iterate until the
-tolerance is met.
+Headline pair: W warm-up + K timed substeps of plain PCISPH (pressure starts
+at 0) and of PCISPH with the learned warm start in its AUTO mode (the network
+starts a substep only where the plain loop is estimated to need three or more
+corrections), both iterating to the tolerance (cap 64).  `value` is the
+learned run's particle-steps/s; the plain run and the speed-up sit beside it.
+Both runs are timed the same way: the last (up to 20) warm-up substeps are
+profiled per kernel with HIP events to pick the dominant kernel, and inside
+the timed region only one launch per substep of that kernel is bracketed by
+events (on the solver's stream).
 
-Timing: W warm-up substeps (the last ones profiled per kernel with HIP events
-to pick the dominant kernel), then exactly K timed substeps bracketed by
-device synchronisations.  Inside the timed region only one launch per substep
-of the dominant kernel is bracketed by events (on the solver's stream), so
-the wall-clock value is not inflated by per-kernel events.  The CPU baseline
-times the FP64 oracle (a restatement of the reference CPU path) on a bounded
-sample of the same workload.
+Whole-run pairs (`config.whole_run`): the same scene over its full 500
+substeps from rest, plain vs learned, to tolerance and at the 8-iteration cap,
+plus the learned run with the network used on every substep; no events.
+
+The CPU baseline times the FP64 oracle (a restatement of the reference CPU
+path) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -98,21 +100,25 @@ def load_traffic(kernel: str, n: int):
     return k.get("bytes_per_launch")
 
 
-def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
+def run_variant(args, sc, warm: int, max_it: int, steps: int, warmup: int, profile: bool):
+    """Run `warmup` untimed and `steps` timed substeps of a fresh solver.
+
+    profile: the last (up to 20) warm-up substeps are profiled per kernel to
+    pick the dominant kernel, and inside the timed region only that kernel's
+    launch of each substep is bracketed by HIP events (one event pair per
+    substep, the same for every variant that is profiled)."""
     import numpy as np
 
     from bubbles_amd import abi
 
     mode = abi.MODE_TEXTBOOK if args.mode == "textbook" else abi.MODE_REFERENCE_EXACT
     solver = abi.Solver(sc)
-    warm = abi.WARM_NONE
-    if warm_mlp:
+    if warm != abi.WARM_NONE:
         from bubbles_amd import learned
 
         solver.attach_mlp(learned.load())
-        warm = abi.WARM_MLP
-    n_prof = min(args.warmup, 20) if profile else 0
-    for _ in range(args.warmup - n_prof):
+    n_prof = min(warmup, 20) if profile else 0
+    for _ in range(warmup - n_prof):
         solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
     table, dominant = {}, None
     if n_prof:
@@ -137,7 +143,7 @@ def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
     iters, errs = [], []
     solver.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         st = solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
         iters.append(st.iterations)
         errs.append(st.max_density_error)
@@ -149,7 +155,7 @@ def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
     solver.close()
     elapsed = t1 - t0
     errs = np.asarray(errs)
-    out = {"value": sc.n * args.steps / elapsed, "elapsed": elapsed,
+    out = {"value": sc.n * steps / elapsed, "elapsed": elapsed, "steps": steps,
            "mean_iterations": float(np.mean(iters)), "max_iterations": int(np.max(iters)),
            "max_density_error": float(errs.max()),
            "converged_fraction": float(np.mean(errs < args.tol)),
@@ -158,6 +164,16 @@ def run_variant(args, sc, warm_mlp: bool, max_it: int, profile: bool):
         rec = live.get(dominant)
         out["dominant_live"] = (rec["total_ms"] / rec["launches"], rec["launches"]) if rec else None
     return out
+
+
+def run_pair(args, sc, max_it, steps, warmup, profile):
+    """Plain (pressure from 0) and learned (WARM_AUTO) runs of the same scene
+    and window, timed the same way."""
+    from bubbles_amd import abi
+
+    plain = run_variant(args, sc, abi.WARM_NONE, max_it, steps, warmup, profile)
+    learned_run = run_variant(args, sc, abi.WARM_AUTO, max_it, steps, warmup, profile)
+    return plain, learned_run
 
 
 def roofline_for(res, n, stream_gbs):
@@ -230,7 +246,7 @@ def run_cpu_baseline(sc, args):
 
     threads = _cpu_share()
     mode = ref_cpu.MODE_TEXTBOOK if args.mode == "textbook" else ref_cpu.MODE_REFERENCE_EXACT
-    cap = args.tol_max_it or args.max_it
+    cap = args.max_it
     v_all, t_all = _time_pcisph(ref_cpu, sc, threads, mode, cap, args.tol, 5, args.cpu_steps)
     v_one, t_one = _time_pcisph(ref_cpu, sc, 1, mode, cap, args.tol, 1, args.cpu_steps_1t)
     c1 = scenes.sph_dam_break()
@@ -259,18 +275,24 @@ def summary(res):
             "converged_fraction": round(res["converged_fraction"], 4)}
 
 
+def pair_summary(plain, learned_run):
+    return {"plain": summary(plain), "learned": summary(learned_run),
+            "speedup": round(learned_run["value"] / plain["value"], 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--mode", choices=["textbook", "exact"], default="textbook")
-    ap.add_argument("--warm", choices=["both", "none", "mlp"], default="both",
-                    help="both: learned run is `value`, plain run beside it")
-    ap.add_argument("--max-it", type=int, default=8)
+    ap.add_argument("--max-it", type=int, default=64,
+                    help="iteration cap of the headline pair (64: iterate until the tolerance is met)")
     ap.add_argument("--tol", type=float, default=0.01)
-    ap.add_argument("--tol-max-it", type=int, default=64,
-                    help="iteration cap of the to-tolerance pair (0: skip)")
+    ap.add_argument("--run-steps", type=int, default=500,
+                    help="length of the whole-run pairs timed from substep 0 (0: skip)")
+    ap.add_argument("--cap8", type=int, default=8,
+                    help="iteration cap of the second whole-run pair (0: skip)")
     ap.add_argument("--scene", choices=["dam_break", "block_drop"], default="dam_break",
                     help="block_drop: the 8,388,608-particle cube-onto-layer scene (dt 2.5e-4)")
     ap.add_argument("--nx", type=int, default=128)
@@ -297,38 +319,35 @@ def main():
         sc = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=2)
         workload = (f"PCISPH dam break {sc.n} particles ({args.nx}x{args.ny}x{args.nz} "
                     f"at 0.01, h 0.02, 2x1x2 tank), dt 5e-4")
-    runs = {}
-    if args.warm in ("both", "none"):
-        runs["plain"] = run_variant(args, sc, False, args.max_it, profile=args.warm == "none")
-    if args.warm in ("both", "mlp"):
-        runs["learned"] = run_variant(args, sc, True, args.max_it, profile=True)
-    for r in runs.values():
-        r["steps"] = args.steps
-    head = runs["learned"] if "learned" in runs else runs["plain"]
-    tol_pair = None
-    if args.tol_max_it and args.warm == "both" and args.mode == "textbook":
-        tp = run_variant(args, sc, False, args.tol_max_it, profile=False)
-        tl = run_variant(args, sc, True, args.tol_max_it, profile=False)
-        tp["steps"] = tl["steps"] = args.steps
-        tol_pair = {"max_it": args.tol_max_it, "plain": summary(tp), "learned": summary(tl),
-                    "speedup": round(tl["value"] / tp["value"], 4)}
+    # headline: W warm-up + K timed substeps, plain and learned, to tolerance
+    plain, head = run_pair(args, sc, args.max_it, args.steps, args.warmup, profile=True)
+    textbook = args.mode == "textbook"
+    whole = {}
+    if args.run_steps and textbook:
+        # the whole run from rest, as the configuration is specified (500
+        # substeps), to tolerance and at the 8-iteration cap
+        whole["to_tolerance"] = pair_summary(*run_pair(args, sc, args.max_it, args.run_steps, 0, False))
+        whole["to_tolerance"]["max_it"] = args.max_it
+        if args.cap8:
+            whole["cap8"] = pair_summary(*run_pair(args, sc, args.cap8, args.run_steps, 0, False))
+            whole["cap8"]["max_it"] = args.cap8
+        always = run_variant(args, sc, abi.WARM_MLP, args.max_it, args.run_steps, 0, False)
+        whole["to_tolerance"]["learned_every_substep"] = summary(always)
     gbs, _ = abi.stream_copy(2.0 ** 30, 20)
     roof = roofline_for(head, sc.n, gbs)
     kbar = head["mean_iterations"]
     step_bytes = substep_bytes(kbar)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(sc, args)
     cfg = {"workload": workload,
-           "n": sc.n, "mode": args.mode, "warm_start": "learned" if "learned" in runs else "none",
+           "n": sc.n, "mode": args.mode, "warm_start": "learned (auto)",
            "max_it": args.max_it, "tol": args.tol,
            "mean_iterations": round(kbar, 3),
            "max_density_error": round(head["max_density_error"], 6),
-           "converged_fraction": round(head["converged_fraction"], 4)}
-    if "plain" in runs and "learned" in runs:
-        cfg["plain"] = summary(runs["plain"])
-        cfg["learned"] = summary(runs["learned"])
-        cfg["speedup_learned_vs_plain"] = round(runs["learned"]["value"] / runs["plain"]["value"], 4)
-    if tol_pair:
-        cfg["to_tolerance"] = tol_pair
+           "converged_fraction": round(head["converged_fraction"], 4),
+           "plain": summary(plain), "learned": summary(head),
+           "speedup_learned_vs_plain": round(head["value"] / plain["value"], 4)}
+    if whole:
+        cfg["whole_run"] = {"substeps": args.run_steps, **whole}
     out = {
         "metric": METRIC,
         "value": round(head["value"], 1),
@@ -349,6 +368,7 @@ def main():
                              "frac": round(step_bytes * head["value"] / 1e9 / PEAK_HBM_GBS, 4)},
         "cpu_baseline": cpu,
         "kernels": head["kernels"],
+        "kernels_plain": plain["kernels"],
     }
     print(json.dumps(out))
 

@@ -20,7 +20,8 @@ HEADER = os.path.join(ROOT, "include", "bubbles_amd.h")
 OK = 0
 PCISPH, SPH = 0, 1
 MODE_REFERENCE_EXACT, MODE_TEXTBOOK = 0, 1
-WARM_NONE, WARM_MLP = 0, 1
+WARM_NONE, WARM_MLP, WARM_AUTO = 0, 1, 2
+WARM_AUTO_MIN_ITERATIONS = 3
 PROFILE_ALL_LAUNCHES = -1000
 BUCKET = 100
 NFEAT = 16
@@ -59,7 +60,8 @@ class Info(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("full_redistribution", C.c_int),
                 ("bucket_overflows", C.c_int), ("lost_particles", C.c_int),
-                ("max_density_error", C.c_double), ("gpu_ms", C.c_double)]
+                ("max_density_error", C.c_double), ("gpu_ms", C.c_double),
+                ("warm_started", C.c_int), ("pad", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

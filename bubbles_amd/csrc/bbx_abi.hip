@@ -1158,7 +1158,20 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     if (max_it < 1 || max_it > kMaxIt)
         return fail(BBX_E_INVALID, "max_it must be in [1, " + std::to_string(kMaxIt) + "]");
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
-    if (warm == BBX_WARM_MLP && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+    if (warm != BBX_WARM_NONE && warm != BBX_WARM_MLP && warm != BBX_WARM_AUTO)
+        return fail(BBX_E_INVALID, "unknown warm-start mode");
+    if (warm != BBX_WARM_NONE && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+    if (warm == BBX_WARM_AUTO) {
+        // The warm start costs about one correction iteration of extra work,
+        // so it only pays where the plain loop needs several.  After a plain
+        // substep it is used when that substep needed
+        // BBX_WARM_AUTO_MIN_ITERATIONS or more corrections; once used, it
+        // stays on until BBX_WARM_AUTO_PROBE warm substeps in a row have
+        // converged in one correction, and plain is tried again.
+        const bool on = s->last_warm ? s->warm_ones < BBX_WARM_AUTO_PROBE
+                                     : s->last_iterations >= BBX_WARM_AUTO_MIN_ITERATIONS;
+        warm = on ? BBX_WARM_MLP : BBX_WARM_NONE;
+    }
     BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
     BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
@@ -1214,9 +1227,12 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     bbx_stats local{};
     BBX_CHECK(read_stats(s, &local, launches));
     s->last_iterations = local.iterations;
+    s->last_warm = warm == BBX_WARM_MLP ? 1 : 0;
+    s->warm_ones = s->last_warm && local.iterations == 1 ? s->warm_ones + 1 : 0;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
     local.gpu_ms = ms;
+    local.warm_started = s->last_warm;
     if (stats) *stats = local;
     if (s->prof.enabled) s->prof.resolve(local.iterations);
     return BBX_OK;

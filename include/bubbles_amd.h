@@ -46,9 +46,19 @@ extern "C" {
 #define BBX_MODE_REFERENCE_EXACT 0
 #define BBX_MODE_TEXTBOOK 1
 
-/* pressure warm start */
+/* pressure warm start: NONE starts every substep from p = 0, MLP from the
+ * attached network's prediction.  AUTO uses the network only where it can
+ * pay for itself (it costs about one correction of extra work): after a
+ * plain substep that needed BBX_WARM_AUTO_MIN_ITERATIONS or more
+ * corrections it switches the network on, and it switches it off again
+ * after BBX_WARM_AUTO_PROBE warm substeps in a row converged in a single
+ * correction.  The correction loop and its tolerance are the same in every
+ * mode. */
 #define BBX_WARM_NONE 0
 #define BBX_WARM_MLP 1
+#define BBX_WARM_AUTO 2
+#define BBX_WARM_AUTO_MIN_ITERATIONS 3
+#define BBX_WARM_AUTO_PROBE 8
 
 /* downloadable / uploadable per-particle fields (by particle id) */
 #define BBX_FIELD_POSITION 0          /* 3 doubles */
@@ -109,6 +119,8 @@ typedef struct bbx_stats {
     int lost_particles;             /* particles outside every cell chain        */
     double max_density_error;       /* max compression error / rho0 (FP64)       */
     double gpu_ms;                  /* device time of the substep                */
+    int warm_started;               /* 1 if the warm-start network started it    */
+    int pad;
 } bbx_stats;
 
 /* ---- library ---- */

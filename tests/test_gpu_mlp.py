@@ -172,9 +172,9 @@ def test_warm_start_substep_matches_oracle(bbx, oracle):
 def test_learned_warm_start_reduces_iterations(bbx):
     """The shipped warm-start network (trained on seeds 1000-1031) cuts the
     mean predict-correct iteration count on a held-out dam break (seed 2000)
-    while the density error stays within the plain solver's envelope.
-    Measured on 1M particles: 7.15 -> 5.0 iterations per substep, max error
-    2.98% -> 2.87% (both runs capped at 8 iterations, tolerance 1%)."""
+    while the density error stays within the plain solver's envelope, on a
+    131,072-particle dam break (64x32x64 at 0.01 in a 1x0.5x1 tank, 240
+    substeps, both runs capped at 8 iterations, tolerance 1%)."""
     from bubbles_amd import learned
 
     net = learned.load()

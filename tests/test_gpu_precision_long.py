@@ -2,9 +2,10 @@
 spacing 0.01, h 0.02, 2x1x2 tank): FP32 + Kahan on the GPU against the FP64
 oracle from the same FP32 start.
 
-Opt-in (several minutes of oracle time on 16 host threads): set
-BBX_LONG_PRECISION=1, and BBX_PRECISION_OUT=<file> to write the tables as
-JSON.  Cases:
+The 4M-20 and learned 262k-200 cases run by default; the 1000-substep
+horizons and the drift study are opt-in (several minutes of oracle time on
+16 host threads each): set BBX_LONG_PRECISION=1, and
+BBX_PRECISION_OUT=<file> to write the tables as JSON.  Cases:
 
 * plain REFERENCE_EXACT (one iteration per substep, so both trajectories
   differ by arithmetic alone): 262,144, 1,048,576 and 4,194,304 particles
@@ -56,9 +57,11 @@ import pytest
 
 from bubbles_amd import scenes
 
-pytestmark = [pytest.mark.gpu, pytest.mark.slow,
-              pytest.mark.skipif(os.environ.get("BBX_LONG_PRECISION") != "1",
-                                 reason="long precision study: set BBX_LONG_PRECISION=1")]
+pytestmark = [pytest.mark.gpu]
+# the 1000-substep horizons and the drift study take several minutes of
+# oracle time each; the 4M-20 and learned 262k-200 cases run by default
+_long = pytest.mark.skipif(os.environ.get("BBX_LONG_PRECISION") != "1",
+                           reason="long precision study: set BBX_LONG_PRECISION=1")
 
 
 def _errors(g, o, h):
@@ -100,16 +103,16 @@ _EVERY_100 = (1, 10) + tuple(range(100, 1001, 100))
 @pytest.mark.parametrize("dims,tank,steps,checkpoints,bound", [
     pytest.param((64, 64, 64), (2.0, 1.0, 2.0), 1000, (1, 10, 100, 300, 1000),
                  {"density_rel_max": 0.5, "density_rel_mean": 3e-4, "fp_rel_max": 2.0, "fp_rel_mean": 2e-3,
-                  "pos_over_h_max": 1.5, "pos_over_h_mean": 3e-3}, id="262k-1000"),
+                  "pos_over_h_max": 1.5, "pos_over_h_mean": 3e-3}, id="262k-1000", marks=_long),
     pytest.param((256, 64, 256), (4.0, 1.0, 4.0), 20, (1, 10, 20),
                  {"density_rel_max": 1e-3, "density_rel_mean": 3e-5, "fp_rel_max": 3e-2, "fp_rel_mean": 3e-4,
                   "pos_over_h_max": 1e-3, "pos_over_h_mean": 3e-5}, id="4M-20"),
     pytest.param((128, 64, 128), (2.0, 1.0, 2.0), 1000, _EVERY_100,
                  {"density_rel_max": 1.0, "density_rel_mean": 2e-4, "fp_rel_max": 4.0, "fp_rel_mean": 1e-3,
-                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1.5e-3}, id="1M-1000"),
+                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1.5e-3}, id="1M-1000", marks=_long),
     pytest.param((256, 64, 256), (4.0, 1.0, 4.0), 1000, _EVERY_100,
                  {"density_rel_max": 1.0, "density_rel_mean": 1e-4, "fp_rel_max": 4.0, "fp_rel_mean": 5e-4,
-                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1e-3}, id="4M-1000"),
+                  "pos_over_h_max": 3.0, "pos_over_h_mean": 1e-3}, id="4M-1000", marks=_long),
 ])
 def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
     sc, g, o = _pair(bbx, oracle, dims, tank)
@@ -133,7 +136,7 @@ def test_plain_exact_long(bbx, oracle, dims, tank, steps, checkpoints, bound):
 
 @pytest.mark.timeout(1800)
 @pytest.mark.parametrize("dims,steps", [pytest.param((64, 64, 64), 200, id="262k-200"),
-                                        pytest.param((128, 64, 128), 1000, id="1M-1000")])
+                                        pytest.param((128, 64, 128), 1000, id="1M-1000", marks=_long)])
 def test_learned_exact_against_fp64(bbx, oracle, dims, steps):
     from bubbles_amd import learned
 
@@ -163,6 +166,7 @@ def test_learned_exact_against_fp64(bbx, oracle, dims, steps):
     assert max(table[1]["density_rel_max"], table[1]["pos_over_h_max"]) <= 1.5e-4
 
 
+@_long
 @pytest.mark.timeout(1800)
 def test_textbook_iteration_drift(bbx, oracle):
     sc, g, o = _pair(bbx, oracle, (64, 64, 64))

@@ -27,22 +27,38 @@ def test_xsph_matches_oracle(bbx, oracle):
     """pseudoViscosity 1000 at dt 1.6e-4 gives 0.16 > 0.1, so XSPH runs
     (src/equations/sph_equations3.cpp:341-382,655-664): after each substep
     the velocity is blended with the poly-normalised spiky average of the
-    neighbours' velocities.  Stated bound: 5e-6 m in position and 5e-4 m/s
-    in velocity after three substeps (the velocities are ~0.05 m/s)."""
-    sc = sph_block(4096)
+    neighbours' velocities.  The block is lifted off the tank's walls (a
+    particle resting exactly one collision radius from a wall flips its
+    contact decision between FP32 and FP64 and dominates any comparison) and
+    given a smooth velocity field of 0.05 m/s, so the smoothing moves the
+    velocities by 2.3e-3 m/s in ten substeps (oracle with and without it).
+    Stated bounds against the oracle: 1e-6 m/s in velocity, 2e-7 m in
+    position, and the device's smoothing effect equals the oracle's within
+    1e-6 m/s."""
+    base = sph_block(4096)
+    pos = (base.positions + 0.3).astype(np.float32).astype(np.float64)
+    L = 0.32
+    vel = 0.05 * np.stack([np.sin(2 * np.pi * pos[:, 1] / L), np.cos(2 * np.pi * pos[:, 2] / L),
+                           np.sin(2 * np.pi * pos[:, 0] / L)], axis=1)
+    sc = dataclasses.replace(base, positions=pos, velocities=vel.astype(np.float32).astype(np.float64))
     g = bbx.Solver(sc, kind=bbx.SPH, pseudo_viscosity=1000.0)
-    o = oracle.Sim(dataclasses.replace(sc, positions=g.get("position")), pseudo_viscosity=1000.0)
-    dt = 1.6e-4
-    for _ in range(3):
-        g.sph_substep(dt)
-        o.sph_substep(dt, oracle.SPH_TWO_PHASE)
-    np.testing.assert_allclose(g.get("position"), o.get("position"), rtol=0, atol=5e-6)
-    np.testing.assert_allclose(g.get("velocity"), o.get("velocity"), rtol=0, atol=5e-4)
-    # the smoothing is active: the same run without it ends elsewhere
     h = bbx.Solver(sc, kind=bbx.SPH)
-    for _ in range(3):
+    o = oracle.Sim(sc, pseudo_viscosity=1000.0)
+    p = oracle.Sim(sc)
+    dt = 1.6e-4
+    for _ in range(10):
+        g.sph_substep(dt)
         h.sph_substep(dt)
-    assert np.abs(h.get("velocity") - g.get("velocity")).max() > 10 * 5e-4
+        o.sph_substep(dt, oracle.SPH_TWO_PHASE)
+        p.sph_substep(dt, oracle.SPH_TWO_PHASE)
+    gv, ov = g.get("velocity"), o.get("velocity")
+    dv = np.abs(gv - ov).max()
+    dx = np.abs(g.get("position") - o.get("position")).max()
+    effect = np.abs(ov - p.get("velocity")).max()
+    deff = np.abs((gv - h.get("velocity")) - (ov - p.get("velocity"))).max()
+    print(f"XSPH: |dv| {dv:.3e} |dx| {dx:.3e} effect {effect:.3e} |d effect| {deff:.3e}")
+    assert effect > 1e-3, effect
+    assert dv <= 1e-6 and dx <= 2e-7 and deff <= 1e-6, (dv, dx, deff)
 
 
 def test_sph_advance_substeps_and_state_match_oracle(bbx, oracle):
