@@ -47,12 +47,11 @@ KERNEL_BYTES = {
     "grid_rank_move": 76,              # slot, x+id, v+p -> x+id, v+p, cell, home
     "tile_tables": 4,                  # home -> per-tile interval tables
     "buckets_density": 20,             # x -> rho, count; + 2 B per bucket entry (see below)
+    "buckets_density_warm": 64,        # x, v -> rho, count, p0, p_warm, rho*, (x, q0); + bucket
     "nonpressure_predict": 56,         # x, v, rho -> f, x*, rho*
-    "nonpressure_features": 108,       # x, v, rho -> f, rho*, 16 features
-    "nonpressure_features_warm": 120,  # x, v, rho -> f, rho*, 16 features, p_warm, p, q
+    "nonpressure_warm": 60,            # x, q0, v, rho -> f, x*, rho*
     "mlp_infer": 68,                   # 16 features -> p
     "predict_pressure": 28,            # x*, p -> p, rho*, q
-    "pressure_force_warm": 52,         # x, q, v, f -> x*
     "pressure_force_predict": 52,      # x, q, v, f -> x*
     "pressure_force_integrate": 96,    # x, q, v, f, p -> x, v+p, f, f_p
 }
@@ -63,7 +62,7 @@ def algorithmic_bytes(name: str, mean_bucket: float):
     """Bytes per particle of `name`; the bucket build also writes the bucket
     itself (the reference's Bucket output), 2 B per entry at the mean count."""
     b = KERNEL_BYTES.get(name)
-    if b is not None and name == "buckets_density":
+    if b is not None and name.startswith("buckets_density"):
         b = b + 2.0 * mean_bucket
     return b
 

@@ -133,6 +133,7 @@ def load():
         "bbx_mlp_gradients": (i, [vp, fp, fp, i, fp, fp]),
         "bbx_mlp_train_step": (i, [vp, fp, fp, i, C.c_float, fp]),
         "bbx_mlp_train": (i, [vp, fp, fp, i, i, i, C.c_float, i, fp]),
+        "bbx_solver_keep_features": (i, [vp, i]),
         "bbx_solver_attach_mlp": (i, [vp, vp]),
         "bbx_pcisph_harvest": (i, [vp, d, i, d, fp, fp, C.POINTER(Stats)]),
         "bbx_profile_enable": (i, [vp, i]),
@@ -542,6 +543,10 @@ class Solver:
     def attach_mlp(self, mlp):
         check(load().bbx_solver_attach_mlp(self.h, mlp.h if mlp else None), "bbx_solver_attach_mlp")
         self._mlp = mlp  # the network object must outlive its use by this solver
+
+    def keep_features(self, keep=True):
+        """Store the features of warm-started substeps (get("features"))."""
+        check(load().bbx_solver_keep_features(self.h, 1 if keep else 0), "bbx_solver_keep_features")
 
     def harvest(self, dt, max_it=8, tol=0.01):
         feats = np.zeros((self.n, NFEAT), dtype=np.float32)

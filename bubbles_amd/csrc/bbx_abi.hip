@@ -1106,7 +1106,7 @@ int bbx_stage_density(bbx_solver *s) {
 int bbx_stage_nonpressure(bbx_solver *s) {
     BBX_CHECK(check_setup(s));
     StepConst st{};
-    BBX_CHECK(run_nonpressure(s, st, 0, 0));
+    BBX_CHECK(run_nonpressure(s, st, 0));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
     return BBX_OK;
 }
@@ -1162,36 +1162,44 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
         return fail(BBX_E_INVALID, "unknown warm-start mode");
     if (warm != BBX_WARM_NONE && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
     if (warm == BBX_WARM_AUTO) {
-        // The warm start costs about one correction iteration of extra work,
-        // so it only pays where the plain loop needs several.  After a plain
-        // substep it is used when that substep needed
-        // BBX_WARM_AUTO_MIN_ITERATIONS or more corrections; once used, it
-        // stays on until BBX_WARM_AUTO_PROBE warm substeps in a row have
-        // converged in one correction, and plain is tried again.
-        const bool on = s->last_warm ? s->warm_ones < BBX_WARM_AUTO_PROBE
-                                     : s->last_iterations >= BBX_WARM_AUTO_MIN_ITERATIONS;
+        // The warm start costs about two thirds of a correction, so it only
+        // pays where the plain loop needs several.  While the flow is calm
+        // it is switched on after a plain substep that needed
+        // BBX_WARM_AUTO_MIN_ITERATIONS or more corrections and off again
+        // after BBX_WARM_AUTO_PROBE warm substeps in a row converged in one.
+        // Once any substep needed BBX_WARM_AUTO_ACTIVE or more the flow is
+        // active and it stays on: a warm-started loop that follows a plain
+        // substep starts from the plain loop's lower converged pressures and
+        // needs more corrections than one that follows a warm substep.
+        bool on;
+        if (s->warm_active)
+            on = true;
+        else
+            on = s->last_warm ? s->warm_ones < BBX_WARM_AUTO_PROBE
+                              : s->last_iterations >= BBX_WARM_AUTO_MIN_ITERATIONS;
         warm = on ? BBX_WARM_MLP : BBX_WARM_NONE;
     }
     BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
     BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
     BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
-    // UpdateGridDistribution + ResetHigherLevel + interaction + density
-    BBX_CHECK(distribute(s, false, true, false));
     const double delta = compute_delta(s, dt);
     s->delta64 = delta;
     StepConst st{};
     st.dt = (float)dt;
     st.dt_over_m = (float)(dt / s->mass);
     st.delta = (float)delta;
+    st.warm_scale = (float)(delta * s->prm.target_density);
     if (warm == BBX_WARM_MLP) {
-        // features, network inference and p = p_warm, rho* = rho in one gather
-        st.warm_scale = (float)(delta * s->prm.target_density);
-        BBX_CHECK(run_nonpressure(s, st, 0, 2));
-        // f_p(p_warm, rho) and the k = 0 prediction
-        BBX_CHECK(run_pressure_force(s, st, -1, max_it, 0, tol));
+        // UpdateGridDistribution + interaction + density, with the features
+        // and the network's pressures p0 in the bucket kernel; then the
+        // viscosity gather also sums f_p(p0, rho* = rho) and predicts k = 0
+        BBX_CHECK(distribute(s, false, true, false, -1, 2, &st));
+        BBX_CHECK(run_nonpressure(s, st, 2));
     } else {
-        BBX_CHECK(run_nonpressure(s, st, 1, 0));
+        // UpdateGridDistribution + ResetHigherLevel + interaction + density
+        BBX_CHECK(distribute(s, false, true, false));
+        BBX_CHECK(run_nonpressure(s, st, 1));
     }
     // The predict-correct loop is queued without per-iteration host round
     // trips: the pressure-force kernel of iteration k reads that iteration's
@@ -1229,6 +1237,7 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     s->last_iterations = local.iterations;
     s->last_warm = warm == BBX_WARM_MLP ? 1 : 0;
     s->warm_ones = s->last_warm && local.iterations == 1 ? s->warm_ones + 1 : 0;
+    if (local.iterations >= BBX_WARM_AUTO_ACTIVE) s->warm_active = 1;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
     local.gpu_ms = ms;
@@ -1311,6 +1320,12 @@ int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m) {
     return BBX_OK;
 }
 
+int bbx_solver_keep_features(bbx_solver *s, int keep) {
+    BBX_CHECK(check_solver(s));
+    s->keep_features = keep ? 1 : 0;
+    return BBX_OK;
+}
+
 int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,
                        float *targets, bbx_stats *stats) {
     BBX_CHECK(check_setup(s));
@@ -1321,13 +1336,12 @@ int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *
     // The features are computed on a throwaway pass that leaves the state
     // untouched (distribute is idempotent for unchanged positions).
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(distribute(s, false, true, false));
     const double delta = compute_delta(s, dt);
     StepConst st{};
     st.dt = (float)dt;
     st.dt_over_m = (float)(dt / s->mass);
     st.delta = (float)delta;
-    BBX_CHECK(run_nonpressure(s, st, 0, 1));
+    BBX_CHECK(distribute(s, false, true, false, -1, 1, &st));
     std::vector<int> ids;
     BBX_CHECK(slot_ids(s, ids));
     std::vector<float> h(n * BBX_NFEAT);

@@ -197,6 +197,7 @@ struct bbx_solver {
     int last_iterations = 0;  // iterations of the previous PCISPH substep
     int last_warm = 0;        // whether it started from the warm-start network
     int warm_ones = 0;        // warm substeps in a row that converged in one correction
+    int warm_active = 0;      // a substep has needed BBX_WARM_AUTO_ACTIVE or more corrections
     bbx::GridGeom grid{};
     bbx::SimConst sc{};
     hipStream_t stream = nullptr;
@@ -229,6 +230,7 @@ struct bbx_solver {
     float4 *force = nullptr, *fp = nullptr, *xstar = nullptr;
     float *feat = nullptr;       // [16][stride]
     float *pwarm = nullptr;      // MLP output
+    int keep_features = 0;       // warm substeps also store their features (bbx_solver_keep_features)
     // reductions / flags
     bbx::LoopState *loop = nullptr;          // predict-correct loop state (device)
     bbx::LoopState *h_loop = nullptr;        // pinned readback
@@ -261,11 +263,16 @@ int launch_fill_int(bbx_solver *s, int *p, int value, size_t n);
 int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
 // append_from >= 0: the append pass after new particles were placed in
 // slots append_from..n-1 (old chains kept, new particles at their tails)
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from = -1);
+// feat_mode (with build_buckets): 0 none; 1 the bucket kernel also computes
+// the warm-start features (stored in feat); 2 it also infers the attached
+// network and starts the substep from its pressures (p0 into pres and pwarm,
+// (x, p0 / rho^2) into xq, rho* = rho); st carries dt, delta and the scale
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from = -1,
+               int feat_mode = 0, const StepConst *st = nullptr);
 int run_density_only(bbx_solver *s);
-// features: 0 none, 1 warm-start features, 2 features + network inference +
-// the warm k = 0 initialisation in the same kernel
-int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features);
+// mode: 0 force only, 1 force + predict(k = 0), 2 force + the warm start's
+// pressure force + predict(k = 0) (the bucket kernel inferred p0)
+int run_nonpressure(bbx_solver *s, const StepConst &st, int mode);
 int run_predict_pressure(bbx_solver *s, const StepConst &st, int k);
 // k < 0: predict only (warm start); otherwise integrate when iteration k is
 // converged (or k == max_it - 1, or exact), else predict iteration k + 1

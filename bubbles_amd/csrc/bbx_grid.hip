@@ -18,6 +18,7 @@
 // (DistributeToCellOpt) concatenates the 27 neighbour cells' old chains in
 // (y,x,z) order, i.e. key = (direction of the old cell seen from the new cell,
 // old storage slot), since storage is the old chain order.
+#include "bbx_mlp_device.h"
 #include "bbx_tile.h"
 
 namespace bbx {
@@ -331,7 +332,23 @@ __global__ void __launch_bounds__(kBlock) k_tile_tables(SimConst c, const int *_
 // tile is written as LDS-local indices (nbr16), otherwise as storage indices
 // (nbr32).  Candidates are visited in the reference's order: neighbour cells
 // y outer, x, z inner (src/core/grid.h:568-591), each cell in chain order.
-template <bool kPressure>
+// The learned warm start fused into the bucket kernel (kFeat > 0): the
+// features of include/bbx_features.h come from the finished bucket, the
+// density and the particle's own state, so they need nothing another
+// particle computes in this launch.  kFeat == 2 then infers the network on
+// the workgroup's 256 particles and starts the substep from its pressures.
+struct WarmIn {
+    StepConst st;
+    const float4 *__restrict__ vel;       // own velocity, w = previous converged pressure
+    int store_feat;                       // write the features to feat
+    float *__restrict__ feat;             // [16][stride]
+    const float *__restrict__ P;          // network parameters (kFeat == 2)
+    float *__restrict__ pwarm;            // network pressure
+    float *__restrict__ rho_star;         // rho* = rho at k = 0
+    float4 *__restrict__ xq;              // (x, p0 / rho^2)
+};
+
+template <bool kPressure, int kFeat>
 __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             const float4 *__restrict__ pos,
                                                             const int *__restrict__ home,
@@ -343,7 +360,8 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             float *__restrict__ rho,
                                                             float *__restrict__ pres,
                                                             const int *__restrict__ tile_tab,
-                                                            int *__restrict__ counters) {
+                                                            int *__restrict__ counters,
+                                                            WarmIn wi) {
     // staged candidate positions (16-byte LDS DMA); four slack slots let the
     // candidate test read whole groups of four past the end of a row
     __shared__ float4 sx[kTileCap + 4];
@@ -397,7 +415,9 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         lds_dma_wait();
     }
     __syncthreads();
-    if (!valid) return;
+    if (kFeat != 2 && !valid) return;  // the inference below needs the whole block at its barriers
+    float fv[kFeat ? BBX_NFEAT : 1];
+    float di = 0.f;
     // The particle's work is instantiated once for LDS-staged and once for
     // global-memory tiles, so no load selects between the two address spaces
     // (that would compile to a slower generic load).
@@ -405,6 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         constexpr bool kStaged = decltype(staged_tag)::value;
         int cnt = 0, dropped = 0;
         float sum = 0.f, comp = 0.f;
+        float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;  // features
         // The current quad of bucket entries.  Staged: a 64-bit shift
         // register of 16-bit entries held as two words (entry k of the quad
         // ends in bits 16k..16k+15); global: four 32-bit entries.
@@ -548,15 +569,63 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         // the table this lane has just written (its own quads: no other lane
         // or wave writes them, and no earlier load in this kernel cached them)
         __builtin_amdgcn_s_waitcnt(0);
+        // the particle's own index in the bucket (its LDS slot in the centre row)
+        const int self = kStaged ? i + tt.rdelta[4] : i;
         for_each_neighbour<kNbrDepth>(nbr16, nbr32, kStaged, i, cnt, c.stride, [&](int j, bool live) {
             const float4 pj = kStaged ? sx[j] : pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float w = poly6_w(c, d2);
             kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
+            if constexpr (kFeat > 0) {
+                // distance moments and centroid over the members j != i
+                const bool fl = live & (j != self);
+                const float u = keep_if(fl, fast_sqrt(d2) * c.inv_h);
+                m1 += u;
+                m2 += u * u;
+                m3 += u * u * u;
+                cx -= keep_if(fl, dx * c.inv_h);
+                cy -= keep_if(fl, dy * c.inv_h);
+                cz -= keep_if(fl, dz * c.inv_h);
+            }
         });
         nbr_cnt[i] = cnt;
-        float di = sum * c.mass;
+        di = sum * c.mass;
         rho[i] = di;
+        if constexpr (kFeat > 0) {
+            const float4 vi = wi.vel[i];
+            const float inv = 1.f / (cnt > 1 ? (float)(cnt - 1) : 1.f);
+            float wall = 3.4e38f;
+            const float p3[3] = {pi.x, pi.y, pi.z};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float lo = p3[a] - (c.tank_c[a] - c.tank_h[a]);
+                const float hi = (c.tank_c[a] + c.tank_h[a]) - p3[a];
+                wall = fminf(wall, fminf(lo, hi));
+            }
+            const float vs = wi.st.dt * c.inv_h;
+            fv[BBX_F_DENSITY_ERR] = di / c.rho0 - 1.f;
+            fv[BBX_F_RESERVED1] = 0.f;
+            fv[BBX_F_COUNT] = (float)cnt;
+            fv[BBX_F_M1] = m1 * inv;
+            fv[BBX_F_M2] = m2 * inv;
+            fv[BBX_F_M3] = m3 * inv;
+            fv[BBX_F_CX] = cx * inv;
+            fv[BBX_F_CY] = cy * inv;
+            fv[BBX_F_CZ] = cz * inv;
+            fv[BBX_F_CNORM] = sqrtf(cx * cx + cy * cy + cz * cz) * inv;
+            fv[BBX_F_PPREV] = wi.st.delta != 0.f ? vi.w / (wi.st.delta * c.rho0) : 0.f;
+            fv[BBX_F_VY] = vi.y * vs;
+            fv[BBX_F_VNORM] = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * vs;
+            fv[BBX_F_RESERVED2] = 0.f;
+            fv[BBX_F_RESERVED3] = 0.f;
+            fv[BBX_F_WALL] = fminf(fmaxf(wall * c.inv_h, 0.f), 4.f);
+            if (kFeat == 1 || wi.store_feat) {
+                const size_t S = (size_t)c.stride;
+#pragma unroll
+                for (int f = 0; f < BBX_NFEAT; f++) wi.feat[(size_t)f * S + i] = fv[f];
+            }
+        }
         if (kPressure) {
             // ComputePressureValue (src/equations/sph_equations3.cpp:7-18)
             float p = c.eos_scale / c.eos_exp * (powf(di / c.rho0, c.eos_exp) - 1.f);
@@ -565,7 +634,71 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         }
         if (dropped) atomicAdd(&counters[CNT_OVERFLOW], dropped);
     };
-    with_staging(staged, per_particle);
+    if (valid) with_staging(staged, per_particle);
+    if constexpr (kFeat == 2) {
+        // The block's 256 feature rows go through LDS (over the staged
+        // candidates, which every thread has finished with) into the MFMA
+        // layout; each wave infers its 64 particles as four 16-sample tiles.
+        constexpr int kRow = BBX_NFEAT + 1;  // padded feature row
+        float *lf = reinterpret_cast<float *>(sx);  // 256 rows
+        float *lp = lf + kBlock * kRow;             // 256 warm-start pressures
+        float *lb = lp + kBlock;                    // b1 | b2 | w3
+        unsigned short *lw = reinterpret_cast<unsigned short *>(lb + 3 * kH);  // BF16 weights of both layers
+        static_assert((kBlock * (kRow + 1) + 3 * kH) * 4 + kLdsWeights * 2 <= (int)sizeof(float4) * kTileCap &&
+                          (kBlock * (kRow + 1) + 3 * kH) % 4 == 0,
+                      "warm-start LDS layout over the staged candidates");
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < BBX_NFEAT; f++) lf[threadIdx.x * kRow + f] = valid ? fv[f] : 0.f;
+        stage_infer_weights(wi.P, lw, lb, (int)threadIdx.x, kBlock);
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int qq = lane >> 4, jj = lane & 15;
+        bf16x4 w1[4], w2[4][4];
+        load_weight_operands_lds(lw, lane, w1, w2);
+        float mean[4], istd[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            mean[v] = wi.P[OFF_MEAN + 4 * qq + v];
+            istd[v] = wi.P[OFF_ISTD + 4 * qq + v];
+        }
+        const float b3 = wi.P[OFF_B3], oscale = wi.P[OFF_OSCALE], alpha = wi.P[OFF_ALPHA];
+        const float pcap = wi.P[OFF_PCAP], resid = wi.P[OFF_RESID];
+#pragma unroll
+        for (int half = 0; half < 2; half++) {  // two pairs of tiles keep the registers below 128
+            float xv[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int smp = wv * 64 + (2 * half + t) * 16 + jj;
+#pragma unroll
+                for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
+            }
+            float yn[2];
+            forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, mean, istd, qq, yn);
+            if (lane < 16) {
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int smp = wv * 64 + (2 * half + t) * 16 + jj;
+                    // the same expression as the standalone inference kernel
+                    float y = yn[t] * oscale;
+                    if (resid != 0.f) y += resid * lf[smp * kRow + BBX_F_PPREV];
+                    y = alpha * fminf(fmaxf(y, 0.f), pcap);
+                    lp[smp] = y * wi.st.warm_scale;
+                }
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            // PredictVelocityAndPositionFor at k = 0 (src/equations/pcisph_equations3.cpp:3-28)
+            // with the network's pressure instead of 0: rho* = rho, p = p0
+            const float p = lp[threadIdx.x];
+            const float d2 = di * di;
+            pres[i] = p;
+            wi.pwarm[i] = p;
+            wi.rho_star[i] = di;
+            wi.xq[i] = make_float4(pi.x, pi.y, pi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
+        }
+    }
 }
 
 // density over the existing buckets (after an upload of positions)
@@ -603,7 +736,11 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // UpdateGridDistribution: full or incremental chain update, then buckets.
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from) {
+int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from,
+               int feat_mode, const StepConst *st) {
+    if (feat_mode && (!build_buckets || with_pressure || !st))
+        return fail(BBX_E_INVALID, "warm-start features need a PCISPH bucket build and step constants");
+    if (feat_mode == 2 && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
     SimConst &c = s->sc;
     const int n = s->n;
     const size_t cells = (size_t)c.g.total + 1;
@@ -655,15 +792,31 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
             BBX_HIP_CHECK(hipGetLastError());
             BBX_PROF_END();
         }
-        BBX_PROF_BEGIN("buckets_density");
+        WarmIn wi{};
+        if (st) wi.st = *st;
+        wi.vel = s->velA;
+        wi.store_feat = s->keep_features;
+        wi.feat = s->feat;
+        wi.P = feat_mode == 2 ? mlp_params_device(s->mlp) : nullptr;
+        wi.pwarm = s->pwarm;
+        wi.rho_star = s->rho_star;
+        wi.xq = s->xq;
+        BBX_PROF_BEGIN(feat_mode == 2 ? "buckets_density_warm"
+                                      : (feat_mode == 1 ? "buckets_density_features" : "buckets_density"));
+#define BBX_BUCKETS(P, F)                                                                                   \
+    k_buckets_density<P, F><<<nblocks(n), kBlock, 0, s->stream>>>(c, s->posA, s->home, s->cell_start,       \
+                                                                   s->cell_count, s->nbr16, s->nbr32,      \
+                                                                   s->nbr_cnt, s->rho, s->pres,            \
+                                                                   s->tile_tab, s->counters, wi)
         if (with_pressure)
-            k_buckets_density<true><<<nblocks(n), kBlock, 0, s->stream>>>(
-                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
-                s->rho, s->pres, s->tile_tab, s->counters);
+            BBX_BUCKETS(true, 0);
+        else if (feat_mode == 2)
+            BBX_BUCKETS(false, 2);
+        else if (feat_mode == 1)
+            BBX_BUCKETS(false, 1);
         else
-            k_buckets_density<false><<<nblocks(n), kBlock, 0, s->stream>>>(
-                c, s->posA, s->home, s->cell_start, s->cell_count, s->nbr16, s->nbr32, s->nbr_cnt,
-                s->rho, s->pres, s->tile_tab, s->counters);
+            BBX_BUCKETS(false, 0);
+#undef BBX_BUCKETS
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     }

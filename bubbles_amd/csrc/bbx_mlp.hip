@@ -61,7 +61,7 @@ struct Acts {
 
 // forward of one 16-sample tile; x4 = raw features 4q..4q+3 of sample l%16
 __device__ __forceinline__ void forward_tile(const Weights &w, const float x4[4], Acts &a, int lane) {
-    for (int v = 0; v < 4; v++) a.xn[v] = (x4[v] - w.mean[v]) * w.istd[v];
+    for (int v = 0; v < 4; v++) a.xn[v] = norm_feature(x4[v], w.mean[v], w.istd[v]);
     bf16x4 xb = pack4(a.xn[0], a.xn[1], a.xn[2], a.xn[3]);
     bf16x4 h1b[4];
     for (int m = 0; m < 4; m++) {

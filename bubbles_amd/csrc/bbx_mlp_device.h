@@ -32,6 +32,15 @@ constexpr int OFF_PCAP = OFF_ALPHA + 1;
 constexpr int OFF_RESID = OFF_PCAP + 1;   // prediction = net * out_scale + resid * x[PPREV]
 constexpr int kParams = OFF_RESID + 1;    // 5349
 
+// Normalised features are clipped to +-kFeatClip standard deviations: rare
+// states far outside the training data (an impact, a splash) then enter the
+// network at the edge of what it was trained on instead of as extreme inputs
+// it would extrapolate from.
+constexpr float kFeatClip = 8.f;
+__device__ __forceinline__ float norm_feature(float x, float mean, float istd) {
+    return fminf(fmaxf((x - mean) * istd, -kFeatClip), kFeatClip);
+}
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -152,6 +161,28 @@ __device__ __forceinline__ void store_infer_prefetch(const InferPrefetch &f, uns
     }
 }
 
+// The same LDS image written by a block of any size straight from the FP32
+// parameters (each weight rounded to BF16 once): W1, W2, then b1 | b2 | w3.
+__device__ __forceinline__ void stage_infer_weights(const float *__restrict__ P, unsigned short *lw, float *lb,
+                                                    int tid, int nthreads) {
+    const float4 *w1 = reinterpret_cast<const float4 *>(P + OFF_W1);
+    const float4 *w2 = reinterpret_cast<const float4 *>(P + OFF_W2);
+    unsigned short *lw2 = lw + kH * kW1Row;
+    for (int k = tid; k < kH * kF / 4; k += nthreads) {
+        const float4 v = w1[k];
+        *reinterpret_cast<bf16x4 *>(lw + (k / (kF / 4)) * kW1Row + 4 * (k % (kF / 4))) = pack4(v.x, v.y, v.z, v.w);
+    }
+    for (int k = tid; k < kH * kH / 4; k += nthreads) {
+        const float4 v = w2[k];
+        *reinterpret_cast<bf16x4 *>(lw2 + (k / (kH / 4)) * kW2Row + 4 * (k % (kH / 4))) = pack4(v.x, v.y, v.z, v.w);
+    }
+    for (int k = tid; k < kH; k += nthreads) {
+        lb[k] = P[OFF_B1 + k];
+        lb[kH + k] = P[OFF_B2 + k];
+        lb[2 * kH + k] = P[OFF_W3 + k];
+    }
+}
+
 // MFMA weight operands of lane `lane` from the staged LDS copy
 __device__ __forceinline__ void load_weight_operands_lds(const unsigned short *lw, int lane, bf16x4 (&w1)[4],
                                                          bf16x4 (&w2)[4][4]) {
@@ -179,8 +210,8 @@ __device__ __forceinline__ void forward_infer_tiles(const bf16x4 (&w1)[4], const
     bf16x4 xb[T];
 #pragma unroll
     for (int t = 0; t < T; t++)
-        xb[t] = pack4((x[t][0] - mean[0]) * istd[0], (x[t][1] - mean[1]) * istd[1],
-                      (x[t][2] - mean[2]) * istd[2], (x[t][3] - mean[3]) * istd[3]);
+        xb[t] = pack4(norm_feature(x[t][0], mean[0], istd[0]), norm_feature(x[t][1], mean[1], istd[1]),
+                      norm_feature(x[t][2], mean[2], istd[2]), norm_feature(x[t][3], mean[3], istd[3]));
     bf16x4 h1b[T][4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {

@@ -86,10 +86,10 @@ __device__ __forceinline__ void integrate_one(const SimConst &c, float dt, float
 // ---------------------------------------------------------------------------
 // non-pressure force (src/equations/sph_equations3.cpp:80-110)
 // mode: 0 = store force only, 1 = + predict(k=0) with f_p = 0,
-//       2 = + warm-start features, 3 = + features, warm-start network
-//       inference and the k = 0 initialisation (p = p_warm, rho* = rho)
-// LDS: neighbour position, (velocity, density); in mode 3 the same LDS then
-// holds the block's features for the MFMA inference.
+//       2 = + the warm start's pressure force f_p(p0) at rho* = rho, whose
+//           (x, q0 = p0 / rho^2) the bucket kernel wrote to xq, and
+//           predict(k=0) with it
+// LDS: neighbour position (mode 2: with q0), (velocity, density).
 //
 // Two lanes per particle: the two LDS arrays allow only two workgroups per
 // CU, so each particle's bucket is split between lanes 2p and 2p + 1 (even
@@ -103,20 +103,13 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
                                                             const float4 *__restrict__ pos,
                                                             const float4 *__restrict__ vel,
                                                             const float *__restrict__ rho,
+                                                            const float4 *__restrict__ xq,
                                                             float4 *__restrict__ force,
                                                             float4 *__restrict__ xstar,
-                                                            float *__restrict__ rho_star,
-                                                            float *__restrict__ feat,
-                                                            const float *__restrict__ P,
-                                                            float *__restrict__ pres,
-                                                            float4 *__restrict__ xq,
-                                                            float *__restrict__ pwarm) {
-    constexpr bool kFeat = kMode >= 2;
-    // neighbour (position) and (velocity, density) as float4 arrays; in mode
-    // 3 the same pool then holds the block's features
-    __shared__ float pool[8 * kTileCap2];
-    float4 *const sx = reinterpret_cast<float4 *>(pool);
-    float4 *const sv = reinterpret_cast<float4 *>(pool + 4 * kTileCap2);
+                                                            float *__restrict__ rho_star) {
+    constexpr bool kWarm = kMode == 2;
+    __shared__ float4 sx[kTileCap2];
+    __shared__ float4 sv[kTileCap2];
     __shared__ TileTab tt;
     const int tile = tile_of_block();
     tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
@@ -128,7 +121,9 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
     const bool valid = i < c.n;
     int cnt = 0;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
-    float rhoi = 0.f;
+    float rhoi = 0.f, qi = 0.f;
+    // (x, q0) in the warm mode, else the positions
+    const float4 *__restrict__ sxsrc = kWarm ? xq : pos;
     // this lane's first two quads: half, half + 2
     NbrQuads<2> first{};
     if (valid) {  // independent of the staging: requested before it
@@ -136,198 +131,91 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
         xi = pos[i];
         vi = vel[i];
         rhoi = rho[i];
+        if (kWarm) qi = xq[i].w;
         first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
     }
     // staged through registers: the (velocity, density) pair is assembled on
     // the way, and seven 4-byte LDS DMA streams measured slower here
     if (in_lds) {
-        tab_stage(tt, sx, [&](int j) { return pos[j]; });
+        tab_stage(tt, sx, [&](int j) { return sxsrc[j]; });
         tab_stage(tt, sv, [&](int j) {
             const float4 v = vel[j];
             return make_float4(v.x, v.y, v.z, rho[j]);
         });
     }
     __syncthreads();
-    if (kMode != 3 && !valid) return;  // the lane pair shares validity; mode 3 keeps the block whole
-    // mode 3: the network's parameters are requested here, so that their
-    // latency hides behind the neighbour loop
-    InferPrefetch net;
-    if constexpr (kMode == 3) prefetch_infer(P, threadIdx.x, net);
-    float fv[BBX_NFEAT];
-    {
-        const int self = staged && valid ? tab_local(tt, i) : i;
-        const float coef = c.visc * c.mass2;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        float m1 = 0.f, m2 = 0.f, m3 = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, divv = 0.f;
-        with_staging(in_lds, [&](auto tag) {
-            constexpr bool kL = decltype(tag)::value;
-            auto body = [&](int j, bool live) {
-                float xjx, xjy, xjz, vjx, vjy, vjz, rj;
-                if (kL) {
-                    const float4 a = sx[j], b = sv[j];
-                    xjx = a.x;
-                    xjy = a.y;
-                    xjz = a.z;
-                    vjx = b.x;
-                    vjy = b.y;
-                    vjz = b.z;
-                    rj = b.w;
-                } else {
-                    const int g = staged ? tab_global(tt, j) : j;
-                    const float4 p = pos[g], v = vel[g];
-                    xjx = p.x;
-                    xjy = p.y;
-                    xjz = p.z;
-                    vjx = v.x;
-                    vjy = v.y;
-                    vjz = v.z;
-                    rj = rho[g];
-                }
-                const float dx = xi.x - xjx, dy = xi.y - xjy, dz = xi.z - xjz;
-                const float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
-                // masks instead of branches: every pair's arithmetic runs, a
-                // dead pair contributes exact zeros, and the neighbour's LDS
-                // reads stay unconditional (so they are issued early)
-                const bool in = live & spiky_in(c, r);
-                const float x = keep_if(in, 1.f - r * c.inv_h);
-                const float irj = fast_rcp(rj);
-                // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j
-                const float s = coef * (c.spiky_lap * x) * irj;
-                const float dvx = vjx - vi.x, dvy = vjy - vi.y, dvz = vjz - vi.z;
-                fx += dvx * s;
-                fy += dvy * s;
-                fz += dvz * s;
-                if constexpr (kFeat) {
-                    const bool fl = live & (j != self);
-                    const float u = keep_if(fl, r * c.inv_h);
-                    m1 += u;
-                    m2 += u * u;
-                    m3 += u * u * u;
-                    cx -= keep_if(fl, dx * c.inv_h);
-                    cy -= keep_if(fl, dy * c.inv_h);
-                    cz -= keep_if(fl, dz * c.inv_h);
-                    // grad_i W = dW/dr (x_i - x_j)/r with dW/dr = -spiky_grad x^2
-                    const float g = -c.spiky_grad * x * x * fast_rcp(r);
-                    divv += keep_if(fl & (r > 0.f) & (x > 0.f), c.mass * irj * g * (dvx * dx + dvy * dy + dvz * dz));
-                }
-            };
-            // quads half, half + 2, half + 4, ... of the bucket, two in flight
-            for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, body, half);
-        });
-        // the pair's partial sums
-        fx += __shfl_xor(fx, 1, 64);
-        fy += __shfl_xor(fy, 1, 64);
-        fz += __shfl_xor(fz, 1, 64);
-        fx += c.mass * c.gx - c.drag * vi.x;
-        fy += c.mass * c.gy - c.drag * vi.y;
-        fz += c.mass * c.gz - c.drag * vi.z;
-        if (kFeat) {
-            m1 += __shfl_xor(m1, 1, 64);
-            m2 += __shfl_xor(m2, 1, 64);
-            m3 += __shfl_xor(m3, 1, 64);
-            cx += __shfl_xor(cx, 1, 64);
-            cy += __shfl_xor(cy, 1, 64);
-            cz += __shfl_xor(cz, 1, 64);
-            divv += __shfl_xor(divv, 1, 64);
-        }
-        if (valid && half == 0) {
-            force[i] = make_float4(fx, fy, fz, 0.f);
-            if (kMode == 1) {
-                xstar[i] = predict_position(c, st, xi, vi, fx, fy, fz);
-                rho_star[i] = rhoi;
+    if (!valid) return;  // the lane pair shares validity
+    const float coef = c.visc * c.mass2;
+    const float kg = c.mass2 * c.spiky_grad;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, epx = 0.f, epy = 0.f, epz = 0.f;  // warm: Kahan f_p
+    with_staging(in_lds, [&](auto tag) {
+        constexpr bool kL = decltype(tag)::value;
+        auto body = [&](int j, bool live) {
+            float4 a, b;
+            if (kL) {
+                a = sx[j];
+                b = sv[j];
+            } else {
+                const int g = staged ? tab_global(tt, j) : j;
+                a = sxsrc[g];
+                const float4 v = vel[g];
+                b = make_float4(v.x, v.y, v.z, rho[g]);
             }
-        }
-        if (kFeat && valid) {
-            const size_t S = (size_t)c.stride;
-            float nn = cnt > 1 ? (float)(cnt - 1) : 1.f;
-            float inv = 1.f / nn;
-            float wall = 3.4e38f;
-            const float p3[3] = {xi.x, xi.y, xi.z};
-            for (int a = 0; a < 3; a++) {
-                float lo = p3[a] - (c.tank_c[a] - c.tank_h[a]);
-                float hi = (c.tank_c[a] + c.tank_h[a]) - p3[a];
-                wall = fminf(wall, fminf(lo, hi));
+            const float dx = xi.x - a.x, dy = xi.y - a.y, dz = xi.z - a.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float r = fast_sqrt(d2);
+            // masks instead of branches: every pair's arithmetic runs, a
+            // dead pair contributes exact zeros, and the neighbour's LDS
+            // reads stay unconditional (so they are issued early)
+            const bool in = live & spiky_in(c, r);
+            const float x = keep_if(in, 1.f - r * c.inv_h);
+            // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j
+            const float s = coef * (c.spiky_lap * x) * fast_rcp(b.w);
+            fx += (b.x - vi.x) * s;
+            fy += (b.y - vi.y) * s;
+            fz += (b.z - vi.z) * s;
+            if constexpr (kWarm) {
+                // PredictPressureForceFor (src/equations/pcisph_equations3.cpp:114-155)
+                // at the current positions with rho* = rho: pairs j != i with
+                // 1e-8 <= r inside the spiky cutoff, the pressure-force
+                // kernel's pair rule (r = d2 rsqrt(d2) is NaN for d2 = 0);
+                // with d = x_i - x_j here, f_p = +sum g d
+                const float ir = fast_rsq(d2);
+                const float rp = d2 * ir;
+                const bool inp = live & spiky_in(c, rp) & (rp >= 1e-8f);
+                const float xp = 1.f - rp * c.inv_h;
+                const float g = keep_if(inp, kg * (qi + a.w) * xp * xp * ir);
+                kahan_add(px, epx, g * dx);
+                kahan_add(py, epy, g * dy);
+                kahan_add(pz, epz, g * dz);
             }
-            float vn = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
-            float fn = sqrtf(fx * fx + fy * fy + fz * fz);
-            float a_scale = st.dt * st.dt / (c.mass * c.h);
-            fv[BBX_F_DENSITY_ERR] = rhoi / c.rho0 - 1.f;
-            fv[BBX_F_DIVV] = st.dt * divv;
-            fv[BBX_F_COUNT] = (float)cnt;
-            fv[BBX_F_M1] = m1 * inv;
-            fv[BBX_F_M2] = m2 * inv;
-            fv[BBX_F_M3] = m3 * inv;
-            fv[BBX_F_CX] = cx * inv;
-            fv[BBX_F_CY] = cy * inv;
-            fv[BBX_F_CZ] = cz * inv;
-            fv[BBX_F_CNORM] = sqrtf(cx * cx + cy * cy + cz * cz) * inv;
-            fv[BBX_F_PPREV] = st.delta != 0.f ? vi.w / (st.delta * c.rho0) : 0.f;
-            fv[BBX_F_VY] = st.dt * vi.y * c.inv_h;
-            fv[BBX_F_VNORM] = st.dt * vn * c.inv_h;
-            fv[BBX_F_AY] = fy * a_scale;
-            fv[BBX_F_ANORM] = fn * a_scale;
-            fv[BBX_F_WALL] = fminf(fmaxf(wall * c.inv_h, 0.f), 4.f);
-            if (half == 0) {
-#pragma unroll
-                for (int f = 0; f < BBX_NFEAT; f++) feat[(size_t)f * S + i] = fv[f];
-                rho_star[i] = rhoi;
-            }
-        }
+        };
+        // quads half, half + 2, half + 4, ... of the bucket, two in flight
+        for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, staged, i, cnt, c.stride, body, half);
+    });
+    // the pair's partial sums
+    fx += __shfl_xor(fx, 1, 64);
+    fy += __shfl_xor(fy, 1, 64);
+    fz += __shfl_xor(fz, 1, 64);
+    fx += c.mass * c.gx - c.drag * vi.x;
+    fy += c.mass * c.gy - c.drag * vi.y;
+    fz += c.mass * c.gz - c.drag * vi.z;
+    float fpx = 0.f, fpy = 0.f, fpz = 0.f;
+    if (kWarm) {
+        // compensated partial sums (sum - compensation), then the pair's
+        fpx = px - epx;
+        fpy = py - epy;
+        fpz = pz - epz;
+        fpx += __shfl_xor(fpx, 1, 64);
+        fpy += __shfl_xor(fpy, 1, 64);
+        fpz += __shfl_xor(fpz, 1, 64);
     }
-    if (kMode == 3) {
-        // The block's 256 feature rows go through LDS into the MFMA layout:
-        // each of the eight waves infers 32 particles as two 16-sample tiles.
-        __syncthreads();  // every thread is done with the staged neighbours
-        constexpr int kRow = BBX_NFEAT + 1;  // padded feature row
-        float *lf = pool;               // 256 padded feature rows
-        float *lp = lf + kBlock * kRow; // 256 warm-start pressures
-        float *lb = pool + 5 * kTileCap2;  // b1 | b2 | w3
-        // BF16 weights of both layers (16-byte aligned: 5 kTileCap2 + 3 kH floats in)
-        unsigned short *lw = reinterpret_cast<unsigned short *>(lb + 3 * kH);
-        static_assert(kBlock * (kRow + 1) <= 5 * kTileCap2 &&
-                          3 * kH + kLdsWeights / 2 <= 3 * kTileCap2 && (5 * kTileCap2 + 3 * kH) % 4 == 0,
-                      "mode-3 LDS layout");
-        if (half == 0) {
-#pragma unroll
-            for (int f = 0; f < BBX_NFEAT; f++) lf[pl * kRow + f] = valid ? fv[f] : 0.f;
-        }
-        static_assert(kNpThreads == kPrefetchThreads, "the prefetch split assumes 512 threads");
-        store_infer_prefetch(net, lw, lb, threadIdx.x);
-        __syncthreads();
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        const int qq = lane >> 4, jj = lane & 15;
-        bf16x4 w1[4], w2[4][4];
-        load_weight_operands_lds(lw, lane, w1, w2);
-        const float b3 = P[OFF_B3], oscale = P[OFF_OSCALE], alpha = P[OFF_ALPHA];
-        const float pcap = P[OFF_PCAP], resid = P[OFF_RESID];
-        float xv[2][4];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int smp = wv * 32 + t * 16 + jj;
-#pragma unroll
-            for (int v = 0; v < 4; v++) xv[t][v] = lf[smp * kRow + 4 * qq + v];
-        }
-        float yn[2];
-        forward_infer_tiles<2>(w1, w2, lb, lb + kH, lb + 2 * kH, b3, xv, net.mean, net.istd, qq, yn);
-        if (lane < 16) {
-#pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const int smp = wv * 32 + t * 16 + jj;
-                // the same expression as the standalone inference kernel
-                float y = yn[t] * oscale;
-                if (resid != 0.f) y += resid * lf[smp * kRow + BBX_F_PPREV];
-                y = alpha * fminf(fmaxf(y, 0.f), pcap);
-                lp[smp] = y * st.warm_scale;
-            }
-        }
-        __syncthreads();
-        if (valid && half == 0) {
-            const float p = lp[pl];
-            const float d = rhoi;
-            const float d2 = d * d;
-            pwarm[i] = p;
-            pres[i] = p;
-            xq[i] = make_float4(xi.x, xi.y, xi.z, (fabsf(d2) < 1e-8f) ? 0.f : p / d2);
+    if (half == 0) {
+        force[i] = make_float4(fx, fy, fz, 0.f);
+        if (kMode >= 1) {
+            xstar[i] = predict_position(c, st, xi, vi, fx + fpx, fy + fpy, fz + fpz);
+            rho_star[i] = rhoi;
         }
     }
 }
@@ -721,41 +609,30 @@ static NbrTab nbr_tab(const bbx_solver *s) {
     return NbrTab{s->nbr16, s->nbr32, s->nbr_cnt, s->tile_tab};
 }
 
-int run_nonpressure(bbx_solver *s, const StepConst &st, int predict, int features) {
+int run_nonpressure(bbx_solver *s, const StepConst &st, int mode) {
     if (s->n == 0) return BBX_OK;
     const unsigned nb = nblocks(s->n);
     const NbrTab t = nbr_tab(s);
-    if (features == 2) {
-        if (!s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
-        BBX_PROF_BEGIN("nonpressure_features_warm");
-        k_nonpressure<3><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat,
-                                                       mlp_params_device(s->mlp), s->pres, s->xq,
-                                                       s->pwarm);
+#define BBX_NP(M)                                                                                     \
+    k_nonpressure<M><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->xq, \
+                                                     s->force, s->xstar, s->rho_star)
+    if (mode == 2) {
+        BBX_PROF_BEGIN("nonpressure_warm");
+        BBX_NP(2);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
-    } else if (features) {
-        BBX_PROF_BEGIN("nonpressure_features");
-        k_nonpressure<2><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, nullptr,
-                                                       nullptr, nullptr, nullptr);
-        BBX_HIP_CHECK(hipGetLastError());
-        BBX_PROF_END();
-    } else if (predict) {
+    } else if (mode == 1) {
         BBX_PROF_BEGIN("nonpressure_predict");
-        k_nonpressure<1><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, nullptr,
-                                                       nullptr, nullptr, nullptr);
+        BBX_NP(1);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     } else {
         BBX_PROF_BEGIN("nonpressure");
-        k_nonpressure<0><<<nb, kNpThreads, 0, s->stream>>>(s->sc, st, t, s->posA, s->velA, s->rho, s->force,
-                                                       s->xstar, s->rho_star, s->feat, nullptr,
-                                                       nullptr, nullptr, nullptr);
+        BBX_NP(0);
         BBX_HIP_CHECK(hipGetLastError());
         BBX_PROF_END();
     }
+#undef BBX_NP
     return BBX_OK;
 }
 

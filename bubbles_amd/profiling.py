@@ -32,9 +32,11 @@ COPY_BYTES = float(2 ** 30)
 SHORT = {
     "k_hash": "grid_hash", "k_tile_tables": "tile_tables", "k_scatter": "grid_scatter", "k_rank_move": "grid_rank_move",
     "k_scan_block": "grid_scan", "k_scan_add": "grid_scan_add",
-    "k_buckets_density<false>": "buckets_density", "k_buckets_density<true>": "buckets_density_tait",
+    "k_buckets_density<false, 0>": "buckets_density", "k_buckets_density<true, 0>": "buckets_density_tait",
+    "k_buckets_density<false, 1>": "buckets_density_features",
+    "k_buckets_density<false, 2>": "buckets_density_warm",
     "k_nonpressure<0>": "nonpressure", "k_nonpressure<1>": "nonpressure_predict",
-    "k_nonpressure<2>": "nonpressure_features", "k_nonpressure<3>": "nonpressure_features_warm",
+    "k_nonpressure<2>": "nonpressure_warm",
     "k_mlp_infer": "mlp_infer",
     "k_warm_init": "warm_init", "k_predict_pressure": "predict_pressure",
     "k_pressure_force": "pressure_force", "k_stream_copy": "stream_copy",
@@ -129,9 +131,6 @@ def traffic(fetch_csv: str, write_csv: str, n: int):
         kernels[name] = {"dispatches": len(v), "fetch_bytes": round(fb), "write_bytes": round(wb),
                          "bytes_per_launch": round(fb + wb),
                          "bytes_per_particle": round((fb + wb) / n, 2)}
-    # the warm-start launch of pressure_force predicts like the others
-    if "pressure_force_predict" in kernels:
-        kernels.setdefault("pressure_force_warm", dict(kernels["pressure_force_predict"]))
     return {"n": n, "fetch_scale": round(kf, 4), "write_scale": round(kw, 4), "kernels": kernels}
 
 

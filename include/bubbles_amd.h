@@ -48,17 +48,19 @@ extern "C" {
 
 /* pressure warm start: NONE starts every substep from p = 0, MLP from the
  * attached network's prediction.  AUTO uses the network only where it can
- * pay for itself (it costs about one correction of extra work): after a
- * plain substep that needed BBX_WARM_AUTO_MIN_ITERATIONS or more
- * corrections it switches the network on, and it switches it off again
- * after BBX_WARM_AUTO_PROBE warm substeps in a row converged in a single
- * correction.  The correction loop and its tolerance are the same in every
- * mode. */
+ * pay for itself (it costs about two thirds of a correction): while the
+ * flow is calm it switches the network on after a plain substep that needed
+ * BBX_WARM_AUTO_MIN_ITERATIONS or more corrections and off again after
+ * BBX_WARM_AUTO_PROBE warm substeps in a row converged in a single
+ * correction; once any substep has needed BBX_WARM_AUTO_ACTIVE or more
+ * corrections the network stays on for the rest of the solver's life.  The
+ * correction loop and its tolerance are the same in every mode. */
 #define BBX_WARM_NONE 0
 #define BBX_WARM_MLP 1
 #define BBX_WARM_AUTO 2
 #define BBX_WARM_AUTO_MIN_ITERATIONS 3
 #define BBX_WARM_AUTO_PROBE 8
+#define BBX_WARM_AUTO_ACTIVE 4
 
 /* downloadable / uploadable per-particle fields (by particle id) */
 #define BBX_FIELD_POSITION 0          /* 3 doubles */
@@ -238,6 +240,10 @@ int bbx_mlp_train(bbx_mlp *m, const float *x, const float *t, int n, int steps, 
 /* Attach (or, with m == NULL, detach) the warm-start network; the solver
  * holds a reference until it detaches the network or is destroyed. */
 int bbx_solver_attach_mlp(bbx_solver *s, bbx_mlp *m);
+/* Keep the features of every warm-started substep for BBX_FIELD_FEATURES
+ * downloads (default off: the fused bucket kernel then stores none, which
+ * saves 64 bytes per particle of writes).  No counterpart in the reference. */
+int bbx_solver_keep_features(bbx_solver *s, int keep);
 /* Harvest (features, converged pressure / P) pairs of one TEXTBOOK substep:
  * runs the substep and writes n*16 features and n targets (by particle id). */
 int bbx_pcisph_harvest(bbx_solver *s, double dt, int max_it, double tol, float *features,

@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY (the checker of the gfx950 MLP kernels).
 
 Parameter layout (FP32 vector, see include/bubbles_amd.h):
   W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1] | mean[16] inv_std[16] out_scale[1]
-Forward:  xn = (x - mean) * inv_std;  z1 = W1 xn + b1;  h1 = gelu(z1)
+Forward:  xn = clip((x - mean) * inv_std, -8, 8);  z1 = W1 xn + b1;  h1 = gelu(z1)
           z2 = W2 h1 + b2;  h2 = gelu(z2);  y = W3 . h2 + b3
 Loss:     0.5 / B * sum w (y - t / out_scale)^2, w = 2(1 - tau) above the
           target and 2 tau below (expectile tau; 0.5 = mean squared error)
@@ -67,12 +67,15 @@ def unpack(p):
     }
 
 
+FEAT_CLIP = 8.0  # normalised features are clipped to +-8 standard deviations
+
+
 def _normalise(P, x, emulate):
     if emulate:
         x32 = np.asarray(x, dtype=np.float32)
         xn = (x32 - P["mean"].astype(np.float32)) * P["istd"].astype(np.float32)
-        return bf16(xn)
-    return (np.asarray(x, dtype=np.float64) - P["mean"]) * P["istd"]
+        return bf16(np.clip(xn, np.float32(-FEAT_CLIP), np.float32(FEAT_CLIP)))
+    return np.clip((np.asarray(x, dtype=np.float64) - P["mean"]) * P["istd"], -FEAT_CLIP, FEAT_CLIP)
 
 
 def forward(params, x, emulate=True):

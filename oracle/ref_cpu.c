@@ -1639,17 +1639,21 @@ int bbo_sph_advance(bbo_sim *s, double interval, int sph_mode) {
     return steps;
 }
 
-/* ---- learned warm-start features ---- */
+/* ---- learned warm-start features (include/bbx_features.h) ----
+ * The state at the start of a substep after the distribution and density
+ * stages: bucket, density, own position and velocity and the previous
+ * substep's converged pressure (s->pressure).  This library's own inputs;
+ * the reference has no learned component. */
 #include "../include/bbx_features.h"
 void bbo_features(bbo_sim *s, double dt, double *out) {
-    double h = s->kernel_radius, m = s->mass, rho0 = s->prm.target_density;
+    double h = s->kernel_radius, rho0 = s->prm.target_density;
     double delta = bbo_compute_delta(s, dt);
     double pref = delta * rho0;
     for (int i = 0; i < s->n; i++) {
         v3 pi = s->pos[i], vi = s->vel[i];
         const int *ids = &s->bucket_ids[(size_t)i * BBO_BUCKET_SIZE];
         int cnt = s->bucket_cnt[i];
-        double m1 = 0, m2 = 0, m3 = 0, divv = 0;
+        double m1 = 0, m2 = 0, m3 = 0;
         v3 c = mk(0, 0, 0);
         for (int q = 0; q < cnt; q++) {
             int j = ids[q];
@@ -1659,14 +1663,9 @@ void bbo_features(bbo_sim *s, double dt, double *out) {
             double u = r / h;
             m1 += u; m2 += u * u; m3 += u * u * u;
             c = vadd(c, vmul(d, 1.0 / h));
-            if (r > 0 && bbo_within_spiky(r, h)) {
-                v3 g = vmul(vmul(d, 1.0 / r), -bbo_spiky_dW(h, r));
-                divv += m / s->density[j] * vdot(vsub(s->vel[j], vi), g);
-            }
         }
         double nn = cnt > 1 ? (double)(cnt - 1) : 1.0;
         double *f = &out[(size_t)i * BBX_NFEAT];
-        v3 fo = s->force[i];
         double wall = 1e30;
         for (int k = 0; k < 3; k++) {
             double pk = k == 0 ? pi.x : (k == 1 ? pi.y : pi.z);
@@ -1677,7 +1676,7 @@ void bbo_features(bbo_sim *s, double dt, double *out) {
             if (dhi < wall) wall = dhi;
         }
         f[BBX_F_DENSITY_ERR] = s->density[i] / rho0 - 1.0;
-        f[BBX_F_DIVV] = dt * divv;
+        f[BBX_F_RESERVED1] = 0.0;
         f[BBX_F_COUNT] = (double)cnt;
         f[BBX_F_M1] = m1 / nn;
         f[BBX_F_M2] = m2 / nn;
@@ -1689,8 +1688,8 @@ void bbo_features(bbo_sim *s, double dt, double *out) {
         f[BBX_F_PPREV] = pref != 0 ? s->pressure[i] / pref : 0;
         f[BBX_F_VY] = dt * vi.y / h;
         f[BBX_F_VNORM] = dt * vlen(vi) / h;
-        f[BBX_F_AY] = dt * dt * fo.y / (m * h);
-        f[BBX_F_ANORM] = dt * dt * vlen(fo) / (m * h);
+        f[BBX_F_RESERVED2] = 0.0;
+        f[BBX_F_RESERVED3] = 0.0;
         f[BBX_F_WALL] = fminr(fmaxr(wall / h, 0.0), 4.0);
     }
 }

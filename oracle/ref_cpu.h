@@ -220,7 +220,7 @@ void bbo_sim_set_colliders(bbo_sim *s, const bbo_collider *cs, int count);
 
 /* learned warm-start features (FP64), 16 per particle, computed from the
  * state after density + non-pressure force of a substep; see features.h */
-void bbo_features(bbo_sim *s, double dt, double *out16);
+void bbo_features(bbo_sim *s, double dt, double *out);
 
 #ifdef __cplusplus
 }

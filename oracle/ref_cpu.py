@@ -464,6 +464,7 @@ class Sim:
         return lib().bbo_sph_advance(self.h, interval, sph_mode)
 
     def features(self, dt):
+        """Warm-start features of the current state (include/bbx_features.h)."""
         out = np.zeros((self.n, 16), dtype=np.float64)
         lib().bbo_features(self.h, dt, _dp(out))
         return out

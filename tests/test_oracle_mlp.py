@@ -141,3 +141,23 @@ def test_adam_first_step_is_lr_sign():
     step = p[:M.N_TRAIN] - p2[:M.N_TRAIN]
     big = np.abs(g[:M.N_TRAIN]) > 1e-2
     np.testing.assert_allclose(step[big], 1e-3 * np.sign(g[:M.N_TRAIN][big]), rtol=1e-4)
+
+
+def test_features_beyond_eight_sigma_are_clipped():
+    """Normalised inputs are clipped to +-8: a feature 100 standard
+    deviations out gives exactly the output of one at 8, and inputs inside
+    the range are untouched (the output then depends on them)."""
+    p = random_params(3)
+    mean, istd = p[M.OFF_MEAN:M.OFF_ISTD], p[M.OFF_ISTD:M.OFF_OSCALE]
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(8, M.F))
+    far, edge, inside = x.copy(), x.copy(), x.copy()
+    far[:, 1] = mean[1] + 100.0 / istd[1]
+    edge[:, 1] = mean[1] + 8.0 / istd[1]
+    inside[:, 1] = mean[1] + 7.0 / istd[1]
+    for emulate in (False, True):
+        yf = M.predict(p, far, emulate=emulate)
+        ye = M.predict(p, edge, emulate=emulate)
+        yi = M.predict(p, inside, emulate=emulate)
+        np.testing.assert_array_equal(yf, ye)
+        assert np.abs(yi - ye).max() > 0

@@ -421,8 +421,9 @@ def _features(s, dt):
 
 
 def test_features_isolated_particle(oracle):
-    """One particle: its bucket holds only itself, so moments, centroid and
-    divergence vanish; the kinematic features follow from v, f and the walls."""
+    """One particle: its bucket holds only itself, so moments and centroid
+    vanish; the kinematic features follow from v and the walls, and the
+    reserved slots hold 0."""
     sc = base_scene()
     p = [0.05, -0.1, 0.02]
     v = [0.1, -0.2, 0.3]
@@ -434,16 +435,12 @@ def test_features_isolated_particle(oracle):
     h, m = inf["kernel_radius"], inf["mass"]
     rho = m * 315.0 / (64.0 * math.pi * h ** 3)
     assert x[F.COUNT] == 1.0
-    for k in (F.M1, F.M2, F.M3, F.CX, F.CY, F.CZ, F.CNORM, F.DIVV):
+    for k in (F.M1, F.M2, F.M3, F.CX, F.CY, F.CZ, F.CNORM, F.RESERVED1, F.RESERVED2, F.RESERVED3):
         assert x[k] == 0.0
     assert x[F.DENSITY_ERR] == pytest.approx(rho / 1000.0 - 1.0, rel=1e-12)
     assert x[F.PPREV] == pytest.approx(7.0 / (s.delta(dt) * 1000.0), rel=1e-12)
     assert x[F.VY] == pytest.approx(dt * -0.2 / h, rel=1e-14)
     assert x[F.VNORM] == pytest.approx(dt * math.sqrt(0.14) / h, rel=1e-14)
-    fy = m * GY - 1e-4 * -0.2
-    fn = math.sqrt((1e-4 * 0.1) ** 2 + fy ** 2 + (1e-4 * 0.3) ** 2)
-    assert x[F.AY] == pytest.approx(dt * dt * fy / (m * h), rel=1e-12)
-    assert x[F.ANORM] == pytest.approx(dt * dt * fn / (m * h), rel=1e-12)
     half = np.asarray(sc.tank_size) / 2
     wall = min(min(pi + hi, hi - pi) for pi, hi in zip(p, half))
     assert x[F.WALL] == pytest.approx(min(wall / h, 4.0), rel=1e-12)
@@ -464,22 +461,26 @@ def test_features_perfect_lattice_interior(oracle):
     for k, pw in ((F.M1, 1), (F.M2, 2), (F.M3, 3)):
         assert x[i, k] == pytest.approx((u ** pw).sum() / 26.0, rel=1e-12)
     assert abs(x[i, F.CX]) < 1e-12 and abs(x[i, F.CY]) < 1e-12 and abs(x[i, F.CZ]) < 1e-12
-    assert x[i, F.DIVV] == 0.0
+    assert abs(x[i, F.CNORM]) < 1e-12
 
 
-def test_features_divergence_sign_and_linearity(oracle):
-    """An expanding field v = a x has positive SPH divergence, a compressing
-    one negative, and the estimate is linear in a."""
+def test_features_centroid_points_to_the_neighbours(oracle):
+    """A particle on the face of a lattice block has neighbours only on one
+    side: the centroid offset points into the block along the face normal,
+    and its norm is the same lattice sum counted from the face."""
     sc = scenes.dam_break(12, 12, 12, spacing=0.01, tank=(0.4, 0.4, 0.4), seed=0)
     pos = scenes.lattice_block((12, 12, 12), 0.01, (-0.06, -0.06, -0.06), 0, jitter=0.0)
-    i = 6 + 12 * (6 + 12 * 6)
-    out = {}
-    for a in (0.5, 1.0, -1.0):
-        s = oracle.Sim(with_points(sc, pos, a * pos), gravity=(0.0, 0.0, 0.0))
-        out[a] = _features(s, 5e-4)[i, F.DIVV]
-    assert out[1.0] > 0 > out[-1.0]
-    assert out[0.5] == pytest.approx(0.5 * out[1.0], rel=1e-12)
-    assert out[-1.0] == pytest.approx(-out[1.0], rel=1e-12)
+    s = oracle.Sim(with_points(sc, pos))
+    x = _features(s, 5e-4)
+    i = 6 + 12 * (0 + 12 * 6)  # y = 0 face, middle of it
+    offs = [(a, b, c) for a in (-1, 0, 1) for b in (0, 1) for c in (-1, 0, 1)
+            if (a, b, c) != (0, 0, 0) and a * a + b * b + c * c <= 3]
+    n = len(offs)
+    assert x[i, F.COUNT] == n + 1
+    cy = sum(b for _, b, _ in offs) * 0.01 / 0.02 / n  # (y_j - y_i) / h over n' = n
+    assert x[i, F.CY] == pytest.approx(cy, rel=1e-12)
+    assert abs(x[i, F.CX]) < 1e-12 and abs(x[i, F.CZ]) < 1e-12
+    assert x[i, F.CNORM] == pytest.approx(cy, rel=1e-12)
 
 
 @pytest.mark.parametrize("spacing,nps", [(0.0097, 0.0), (0.0103, 0.0), (0.0103, 0.5)])

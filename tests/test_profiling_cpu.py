@@ -20,9 +20,10 @@ def _write(path, counter, rows):
 
 
 def test_short_names():
-    assert profiling.short_name("void bbx::k_buckets_density<false>(bbx::SimConst, int)") == "buckets_density"
+    assert profiling.short_name("void bbx::k_buckets_density<false, 0>(bbx::SimConst, int)") == "buckets_density"
+    assert profiling.short_name("void bbx::k_buckets_density<false, 2>(bbx::SimConst, int)") == "buckets_density_warm"
     assert profiling.short_name("bbx::k_pressure_force(bbx::SimConst)") == "pressure_force"
-    assert profiling.short_name("bbx::k_nonpressure<3>(x)") == "nonpressure_features_warm"
+    assert profiling.short_name("bbx::k_nonpressure<2>(x)") == "nonpressure_warm"
 
 
 def test_traffic_calibration_and_roles(tmp_path):
