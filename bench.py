@@ -117,16 +117,15 @@ def run_variant(args, sc, warm: int, max_it: int, steps: int, warmup: int, profi
 
         solver.attach_mlp(learned.load())
     n_prof = min(warmup, 20) if profile else 0
-    for _ in range(warmup - n_prof):
-        solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
+    if warmup - n_prof:
+        solver.pcisph_run(sc.dt, warmup - n_prof, max_it, args.tol, mode, warm)
     table, dominant = {}, None
     if n_prof:
         solver.profile_select(None)
         solver.profile(True)
         solver.synchronize()
         solver.profile_reset()
-        for _ in range(n_prof):
-            solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
+        solver.pcisph_run(sc.dt, n_prof, max_it, args.tol, mode, warm)
         solver.synchronize()
         prof = solver.profile_get()
         solver.profile(False)
@@ -139,15 +138,13 @@ def run_variant(args, sc, warm: int, max_it: int, steps: int, warmup: int, profi
         solver.profile_select(base, it)
         solver.profile(True)
         solver.profile_reset()
-    iters, errs = [], []
     solver.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        st = solver.pcisph_substep(sc.dt, max_it, args.tol, mode, warm)
-        iters.append(st.iterations)
-        errs.append(st.max_density_error)
+    stats = solver.pcisph_run(sc.dt, steps, max_it, args.tol, mode, warm)  # the timed substeps, one call
     solver.synchronize()
     t1 = time.perf_counter()
+    iters = [st.iterations for st in stats]
+    errs = [st.max_density_error for st in stats]
     live = solver.profile_get() if dominant else {}
     solver.profile(False)
     mean_bucket = float(np.mean(solver.bucket_counts()))

@@ -134,6 +134,7 @@ def load():
         "bbx_mlp_train_step": (i, [vp, fp, fp, i, C.c_float, fp]),
         "bbx_mlp_train": (i, [vp, fp, fp, i, i, i, C.c_float, i, fp]),
         "bbx_solver_keep_features": (i, [vp, i]),
+        "bbx_pcisph_run": (i, [vp, d, i, i, d, i, i, C.POINTER(Stats)]),
         "bbx_solver_attach_mlp": (i, [vp, vp]),
         "bbx_pcisph_harvest": (i, [vp, d, i, d, fp, fp, C.POINTER(Stats)]),
         "bbx_profile_enable": (i, [vp, i]),
@@ -523,6 +524,12 @@ class Solver:
         check(load().bbx_pcisph_substep(self.h, dt, max_it, tol, mode, warm, C.byref(st)),
               "bbx_pcisph_substep")
         return st
+
+    def pcisph_run(self, dt, count, max_it=5, tol=0.01, mode=MODE_REFERENCE_EXACT, warm=WARM_NONE):
+        """`count` substeps of fixed dt in one call: a list of Stats."""
+        st = (Stats * max(count, 1))()
+        check(load().bbx_pcisph_run(self.h, dt, count, max_it, tol, mode, warm, st), "bbx_pcisph_run")
+        return list(st[:count])
 
     def pcisph_advance(self, interval, max_it=5, tol=0.01, mode=MODE_REFERENCE_EXACT, warm=WARM_NONE):
         n = C.c_int()

@@ -75,8 +75,10 @@ static std::string role_name(const std::string &base, int iter, int iterations) 
     return base;
 }
 
-int Profiler::resolve(int iterations) {
-    for (auto &p : pending) {
+int Profiler::resolve(int iterations, size_t upto) {
+    const size_t m = upto < pending.size() ? upto : pending.size();
+    for (size_t k = 0; k < m; k++) {
+        const Pending &p = pending[k];
         float ms = 0.f;
         if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             const int id = name_id(role_name(names[p.id], p.iter, iterations));
@@ -86,7 +88,7 @@ int Profiler::resolve(int iterations) {
         pool.push_back(p.a);
         pool.push_back(p.b);
     }
-    pending.clear();
+    pending.erase(pending.begin(), pending.begin() + (std::ptrdiff_t)m);
     return BBX_OK;
 }
 
@@ -392,9 +394,12 @@ static void free_all(bbx_solver *s) {
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     if (s->h_max_force) (void)hipHostFree(s->h_max_force);
     s->h_loop = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
-    if (s->ev0) (void)hipEventDestroy(s->ev0);
-    if (s->ev1) (void)hipEventDestroy(s->ev1);
-    s->ev0 = s->ev1 = nullptr;
+    for (int k = 0; k < 2; k++) {
+        if (s->ev0[k]) (void)hipEventDestroy(s->ev0[k]);
+        if (s->ev1[k]) (void)hipEventDestroy(s->ev1[k]);
+        if (s->ev_read[k]) (void)hipEventDestroy(s->ev_read[k]);
+        s->ev0[k] = s->ev1[k] = s->ev_read[k] = nullptr;
+    }
     s->prof.destroy();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
@@ -535,8 +540,10 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned) * kSlots) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess))
-        rc = fail(BBX_E_HIP, "event create");
+    for (int k = 0; k < 2 && !rc; k++)
+        if (hipEventCreate(&s->ev0[k]) != hipSuccess || hipEventCreate(&s->ev1[k]) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_read[k], hipEventDisableTiming) != hipSuccess)
+            rc = fail(BBX_E_HIP, "event create");
     if (rc) {
         free_all(s);
         delete s;
@@ -860,7 +867,9 @@ int bbx_solver_set_flag(bbx_solver *s, int flag) {
 // One readback per call: counters, max |f| and (PCISPH) the loop state.
 // iterations = index of the first iteration whose done flag is set, + 1;
 // the reported error is that iteration's FP64 max error / rho0.
-static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
+// Queue the copies of the step's counters (and of the loop record, for a
+// PCISPH loop of loop_len launches) into pinned host memory.
+static int queue_stats(bbx_solver *s, int loop_len) {
     BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->counters, sizeof(int) * CNT_N, hipMemcpyDeviceToHost,
                                  s->stream));
     if (loop_len > 0)
@@ -868,26 +877,34 @@ static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
                                      hipMemcpyDeviceToHost, s->stream));
     BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
                                  hipMemcpyDeviceToHost, s->stream));
-    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
-    if (st) {
-        st->full_redistribution = s->h_counters[CNT_FULL];
-        st->bucket_overflows = s->h_counters[CNT_OVERFLOW];
-        st->lost_particles = s->h_counters[CNT_LOST];
-        st->iterations = 0;
-        st->max_density_error = 0;
-        if (loop_len > 0) {
-            int it = loop_len;
-            for (int k = 0; k < loop_len; k++)
-                if (s->h_loop->done[k]) {
-                    it = k + 1;
-                    break;
-                }
-            double e;
-            std::memcpy(&e, &s->h_loop->err[it - 1], sizeof(double));
-            st->iterations = it;
-            st->max_density_error = e / s->prm.target_density;
-        }
+    return BBX_OK;
+}
+
+// Statistics from the copies queue_stats made (they must have landed).
+static void parse_stats(bbx_solver *s, bbx_stats *st, int loop_len) {
+    st->full_redistribution = s->h_counters[CNT_FULL];
+    st->bucket_overflows = s->h_counters[CNT_OVERFLOW];
+    st->lost_particles = s->h_counters[CNT_LOST];
+    st->iterations = 0;
+    st->max_density_error = 0;
+    if (loop_len > 0) {
+        int it = loop_len;
+        for (int k = 0; k < loop_len; k++)
+            if (s->h_loop->done[k]) {
+                it = k + 1;
+                break;
+            }
+        double e;
+        std::memcpy(&e, &s->h_loop->err[it - 1], sizeof(double));
+        st->iterations = it;
+        st->max_density_error = e / s->prm.target_density;
     }
+}
+
+static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
+    BBX_CHECK(queue_stats(s, loop_len));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (st) parse_stats(s, st, loop_len);
     return BBX_OK;
 }
 
@@ -1151,8 +1168,7 @@ int bbx_number_of_time_steps(bbx_solver *s, double interval, double scale, unsig
     return time_steps(s, interval, scale, n);
 }
 
-int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
-                       bbx_stats *stats) {
+static int check_substep(bbx_solver *s, double dt, int max_it, int warm) {
     BBX_CHECK(check_setup(s));
     if (s->kind != BBX_PCISPH) return fail(BBX_E_INVALID, "not a PCISPH solver");
     if (max_it < 1 || max_it > kMaxIt)
@@ -1161,6 +1177,30 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     if (warm != BBX_WARM_NONE && warm != BBX_WARM_MLP && warm != BBX_WARM_AUTO)
         return fail(BBX_E_INVALID, "unknown warm-start mode");
     if (warm != BBX_WARM_NONE && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+    return BBX_OK;
+}
+
+// A PCISPH substep is queued in two parts.  The front does not depend on the
+// previous substep's outcome: counters cleared and the particles
+// redistributed over the grid (UpdateGridDistribution up to the bucket
+// build).  The back needs that outcome (the warm-start decision and the
+// loop's launch count): bucket build + density, non-pressure forces, the
+// predict-correct loop, XSPH, and the statistics copied to pinned memory.
+static int substep_front(bbx_solver *s, int slot) {
+    BBX_HIP_CHECK(hipEventRecord(s->ev0[slot], s->stream));
+    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
+    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
+    BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
+    return distribute_grid(s, false, -1);
+}
+
+struct SubstepPlan {
+    int warm = BBX_WARM_NONE;  // the start actually used (AUTO resolved)
+    int launches = 0;          // loop launches queued
+};
+
+static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double tol, int mode, int warm,
+                        SubstepPlan *plan) {
     if (warm == BBX_WARM_AUTO) {
         // The warm start costs about two thirds of a correction, so it only
         // pays where the plain loop needs several.  While the flow is calm
@@ -1179,10 +1219,6 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
                               : s->last_iterations >= BBX_WARM_AUTO_MIN_ITERATIONS;
         warm = on ? BBX_WARM_MLP : BBX_WARM_NONE;
     }
-    BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
-    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
-    BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
     const double delta = compute_delta(s, dt);
     s->delta64 = delta;
     StepConst st{};
@@ -1191,14 +1227,14 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     st.delta = (float)delta;
     st.warm_scale = (float)(delta * s->prm.target_density);
     if (warm == BBX_WARM_MLP) {
-        // UpdateGridDistribution + interaction + density, with the features
-        // and the network's pressures p0 in the bucket kernel; then the
-        // viscosity gather also sums f_p(p0, rho* = rho) and predicts k = 0
-        BBX_CHECK(distribute(s, false, true, false, -1, 2, &st));
+        // interaction + density, with the features and the network's
+        // pressures p0 in the bucket kernel; then the viscosity gather also
+        // sums f_p(p0, rho* = rho) and predicts k = 0
+        BBX_CHECK(build_buckets(s, false, 2, &st));
         BBX_CHECK(run_nonpressure(s, st, 2));
     } else {
-        // UpdateGridDistribution + ResetHigherLevel + interaction + density
-        BBX_CHECK(distribute(s, false, true, false));
+        // ResetHigherLevel + interaction + density
+        BBX_CHECK(build_buckets(s, false, 0, &st));
         BBX_CHECK(run_nonpressure(s, st, 1));
     }
     // The predict-correct loop is queued without per-iteration host round
@@ -1231,19 +1267,60 @@ int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mod
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
-    BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    BBX_HIP_CHECK(hipEventRecord(s->ev1[slot], s->stream));
+    BBX_CHECK(queue_stats(s, launches));
+    BBX_HIP_CHECK(hipEventRecord(s->ev_read[slot], s->stream));
+    plan->warm = warm;
+    plan->launches = launches;
+    return BBX_OK;
+}
+
+// Wait for the back's statistics and update what the next substep's
+// decisions read.  prof_upto: profiler records queued before the next
+// substep's front (those after it are still running).
+static int substep_finish(bbx_solver *s, int slot, const SubstepPlan &plan, bbx_stats *stats,
+                          size_t prof_upto) {
+    BBX_HIP_CHECK(hipEventSynchronize(s->ev_read[slot]));
     bbx_stats local{};
-    BBX_CHECK(read_stats(s, &local, launches));
+    parse_stats(s, &local, plan.launches);
     s->last_iterations = local.iterations;
-    s->last_warm = warm == BBX_WARM_MLP ? 1 : 0;
+    s->last_warm = plan.warm == BBX_WARM_MLP ? 1 : 0;
     s->warm_ones = s->last_warm && local.iterations == 1 ? s->warm_ones + 1 : 0;
     if (local.iterations >= BBX_WARM_AUTO_ACTIVE) s->warm_active = 1;
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    (void)hipEventElapsedTime(&ms, s->ev0[slot], s->ev1[slot]);
     local.gpu_ms = ms;
     local.warm_started = s->last_warm;
     if (stats) *stats = local;
-    if (s->prof.enabled) s->prof.resolve(local.iterations);
+    if (s->prof.enabled) s->prof.resolve(local.iterations, prof_upto);
+    return BBX_OK;
+}
+
+int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
+                       bbx_stats *stats) {
+    BBX_CHECK(check_substep(s, dt, max_it, warm));
+    SubstepPlan plan;
+    BBX_CHECK(substep_front(s, 0));
+    BBX_CHECK(substep_back(s, 0, dt, max_it, tol, mode, warm, &plan));
+    return substep_finish(s, 0, plan, stats, s->prof.pending.size());
+}
+
+int bbx_pcisph_run(bbx_solver *s, double dt, int count, int max_it, double tol, int mode, int warm,
+                   bbx_stats *stats) {
+    BBX_CHECK(check_substep(s, dt, max_it, warm));
+    if (count < 0) return fail(BBX_E_INVALID, "count must be >= 0");
+    // Substep k + 1's front is queued before the host waits for substep k's
+    // statistics, so the device redistributes the grid while the host reads
+    // them; the results are those of `count` bbx_pcisph_substep calls.
+    if (count > 0) BBX_CHECK(substep_front(s, 0));
+    for (int k = 0; k < count; k++) {
+        const int slot = k & 1;
+        SubstepPlan plan;
+        BBX_CHECK(substep_back(s, slot, dt, max_it, tol, mode, warm, &plan));
+        const size_t upto = s->prof.pending.size();
+        if (k + 1 < count) BBX_CHECK(substep_front(s, slot ^ 1));
+        BBX_CHECK(substep_finish(s, slot, plan, stats ? stats + k : nullptr, upto));
+    }
     return BBX_OK;
 }
 
@@ -1269,7 +1346,7 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     BBX_CHECK(check_setup(s));
     if (s->kind != BBX_SPH) return fail(BBX_E_INVALID, "not an SPH solver");
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
-    BBX_HIP_CHECK(hipEventRecord(s->ev0, s->stream));
+    BBX_HIP_CHECK(hipEventRecord(s->ev0[0], s->stream));
     BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
     BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
     BBX_CHECK(distribute(s, false, true, true));
@@ -1280,12 +1357,12 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     BBX_CHECK(run_integrate(s, st));
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
-    BBX_HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    BBX_HIP_CHECK(hipEventRecord(s->ev1[0], s->stream));
     bbx_stats local{};
     BBX_CHECK(read_stats(s, &local));
     local.max_density_error = 0;
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    (void)hipEventElapsedTime(&ms, s->ev0[0], s->ev1[0]);
     local.gpu_ms = ms;
     if (stats) *stats = local;
     if (s->prof.enabled) s->prof.resolve();

@@ -160,8 +160,10 @@ struct Profiler {
     void begin(hipStream_t s, hipEvent_t *a);
     void end(hipStream_t s, int id, int iter, hipEvent_t a);
     // iterations >= 0 names the loop kernels by role: predict_pressure or
-    // predict_pressure_skipped; pressure_force_{warm,predict,integrate,skipped}
-    int resolve(int iterations = -1);
+    // predict_pressure_skipped; pressure_force_{warm,predict,integrate,skipped}.
+    // Only the first `upto` pending records are resolved (those of one
+    // substep when the next one's front is already queued).
+    int resolve(int iterations = -1, size_t upto = SIZE_MAX);
     void reset();
     void destroy();
 };
@@ -242,7 +244,10 @@ struct bbx_solver {
     unsigned long long *h_max_err = nullptr;
     int *h_counters = nullptr;               // CNT_N counters + one loop flag
     unsigned *h_max_force = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // substep timing: start and end of the queued work, and the end of the
+    // statistics readback; two sets, so that a run of PCISPH substeps can
+    // queue the next substep's grid work before it reads this one's
+    hipEvent_t ev0[2] = {}, ev1[2] = {}, ev_read[2] = {};
 
     bbx_mlp *mlp = nullptr;
     bbx::Profiler prof;
@@ -267,8 +272,12 @@ int launch_scan(bbx_solver *s, const int *in, int *out, size_t n);
 // the warm-start features (stored in feat); 2 it also infers the attached
 // network and starts the substep from its pressures (p0 into pres and pwarm,
 // (x, p0 / rho^2) into xq, rho* = rho); st carries dt, delta and the scale
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from = -1,
+int distribute(bbx_solver *s, bool force_full, bool buckets, bool with_pressure, int append_from = -1,
                int feat_mode = 0, const StepConst *st = nullptr);
+// the two halves of distribute: the grid (hash, scan, scatter, move) and the
+// tile tables + bucket build (the latter is where the warm start is chosen)
+int distribute_grid(bbx_solver *s, bool force_full, int append_from);
+int build_buckets(bbx_solver *s, bool with_pressure, int feat_mode, const StepConst *st);
 int run_density_only(bbx_solver *s);
 // mode: 0 force only, 1 force + predict(k = 0), 2 force + the warm start's
 // pressure force + predict(k = 0) (the bucket kernel inferred p0)

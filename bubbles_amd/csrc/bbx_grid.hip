@@ -736,11 +736,7 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // UpdateGridDistribution: full or incremental chain update, then buckets.
-int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pressure, int append_from,
-               int feat_mode, const StepConst *st) {
-    if (feat_mode && (!build_buckets || with_pressure || !st))
-        return fail(BBX_E_INVALID, "warm-start features need a PCISPH bucket build and step constants");
-    if (feat_mode == 2 && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+int distribute_grid(bbx_solver *s, bool force_full, int append_from) {
     SimConst &c = s->sc;
     const int n = s->n;
     const size_t cells = (size_t)c.g.total + 1;
@@ -784,7 +780,16 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
         std::swap(s->velA, s->velB);
         std::swap(s->cellA, s->cellB);
     }
-    if (build_buckets && n > 0) {
+    return BBX_OK;
+}
+
+int build_buckets(bbx_solver *s, bool with_pressure, int feat_mode, const StepConst *st) {
+    if (feat_mode && (with_pressure || !st))
+        return fail(BBX_E_INVALID, "warm-start features need a PCISPH bucket build and step constants");
+    if (feat_mode == 2 && !s->mlp) return fail(BBX_E_INVALID, "no MLP attached");
+    SimConst &c = s->sc;
+    const int n = s->n;
+    if (n > 0) {
         {
             BBX_PROF_BEGIN("tile_tables");
             k_tile_tables<<<(unsigned)((s->ntiles + kTabWaves - 1) / kTabWaves), kBlock, 0, s->stream>>>(
@@ -821,6 +826,13 @@ int distribute(bbx_solver *s, bool force_full, bool build_buckets, bool with_pre
         BBX_PROF_END();
     }
     return BBX_OK;
+}
+
+int distribute(bbx_solver *s, bool force_full, bool buckets, bool with_pressure, int append_from,
+               int feat_mode, const StepConst *st) {
+    if (feat_mode && !buckets) return fail(BBX_E_INVALID, "warm-start features need a bucket build");
+    BBX_CHECK(distribute_grid(s, force_full, append_from));
+    return buckets ? build_buckets(s, with_pressure, feat_mode, st) : BBX_OK;
 }
 
 int run_density_only(bbx_solver *s) {

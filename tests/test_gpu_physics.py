@@ -114,6 +114,32 @@ def test_pcisph_multistep_drift_reference_exact(bbx, oracle):
     assert np.median(rel) <= 1e-5 and rel.max() <= 1e-3, (np.median(rel), rel.max())
 
 
+@pytest.mark.parametrize("warm", ["none", "auto"])
+def test_pcisph_run_equals_repeated_substeps(bbx, warm):
+    """bbx_pcisph_run(count) is `count` calls of bbx_pcisph_substep: the same
+    state bit for bit and the same per-substep statistics, with and without
+    the network's AUTO start (40 TEXTBOOK substeps into the collapse)."""
+    from bubbles_amd import learned
+
+    sc = small_dam()
+    w = bbx.WARM_AUTO if warm == "auto" else bbx.WARM_NONE
+    a, b = bbx.Solver(sc), bbx.Solver(sc)
+    if warm == "auto":
+        a.attach_mlp(learned.load())
+        b.attach_mlp(learned.load())
+    one = [a.pcisph_substep(sc.dt, 16, 0.01, bbx.MODE_TEXTBOOK, w) for _ in range(40)]
+    run = b.pcisph_run(sc.dt, 40, 16, 0.01, bbx.MODE_TEXTBOOK, w)
+    assert len(run) == 40
+    assert [s.iterations for s in one] == [s.iterations for s in run]
+    assert [s.warm_started for s in one] == [s.warm_started for s in run]
+    assert [s.max_density_error for s in one] == [s.max_density_error for s in run]
+    for f in ("position", "velocity", "pressure", "density"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert b.pcisph_run(sc.dt, 0, 16, 0.01, bbx.MODE_TEXTBOOK, w) == []
+    with pytest.raises(bbx.BbxError):
+        b.pcisph_run(sc.dt, -1, 16, 0.01, bbx.MODE_TEXTBOOK, w)
+
+
 def test_pcisph_textbook_decisions_from_synchronised_states(bbx, oracle):
     """Every 4th of 48 GPU TEXTBOOK substeps, the oracle is reset to the GPU
     state and both run the next substep: the iteration count (the FP64
