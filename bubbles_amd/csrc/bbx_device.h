@@ -72,6 +72,16 @@ __device__ __forceinline__ bool in_bucket(const SimConst &c, float4 a, float4 b)
     return within_std_fp64(a, b, c.h2d);
 }
 
+// |a - b|^2 - off in FP32 with the offset folded into the first FMA.  Every
+// rounding acts on a value of magnitude <= max(|a - b|^2, off), so the
+// result is within 5 * 2^-24 * max(|a - b|^2, off) of the exact value: with
+// off = band_c, a result outside +-band_w has the sign of the exact
+// |a - b|^2 - (h^2 - 1e-8), i.e. decides the cutoff as the FP64 rule does.
+__device__ __forceinline__ float cut_offset(float4 a, float4 b, float off) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, -off)));
+}
+
 // The reference's cutoff rule "zero when |of| < 1e-8 or of > 0"
 // (src/core/kernel.cpp:123-132,181-226) is one comparison: for any
 // non-NaN of it is "inside iff of <= -1e-8"; a NaN is outside.

@@ -446,14 +446,11 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
         };
         // append accepted candidate j (storage index; LDS index j + sh) to the
-        // bucket, up to the cap.  Only the index is handled here: the emission
-        // loops run as long as the wave's longest cell, so the density sum
-        // runs afterwards over the finished bucket, one uniform pass.
-        auto emit = [&](int j, int sh) {
-            if (cnt >= kBucket) {
-                dropped++;
-                return;
-            }
+        // bucket (emit: up to the cap).  Only the index is handled here: the
+        // emission loops run as long as the wave's longest cell, so the
+        // density sum runs afterwards over the finished bucket, one uniform
+        // pass.
+        auto append = [&](int j, int sh) {
             if (kStaged) {
                 qlo = __builtin_amdgcn_alignbit(qhi, qlo, 16);
                 qhi = __builtin_amdgcn_alignbit((unsigned)(j + sh), qhi, 16);
@@ -466,23 +463,34 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             cnt++;
             if ((cnt & 3) == 0) store_quad(cnt, 4);
         };
+        auto emit = [&](int j, int sh) {
+            if (cnt >= kBucket)
+                dropped++;
+            else
+                append(j, sh);
+        };
         if (hc >= 0) {
             int sh[9];
 #pragma unroll
             for (int r = 0; r < 9; r++) sh[r] = kStaged ? tt.rdelta[r] : 0;
+            bool emitted = false;
             if (narrow) {
                 // Pass 1: each row's candidates, four at a time with no
-                // branches, into an acceptance mask.  The FP32 squared
-                // distance accepts below lo2 and rejects above hi2; a
-                // candidate between the two (rare) makes the lane re-test its
-                // row with the reference's FP64 rule.
+                // branches, into an acceptance mask.  The FP32 test uses
+                // t = |pi - pj|^2 - band_c (the offset folded into the first
+                // FMA): t < 0 accepts; a candidate with |t| <= band_w (rare)
+                // makes the lane re-test its row with the reference's FP64
+                // rule.  The sign bits of a group form a nibble that enters a
+                // 64-bit shift register {hi, lo} at its top.
                 unsigned long long M[9];
+                int total = 0;
 #pragma unroll
                 for (int r = 0; r < 9; r++) {
                     const int len = (int)(O[r] >> 16);
-                    unsigned mlo = 0u, mhi = 0u;
+                    unsigned lo = 0u, hi = 0u;
                     float bdist = 3.4e38f;
-                    for (int t0 = 0; t0 < len; t0 += 4) {
+                    int t0 = 0;
+                    for (; t0 < len; t0 += 4) {
                         float4 pc[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
@@ -491,61 +499,67 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                             else
                                 pc[q] = pos[min(A[r] + t0 + q, c.n - 1)];
                         }
-                        unsigned bits = 0u;
+                        float tq[4];
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const float dx = pi.x - pc[q].x, dy = pi.y - pc[q].y, dz = pi.z - pc[q].z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            bits |= (d2 <= c.hi2 ? 1u : 0u) << q;
-                            bdist = fminf(bdist, fabsf(d2 - c.band_c));  // closest to the cutoff
-                        }
-                        if (t0 < 32)
-                            mlo |= bits << t0;
-                        else
-                            mhi |= bits << (t0 - 32);
+                        for (int q = 0; q < 4; q++) tq[q] = cut_offset(pi, pc[q], c.band_c);
+                        bdist = fminf(bdist, fminf(fabsf(tq[0]), fabsf(tq[1])));
+                        bdist = fminf(bdist, fminf(fabsf(tq[2]), fabsf(tq[3])));
+                        unsigned nib = __float_as_uint(tq[3]) >> 31;  // bit q: candidate t0 + q
+                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[2]), 31);
+                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[1]), 31);
+                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[0]), 31);
+                        lo = __builtin_amdgcn_alignbit(hi, lo, 4);
+                        hi = __builtin_amdgcn_alignbit(nib, hi, 4);
                     }
-                    unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+                    // t0 = 4 * groups; candidate t sits at bit 64 - t0 + t
+                    unsigned long long m = ((unsigned long long)hi << 32) | lo;
+                    m = t0 > 0 ? m >> (64 - t0) : 0ull;
                     if (len < 64) m &= (1ull << len) - 1ull;
                     if (bdist <= c.band_w) {
                         // candidates inside the band: the exact FP64 rule
                         for (int t = 0; t < len; t++) {
                             const float4 pj = kStaged ? sx[A[r] + sh[r] + t] : pos[A[r] + t];
-                            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            if (d2 >= c.lo2 && d2 <= c.hi2) {
+                            if (fabsf(cut_offset(pi, pj, c.band_c)) <= c.band_w) {
                                 const unsigned long long bit = 1ull << t;
                                 m = within_std_fp64(pi, pj, c.h2d) ? (m | bit) : (m & ~bit);
                             }
                         }
                     }
                     M[r] = m;
+                    total += __popcll(m);
                 }
                 // Pass 2: emit in the reference's order, cells (y, x, z), chain
                 // order inside each cell.  x runs as a loop with its cell bounds
                 // selected, which keeps the kernel's code (nine emission sites
                 // instead of 27) small.  A narrow row's cells hold <= 32
-                // candidates, so each cell's bits fit one 32-bit word.
+                // candidates, so each cell's bits fit one 32-bit word.  The
+                // bucket cap cannot bind here (a lane over it takes the
+                // cell-by-cell path below, which truncates).
+                if (total <= kBucket) {
+                    emitted = true;
 #pragma unroll
-                for (int dy = 0; dy < 3; dy++)
+                    for (int dy = 0; dy < 3; dy++)
 #pragma unroll 1
-                    for (int dx = 0; dx < 3; dx++)
+                        for (int dx = 0; dx < 3; dx++)
 #pragma unroll
-                        for (int dz = 0; dz < 3; dz++) {
-                            const int r = dz * 3 + dy;
-                            const int lo = dx == 0 ? 0 : (int)((O[r] >> (8 * (dx - 1))) & 0xffu);
-                            const int hi = (int)((O[r] >> (8 * dx)) & 0xffu);
-                            const int w = hi - lo;
-                            if (w <= 0) continue;
-                            unsigned m = (unsigned)(M[r] >> lo);
-                            if (w < 32) m &= (1u << w) - 1u;
-                            while (m) {
-                                const int b = __builtin_ctz(m);
-                                m &= m - 1u;
-                                emit(A[r] + lo + b, sh[r]);
+                            for (int dz = 0; dz < 3; dz++) {
+                                const int r = dz * 3 + dy;
+                                const int lo = dx == 0 ? 0 : (int)((O[r] >> (8 * (dx - 1))) & 0xffu);
+                                const int hi = (int)((O[r] >> (8 * dx)) & 0xffu);
+                                const int w = hi - lo;
+                                if (w <= 0) continue;
+                                unsigned m = (unsigned)(M[r] >> lo);
+                                if (w < 32) m &= (1u << w) - 1u;
+                                while (m) {
+                                    const int b = __builtin_ctz(m);
+                                    m &= m - 1u;
+                                    append(A[r] + lo + b, sh[r]);
+                                }
                             }
-                        }
-            } else {
-                // a wide row or a crowded cell: cell by cell
+                }
+            }
+            if (!emitted) {
+                // a wide row, a crowded cell or a bucket over the cap: cell by cell
                 for (int dy = 0; dy < 3; dy++)
                     for (int dx = 0; dx < 3; dx++) {
                         const int x = u[0] + dx - 1;

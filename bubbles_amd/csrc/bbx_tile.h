@@ -166,6 +166,10 @@ __device__ __forceinline__ void with_staging(bool staged, F f) {
 // (every particle has 25 quads of table).
 constexpr int kNbrDepth = 4;  // quads (16 entries) in flight per lane in the hot gathers
 
+// A quad as loaded: a staged tile's four 16-bit entries stay packed in .x and
+// .y until they are used (quad_unpack), so nothing reads a loaded register
+// right after its load and the loads in flight are never waited for early;
+// otherwise four 32-bit entries.
 template <int D>
 struct NbrQuads {
     int4 q[D];
@@ -176,10 +180,16 @@ __device__ __forceinline__ int4 nbr_quad(const unsigned short *__restrict__ n16,
                                          int stride, int q) {
     const size_t at = (size_t)q * stride + i;
     if (staged) {
-        const ushort4 v = reinterpret_cast<const ushort4 *>(n16)[at];
-        return make_int4(v.x, v.y, v.z, v.w);
+        const uint2 v = reinterpret_cast<const uint2 *>(n16)[at];
+        return make_int4((int)v.x, (int)v.y, 0, 0);
     }
     return reinterpret_cast<const int4 *>(n32)[at];
+}
+
+__device__ __forceinline__ int4 quad_unpack(const int4 &r, bool staged) {
+    if (staged)
+        return make_int4(r.x & 0xffff, (int)((unsigned)r.x >> 16), r.y & 0xffff, (int)((unsigned)r.y >> 16));
+    return r;
 }
 
 // quads q0, q0 + step, ..., q0 + (D - 1) step of particle i (q0 + (D-1) step < 25)
@@ -193,11 +203,12 @@ __device__ __forceinline__ NbrQuads<D> nbr_prefetch(const unsigned short *__rest
     return p;
 }
 
-// The four entries of one quad go through body(j, live) without branches:
-// an entry at or past the bucket count gets j = 0 (a valid index in either
-// index space) and live = false.  Straight-line code lets the four entries'
-// neighbour loads issue together instead of one load-wait per entry.  The
-// body must make a dead entry contribute nothing (or an exact zero).
+// The four entries of one (unpacked) quad go through body(j, live) without
+// branches: an entry at or past the bucket count gets j = 0 (a valid index
+// in either index space) and live = false.  Straight-line code lets the four
+// entries' neighbour loads issue together instead of one load-wait per
+// entry.  The body must make a dead entry contribute nothing (or an exact
+// zero).
 template <typename Body>
 __device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &body) {
     body(k < cnt ? v.x : 0, k < cnt);
@@ -210,7 +221,9 @@ __device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &b
 // the prefetched first D quads and keeping the next D quads in flight while
 // the current D are processed (quads 0, step, 2 step, ... from quad `first0`;
 // step > 1 splits a bucket between lanes).  body(j, live) gets an LDS-local
-// index (staged tile) or a storage index.
+// index (staged tile) or a storage index.  The loads of the next quads are
+// unconditional: a quad past the bucket's last one reloads the last (a cache
+// hit), so no load sits in a branch that would have to wait for it.
 template <int D, int kStep = 1, typename Body>
 __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first,
                                                         const unsigned short *__restrict__ n16,
@@ -218,20 +231,18 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
                                                         int i, int cnt, int stride, Body body,
                                                         int first0 = 0) {
     const int nq = (cnt + 3) >> 2;
+    const int last = nq > 0 ? nq - 1 : 0;
     int4 cur[D];
 #pragma unroll
     for (int d = 0; d < D; d++) cur[d] = first.q[d];
     for (int q = first0; q < nq; q += D * kStep) {
         int4 nxt[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const int qn = q + (D + d) * kStep;
-            nxt[d] = qn < nq ? nbr_quad(n16, n32, staged, i, stride, qn) : make_int4(0, 0, 0, 0);
-        }
+        for (int d = 0; d < D; d++) nxt[d] = nbr_quad(n16, n32, staged, i, stride, min(q + (D + d) * kStep, last));
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const int qc = q + d * kStep;
-            if (qc < nq) quad_body(cur[d], qc * 4, cnt, body);
+            if (qc < nq) quad_body(quad_unpack(cur[d], staged), qc * 4, cnt, body);
         }
 #pragma unroll
         for (int d = 0; d < D; d++) cur[d] = nxt[d];
