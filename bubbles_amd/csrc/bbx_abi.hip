@@ -160,6 +160,8 @@ int build_grid_geom(const bbx_params &p, GridGeom *g) {
         total *= g->n[k];
     }
     if (total > 2.0e9) return fail(BBX_E_RANGE, "grid has more than 2^31 cells");
+    for (int k = 0; k < 3; k++)  // the device's FP32 cell-id decode is exact below 2^22 per axis
+        if (g->n[k] >= (1 << 22)) return fail(BBX_E_RANGE, "grid has 2^22 or more cells along an axis");
     g->total = (unsigned)total;
     return BBX_OK;
 }
@@ -326,6 +328,8 @@ void fill_sim_const(bbx_solver *s) {
     }
     // rows of up to 64 candidates take the two-pass mask path of the bucket
     // build; BBX_BUCKET_ROW_MAX (0 .. 64) lowers the limit to test the other path
+    c.inv_nx = (float)(1.0 / (double)c.g.n[0]);
+    c.inv_nxy = (float)(1.0 / ((double)c.g.n[0] * (double)c.g.n[1]));
     c.row_max = 64;
     if (const char *e = std::getenv("BBX_BUCKET_ROW_MAX")) {
         int v = std::atoi(e);

@@ -105,6 +105,7 @@ struct SimConst {
     int tile_cap;      // tiles with more staged candidates read global memory
     int tile_cap2;     // the same for kernels that stage two arrays
     int row_max;       // bucket build: rows up to this many candidates use the mask path (<= 64)
+    float inv_nx, inv_nxy;  // 1 / n[0], 1 / (n[0] n[1]) (cell id decode)
     int ncoll;         // 0: the tank (reversed box above); else the collider list
     const DevCollider *coll;
 };

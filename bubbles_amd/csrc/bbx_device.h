@@ -45,6 +45,36 @@ __device__ __forceinline__ void unlinear(const GridGeom &g, int id, int u[3]) {
     u[0] = rem - u[1] * g.n[0];
 }
 
+// unlinear with the two divisions replaced by FP32 reciprocal products and
+// one correction each: the FP32 quotient of a cell id (< 2^31) by n is off
+// by less than one for every grid whose quotient fits 2^22, so one +-1 step
+// makes it exact.
+__device__ __forceinline__ void unlinear_fast(const GridGeom &g, float inv_nx, float inv_nxy, int id,
+                                              int u[3]) {
+    const int nx = g.n[0], nxy = g.n[0] * g.n[1];
+    int z = (int)((float)id * inv_nxy);
+    int rem = id - z * nxy;
+    if (rem < 0) {
+        z--;
+        rem += nxy;
+    } else if (rem >= nxy) {
+        z++;
+        rem -= nxy;
+    }
+    int y = (int)((float)rem * inv_nx);
+    int x = rem - y * nx;
+    if (x < 0) {
+        y--;
+        x += nx;
+    } else if (x >= nx) {
+        y++;
+        x -= nx;
+    }
+    u[0] = x;
+    u[1] = y;
+    u[2] = z;
+}
+
 // IsWithinStd(Distance(pi, pj), h) in FP64 (src/core/kernel.cpp:229-234,
 // src/core/geometry.h:632-633): d = sqrt(dx^2+dy^2+dz^2), d2 = d*d,
 // inside iff !(|d2 - h2| < 1e-8 || d2 - h2 > 0).

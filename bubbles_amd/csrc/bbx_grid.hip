@@ -374,8 +374,8 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     // tile table or the staging: they are requested first.  Row r =
     // (dz+1)*3 + (dy+1): the cells x-1, x, x+1 of a row are contiguous in
     // storage; A[r] = storage start of cell x-1, and O[r] packs the offsets
-    // of cells x, x+1 and of the row end as bytes (rows outside the grid are
-    // empty).  A row of at most 64 candidates whose cells hold at most 32
+    // of cells x-1 (zero), x, x+1 and of the row end as bytes 0-3 (rows
+    // outside the grid are empty).  A row of at most 64 candidates whose cells hold at most 32
     // each is "narrow": it is tested into a bit mask; otherwise the particle
     // walks cell by cell.
     const int nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2], nxy = nx * ny;
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     }
     if (valid) pi = pos[i];
     if (hc >= 0) {
-        unlinear(c.g, hc, u);
+        unlinear_fast(c.g, c.inv_nx, c.inv_nxy, hc, u);
 #pragma unroll
         for (int r = 0; r < 9; r++) {
             const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                 const int b0 = u[0] > 0 ? start[cx - 1] : b1;
                 const int b3 = u[0] + 1 < nx ? start[cx + 2] : b2;
                 A[r] = b0;
-                O[r] = (unsigned)(b1 - b0) | ((unsigned)(b2 - b0) << 8) | ((unsigned)(b3 - b0) << 16);
+                O[r] = ((unsigned)(b1 - b0) << 8) | ((unsigned)(b2 - b0) << 16) | ((unsigned)(b3 - b0) << 24);
                 narrow = narrow && (b3 - b0 <= c.row_max) && (b1 - b0 <= 32) && (b2 - b1 <= 32) &&
                          (b3 - b2 <= 32);
             }
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                 int total = 0;
 #pragma unroll
                 for (int r = 0; r < 9; r++) {
-                    const int len = (int)(O[r] >> 16);
+                    const int len = (int)(O[r] >> 24);
                     unsigned lo = 0u, hi = 0u;
                     float bdist = 3.4e38f;
                     int t0 = 0;
@@ -502,8 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                         float tq[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) tq[q] = cut_offset(pi, pc[q], c.band_c);
-                        bdist = fminf(bdist, fminf(fabsf(tq[0]), fabsf(tq[1])));
-                        bdist = fminf(bdist, fminf(fabsf(tq[2]), fabsf(tq[3])));
+                        bdist = fminf(fminf(bdist, fabsf(tq[0])), fabsf(tq[1]));
+                        bdist = fminf(fminf(bdist, fabsf(tq[2])), fabsf(tq[3]));
                         unsigned nib = __float_as_uint(tq[3]) >> 31;  // bit q: candidate t0 + q
                         nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[2]), 31);
                         nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[1]), 31);
@@ -544,12 +544,12 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
 #pragma unroll
                             for (int dz = 0; dz < 3; dz++) {
                                 const int r = dz * 3 + dy;
-                                const int lo = dx == 0 ? 0 : (int)((O[r] >> (8 * (dx - 1))) & 0xffu);
-                                const int hi = (int)((O[r] >> (8 * dx)) & 0xffu);
+                                const int lo = (int)__builtin_amdgcn_ubfe(O[r], 8 * dx, 8);
+                                const int hi = (int)__builtin_amdgcn_ubfe(O[r], 8 * dx + 8, 8);
                                 const int w = hi - lo;
                                 if (w <= 0) continue;
                                 unsigned m = (unsigned)(M[r] >> lo);
-                                if (w < 32) m &= (1u << w) - 1u;
+                                if (w < 32) m = __builtin_amdgcn_ubfe(m, 0, w);
                                 while (m) {
                                     const int b = __builtin_ctz(m);
                                     m &= m - 1u;
