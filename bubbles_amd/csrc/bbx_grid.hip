@@ -289,38 +289,51 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
 // ---------------------------------------------------------------------------
 // 5. tile tables: one wave per tile of kBlock particles
 // ---------------------------------------------------------------------------
-// The tile's home cells span [cmin, cmax] (min and max over its particles;
-// storage is sorted by cell, except particles in no chain at the end); its
-// nine neighbour rows become at most nine merged storage intervals
-// (tab_row, tab_merge).  Every gather of the substep reads this table.
+// The tile's home cells span [cmin, cmax]; its nine neighbour rows become at
+// most nine merged storage intervals (tab_row, tab_merge).  Every gather of
+// the substep reads this table.  Storage is sorted by cell except for the
+// particles in no chain (home -1) at its end, so unless the tile's last
+// particle is one of those, cmin and cmax are the homes of its first and
+// last particles (otherwise a wave reduction over the tile finds them).  The
+// row ranges of lanes 0-8 are gathered by shuffles and merged by lane 0.
 constexpr int kTabWaves = kBlock / 64;
 __global__ void __launch_bounds__(kBlock) k_tile_tables(SimConst c, const int *__restrict__ home,
                                                         const int *__restrict__ start, int ntiles,
                                                         int *__restrict__ tile_tab) {
     __shared__ TileTab tt[kTabWaves];
-    __shared__ int row_a[kTabWaves][9], row_b[kTabWaves][9];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tile = blockIdx.x * kTabWaves + w;  // wave-uniform
-    int lo = 0x7fffffff, hi = -1;
-    if (tile < ntiles) {
+    if (tile >= ntiles) return;
+    const int t0 = tile * kBlock, t1 = min(t0 + kBlock, c.n) - 1;
+    int lo = 0x7fffffff, hi = home[t1];
+    if (hi >= 0) {
+        lo = home[t0];
+    } else {
 #pragma unroll
         for (int k = 0; k < kBlock / 64; k++) {
-            const int i = tile * kBlock + k * 64 + lane;
+            const int i = t0 + k * 64 + lane;
             const int hc = i < c.n ? home[i] : -1;
             if (hc >= 0) {
                 lo = min(lo, hc);
                 hi = max(hi, hc);
             }
         }
+        lo = wave_min_i32(lo);
+        hi = wave_max_i32(hi);
     }
-    lo = wave_min_i32(lo);
-    hi = wave_max_i32(hi);
-    if (tile < ntiles && lane < 9) tab_row(c.g, hi >= 0 ? lo : 0, hi, start, lane, row_a[w][lane], row_b[w][lane]);
-    __syncthreads();
-    if (tile < ntiles && lane == 0) tab_merge(row_a[w], row_b[w], c.tile_cap, tt[w]);
-    __syncthreads();
-    if (tile < ntiles && lane < kTabWords)
-        tile_tab[(size_t)tile * kTabInts + lane] = reinterpret_cast<const int *>(&tt[w])[lane];
+    int a = -1, b = -1;
+    if (lane < 9) tab_row(c.g, hi >= 0 ? lo : 0, hi, start, lane, a, b);
+    int ra[9], rb[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        ra[r] = __shfl(a, r, 64);
+        rb[r] = __shfl(b, r, 64);
+    }
+    if (lane == 0 && !tab_merge_ordered(ra, rb, c.tile_cap, tt[w])) tab_merge(ra, rb, c.tile_cap, tt[w]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < kTabWords) tile_tab[(size_t)tile * kTabInts + lane] = reinterpret_cast<const int *>(&tt[w])[lane];
 }
 
 // ---------------------------------------------------------------------------

@@ -147,6 +147,53 @@ __device__ __forceinline__ void tab_merge(const int *row_a, const int *row_b, in
                 if (row_a[r] >= t.start[q] && row_a[r] < t.start[q] + t.len[q]) rdelta[r] = t.off[q] - t.start[q];
 }
 
+// tab_merge for the common case where the non-empty rows' ranges come in
+// order of their starts (row r's cells begin dy*nx + dz*nx*ny - 1 past the
+// tile's first cell, increasing with r when the grid has three or more cells
+// along y, and start[] is non-decreasing): one pass in row order merges them
+// into the same table as tab_merge.  Returns false (table untouched) when
+// the rows are out of order; tab_merge then sorts them.
+__device__ __forceinline__ bool tab_merge_ordered(const int *row_a, const int *row_b, int cap, TileTab &t) {
+    int prev = -1;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        if (row_a[r] < 0) continue;
+        if (row_a[r] < prev) return false;
+        prev = row_a[r];
+    }
+    int k = -1, end = 0, off = 0;
+    int ivl[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        const int a = row_a[r], b = row_b[r];
+        ivl[r] = -1;
+        if (a < 0) continue;
+        if (k >= 0 && a <= end) {
+            end = max(end, b);
+        } else {
+            if (k >= 0) {
+                t.len[k] = end - t.start[k];
+                off += t.len[k];
+            }
+            k++;
+            t.start[k] = a;
+            t.off[k] = off;
+            end = b;
+        }
+        ivl[r] = k;
+    }
+    if (k >= 0) {
+        t.len[k] = end - t.start[k];
+        off += t.len[k];
+    }
+    t.n_ivl = k + 1;
+    t.total = off;
+    t.staged = off <= cap ? 1 : 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) t.rdelta[r] = ivl[r] >= 0 ? t.off[ivl[r]] - t.start[ivl[r]] : 0;
+    return true;
+}
+
 // Run f(std::true_type) for an LDS-staged tile and f(std::false_type)
 // otherwise, so code inside f that reads neighbours from LDS or from global
 // memory is compiled separately and never selects between the two address
