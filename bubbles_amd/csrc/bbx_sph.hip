@@ -134,13 +134,13 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
         if (kWarm) qi = xq[i].w;
         first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
     }
-    // staged through registers: the (velocity, density) pair is assembled on
+    // staged through registers: the (velocity, 1 / density) pair is assembled on
     // the way, and seven 4-byte LDS DMA streams measured slower here
     if (in_lds) {
         tab_stage(tt, sx, [&](int j) { return sxsrc[j]; });
         tab_stage(tt, sv, [&](int j) {
             const float4 v = vel[j];
-            return make_float4(v.x, v.y, v.z, rho[j]);
+            return make_float4(v.x, v.y, v.z, fast_rcp(rho[j]));  // 1 / rho_j, once per staged particle
         });
     }
     __syncthreads();
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
                 const int g = staged ? tab_global(tt, j) : j;
                 a = sxsrc[g];
                 const float4 v = vel[g];
-                b = make_float4(v.x, v.y, v.z, rho[g]);
+                b = make_float4(v.x, v.y, v.z, fast_rcp(rho[g]));
             }
             const float dx = xi.x - a.x, dy = xi.y - a.y, dz = xi.z - a.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
             const bool in = live & spiky_in(c, r);
             const float x = keep_if(in, 1.f - r * c.inv_h);
             // viscosity term mu m^2 (v_j - v_i) d2W(r) / rho_j
-            const float s = coef * (c.spiky_lap * x) * fast_rcp(b.w);
+            const float s = coef * (c.spiky_lap * x) * b.w;
             fx += (b.x - vi.x) * s;
             fy += (b.y - vi.y) * s;
             fz += (b.z - vi.z) * s;
