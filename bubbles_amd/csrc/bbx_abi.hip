@@ -391,12 +391,13 @@ static void free_all(bbx_solver *s) {
     dfree(s->nbr16); dfree(s->nbr32); dfree(s->nbr_cnt); dfree(s->tile_tab);
     dfree(s->rho); dfree(s->pres); dfree(s->rho_star); dfree(s->xq);
     dfree(s->force); dfree(s->fp); dfree(s->xstar); dfree(s->feat); dfree(s->pwarm);
-    dfree(s->loop); dfree(s->max_force); dfree(s->flag); dfree(s->counters);
+    dfree(s->step); dfree(s->flag);
+    s->loop = nullptr; s->max_force = nullptr; s->counters = nullptr;
     dfree(s->coll);
     s->ncoll = 0;
-    if (s->h_loop) (void)hipHostFree(s->h_loop);
-    if (s->h_counters) (void)hipHostFree(s->h_counters);
-    if (s->h_max_force) (void)hipHostFree(s->h_max_force);
+    if (s->h_step) (void)hipHostFree(s->h_step);
+    if (s->h_flag) (void)hipHostFree(s->h_flag);
+    s->h_step = nullptr; s->h_flag = nullptr;
     s->h_loop = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
     for (int k = 0; k < 2; k++) {
         if (s->ev0[k]) (void)hipEventDestroy(s->ev0[k]);
@@ -534,16 +535,22 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
     if (!rc) rc = dalloc(&s->xstar, N);
     if (!rc) rc = dalloc(&s->feat, N * BBX_NFEAT);
     if (!rc) rc = dalloc(&s->pwarm, N);
-    if (!rc) rc = dalloc(&s->loop, 1);
-    if (!rc) rc = dalloc(&s->max_force, kSlots);
+    if (!rc) rc = dalloc(&s->step, 1);
+    if (!rc) {
+        s->loop = &s->step->loop;
+        s->max_force = s->step->max_force;
+        s->counters = s->step->counters;
+    }
     if (!rc) rc = dalloc(&s->flag, 1);
-    if (!rc) rc = dalloc(&s->counters, CNT_N);
-    if (!rc && hipHostMalloc((void **)&s->h_loop, sizeof(LoopState)) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_step, sizeof(StepBlock)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && hipHostMalloc((void **)&s->h_counters, sizeof(int) * (CNT_N + 1)) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_flag, sizeof(int)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && hipHostMalloc((void **)&s->h_max_force, sizeof(unsigned) * kSlots) != hipSuccess)
-        rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc) {
+        s->h_loop = &s->h_step->loop;
+        s->h_max_force = s->h_step->max_force;
+        s->h_counters = s->h_step->counters;
+    }
     for (int k = 0; k < 2 && !rc; k++)
         if (hipEventCreate(&s->ev0[k]) != hipSuccess || hipEventCreate(&s->ev1[k]) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_read[k], hipEventDisableTiming) != hipSuccess)
@@ -855,9 +862,9 @@ int bbx_solver_upload(bbx_solver *s, int field, const double *in) {
 int bbx_solver_get_flag(bbx_solver *s, int *flag) {
     BBX_CHECK(check_solver(s));
     if (!flag) return fail(BBX_E_INVALID, "null flag");
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_flag, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
-    *flag = s->h_counters[0];
+    *flag = *s->h_flag;
     return BBX_OK;
 }
 
@@ -871,16 +878,10 @@ int bbx_solver_set_flag(bbx_solver *s, int flag) {
 // One readback per call: counters, max |f| and (PCISPH) the loop state.
 // iterations = index of the first iteration whose done flag is set, + 1;
 // the reported error is that iteration's FP64 max error / rho0.
-// Queue the copies of the step's counters (and of the loop record, for a
-// PCISPH loop of loop_len launches) into pinned host memory.
-static int queue_stats(bbx_solver *s, int loop_len) {
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters, s->counters, sizeof(int) * CNT_N, hipMemcpyDeviceToHost,
-                                 s->stream));
-    if (loop_len > 0)
-        BBX_HIP_CHECK(hipMemcpyAsync(s->h_loop, s->loop, offsetof(LoopState, slot),
-                                     hipMemcpyDeviceToHost, s->stream));
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
-                                 hipMemcpyDeviceToHost, s->stream));
+// Queue the copy of the step's counters, max-force slots and loop record
+// into pinned host memory (one copy: they are one block).
+static int queue_stats(bbx_solver *s) {
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_step, s->step, kStepReadback, hipMemcpyDeviceToHost, s->stream));
     return BBX_OK;
 }
 
@@ -906,7 +907,7 @@ static void parse_stats(bbx_solver *s, bbx_stats *st, int loop_len) {
 }
 
 static int read_stats(bbx_solver *s, bbx_stats *st, int loop_len = 0) {
-    BBX_CHECK(queue_stats(s, loop_len));
+    BBX_CHECK(queue_stats(s));
     BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
     if (st) parse_stats(s, st, loop_len);
     return BBX_OK;
@@ -1192,9 +1193,7 @@ static int check_substep(bbx_solver *s, double dt, int max_it, int warm) {
 // predict-correct loop, XSPH, and the statistics copied to pinned memory.
 static int substep_front(bbx_solver *s, int slot) {
     BBX_HIP_CHECK(hipEventRecord(s->ev0[slot], s->stream));
-    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
-    BBX_CHECK(launch_fill_int(s, (int *)s->loop, 0, sizeof(LoopState) / sizeof(int)));
+    BBX_CHECK(launch_fill_int(s, (int *)s->step, 0, sizeof(StepBlock) / sizeof(int)));
     return distribute_grid(s, false, -1);
 }
 
@@ -1263,16 +1262,16 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
         }
         launches = goal;
         if (launches >= cap) break;
-        BBX_HIP_CHECK(hipMemcpyAsync(s->h_counters + CNT_N, &s->loop->done[launches - 1], sizeof(int),
+        BBX_HIP_CHECK(hipMemcpyAsync(s->h_flag, &s->loop->done[launches - 1], sizeof(int),
                                      hipMemcpyDeviceToHost, s->stream));
         BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
-        if (s->h_counters[CNT_N]) break;
+        if (*s->h_flag) break;
         goal = launches + (launches > 4 ? launches / 2 : 2);
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
     BBX_HIP_CHECK(hipEventRecord(s->ev1[slot], s->stream));
-    BBX_CHECK(queue_stats(s, launches));
+    BBX_CHECK(queue_stats(s));
     BBX_HIP_CHECK(hipEventRecord(s->ev_read[slot], s->stream));
     plan->warm = warm;
     plan->launches = launches;
@@ -1351,8 +1350,7 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     if (s->kind != BBX_SPH) return fail(BBX_E_INVALID, "not an SPH solver");
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
     BBX_HIP_CHECK(hipEventRecord(s->ev0[0], s->stream));
-    BBX_CHECK(launch_fill_int(s, s->counters, 0, CNT_N));
-    BBX_CHECK(launch_fill_int(s, (int *)s->max_force, 0, kSlots));
+    BBX_CHECK(launch_fill_int(s, (int *)s->step, 0, sizeof(StepBlock) / sizeof(int)));
     BBX_CHECK(distribute(s, false, true, true));
     StepConst st{};
     st.dt = (float)dt;

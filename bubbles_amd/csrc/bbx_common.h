@@ -141,6 +141,18 @@ enum {
     CNT_N = 8
 };
 
+// The per-substep device state in one allocation: cleared by one fill at the
+// start of a substep and read back by one copy (the first kStepReadback
+// bytes: counters, max-force slots, the loop's per-iteration errors and done
+// flags); the per-slot error maxima follow.
+struct StepBlock {
+    int counters[CNT_N];
+    unsigned max_force[kSlots];
+    LoopState loop;
+};
+constexpr size_t kStepReadback = offsetof(StepBlock, loop) + offsetof(LoopState, slot);
+static_assert(sizeof(StepBlock) % sizeof(int) == 0, "StepBlock is cleared as ints");
+
 // ---------------------------------------------------------------------------
 // kernel profiler (HIP events on the solver stream)
 // ---------------------------------------------------------------------------
@@ -234,17 +246,18 @@ struct bbx_solver {
     float *feat = nullptr;       // [16][stride]
     float *pwarm = nullptr;      // MLP output
     int keep_features = 0;       // warm substeps also store their features (bbx_solver_keep_features)
-    // reductions / flags
+    // reductions / flags: loop, max_force and counters point into one
+    // StepBlock allocation, h_loop, h_max_force and h_counters into its
+    // pinned copy
+    bbx::StepBlock *step = nullptr, *h_step = nullptr;
     bbx::LoopState *loop = nullptr;          // predict-correct loop state (device)
-    bbx::LoopState *h_loop = nullptr;        // pinned readback
-    unsigned long long *max_err = nullptr;   // FP64 bits of max |error|
+    bbx::LoopState *h_loop = nullptr;
     unsigned *max_force = nullptr;           // kSlots FP32 bits of max |f|
-    int *flag = nullptr;                     // requiresHigherLevelUpdate
     int *counters = nullptr;                 // CNT_*
-    // pinned readback
-    unsigned long long *h_max_err = nullptr;
-    int *h_counters = nullptr;               // CNT_N counters + one loop flag
+    int *h_counters = nullptr;
     unsigned *h_max_force = nullptr;
+    int *flag = nullptr;                     // requiresHigherLevelUpdate
+    int *h_flag = nullptr;                   // pinned: one int read back alone
     // substep timing: start and end of the queued work, and the end of the
     // statistics readback; two sets, so that a run of PCISPH substeps can
     // queue the next substep's grid work before it reads this one's

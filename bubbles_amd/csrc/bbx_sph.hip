@@ -135,13 +135,22 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
         first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
     }
     // staged through registers: the (velocity, 1 / density) pair is assembled on
-    // the way, and seven 4-byte LDS DMA streams measured slower here
+    // the way, and seven 4-byte LDS DMA streams measured slower here; both
+    // arrays in one pass, so each slot's storage index is found once
     if (in_lds) {
-        tab_stage(tt, sx, [&](int j) { return sxsrc[j]; });
-        tab_stage(tt, sv, [&](int j) {
-            const float4 v = vel[j];
-            return make_float4(v.x, v.y, v.z, fast_rcp(rho[j]));  // 1 / rho_j, once per staged particle
-        });
+        struct Slot {
+            float4 a, b;
+        };
+        tab_stage_with(
+            tt,
+            [&](int j) {
+                const float4 v = vel[j];
+                return Slot{sxsrc[j], make_float4(v.x, v.y, v.z, fast_rcp(rho[j]))};  // 1 / rho_j once
+            },
+            [&](int e, const Slot &v) {
+                sx[e] = v.a;
+                sv[e] = v.b;
+            });
     }
     __syncthreads();
     if (!valid) return;  // the lane pair shares validity

@@ -347,7 +347,7 @@ __device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); 
 template <typename Fn, typename Store>
 __device__ __forceinline__ void tab_stage_with(const TileTab &t, Fn fn, Store store) {
     using T = decltype(fn(0));
-    constexpr int kBatch = 8;
+    constexpr int kBatch = sizeof(T) > 16 ? 4 : 8;  // registers held per batch
     const int total = t.total, nivl = t.n_ivl;
     int off[kMaxIvl], shift[kMaxIvl];  // LDS offset and storage - LDS shift of each interval
 #pragma unroll
