@@ -123,6 +123,15 @@ __device__ __forceinline__ float poly6_w(const SimConst &c, float d2) {
     return cutoff_in(d2 - c.h2) ? c.poly6 * x * x * x : 0.f;
 }
 
+// poly6 W of a bucket member at the positions its bucket was built from:
+// the FP64 rule put it at d^2 <= h^2 - 1e-8, where the FP32 d2 is below h^2
+// by far more than its rounding, so the kernel's own cutoff can only drop a
+// term of relative size (1e-8 / h^2)^3 and is not evaluated.
+__device__ __forceinline__ float poly6_w_member(const SimConst &c, float d2) {
+    const float x = 1.f - d2 * c.inv_h2;
+    return c.poly6 * x * x * x;
+}
+
 // spiky cutoff (src/core/kernel.cpp:181-226): zero when |r-h| < 1e-8 or r > h
 __device__ __forceinline__ bool spiky_in(const SimConst &c, float r) { return cutoff_in(r - c.h); }
 

@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             const float4 pj = kStaged ? sx[j] : pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            const float w = poly6_w(c, d2);
+            const float w = poly6_w_member(c, d2);
             kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
             if constexpr (kFeat > 0) {
                 // distance moments and centroid over the members j != i
