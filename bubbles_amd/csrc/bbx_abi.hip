@@ -397,7 +397,9 @@ static void free_all(bbx_solver *s) {
     s->ncoll = 0;
     if (s->h_step) (void)hipHostFree(s->h_step);
     if (s->h_flag) (void)hipHostFree(s->h_flag);
+    if (s->h_signal) (void)hipHostFree(s->h_signal);
     s->h_step = nullptr; s->h_flag = nullptr;
+    s->h_signal = nullptr; s->d_signal = nullptr;
     s->h_loop = nullptr; s->h_counters = nullptr; s->h_max_force = nullptr;
     for (int k = 0; k < 2; k++) {
         if (s->ev0[k]) (void)hipEventDestroy(s->ev0[k]);
@@ -546,6 +548,13 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_flag, sizeof(int)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc && hipHostMalloc((void **)&s->h_signal, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        rc = fail(BBX_E_NOMEM, "pinned alloc");
+    if (!rc) {
+        *s->h_signal = 0u;
+        if (hipHostGetDevicePointer((void **)&s->d_signal, s->h_signal, 0) != hipSuccess)
+            rc = fail(BBX_E_HIP, "device address of the loop signal");
+    }
     if (!rc) {
         s->h_loop = &s->h_step->loop;
         s->h_max_force = s->h_step->max_force;
@@ -1202,8 +1211,41 @@ struct SubstepPlan {
     int launches = 0;          // loop launches queued
 };
 
+// Wait for the pressure-force kernel of iteration k (queued, of the current
+// substep) to decide; *done = whether it integrates.  The host spins on the
+// pinned word the kernel stores its decision in.  Should the stream drain
+// without it (a decision the host cannot see), the done flag is read back
+// through a copy instead.
+static int wait_loop_decision(bbx_solver *s, int k, int *done) {
+    const unsigned want = s->loop_seq & 0x7fffffu;
+    auto decided = [&](unsigned w) { return (w >> 9) == want && ((w & 1u) || (int)((w >> 1) & 0xffu) >= k); };
+    for (unsigned spin = 1;; spin++) {
+        unsigned w = __atomic_load_n(s->h_signal, __ATOMIC_ACQUIRE);
+        if (decided(w)) {
+            *done = (int)(w & 1u) && (int)((w >> 1) & 0xffu) <= k;
+            return BBX_OK;
+        }
+        if ((spin & 4095u) == 0u) {
+            const hipError_t q = hipStreamQuery(s->stream);
+            if (q == hipErrorNotReady) continue;
+            BBX_HIP_CHECK(q);
+            w = __atomic_load_n(s->h_signal, __ATOMIC_ACQUIRE);
+            if (decided(w)) {
+                *done = (int)(w & 1u) && (int)((w >> 1) & 0xffu) <= k;
+                return BBX_OK;
+            }
+            break;
+        }
+    }
+    BBX_HIP_CHECK(hipMemcpyAsync(s->h_flag, &s->loop->done[k], sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    *done = *s->h_flag;
+    return BBX_OK;
+}
+
 static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double tol, int mode, int warm,
                         SubstepPlan *plan) {
+    s->loop_seq++;  // numbers this substep's loop decisions
     if (warm == BBX_WARM_AUTO) {
         // The warm start costs about two thirds of a correction, so it only
         // pays where the plain loop needs several.  While the flow is calm
@@ -1240,33 +1282,35 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
         BBX_CHECK(build_buckets(s, false, 0, &st));
         BBX_CHECK(run_nonpressure(s, st, 1));
     }
-    // The predict-correct loop is queued without per-iteration host round
-    // trips: the pressure-force kernel of iteration k reads that iteration's
-    // FP64 error on the device and either integrates or predicts iteration
-    // k + 1; kernels of later iterations exit at entry.  The host queues as
-    // many iterations as the previous substep needed plus one; only when
-    // those have not integrated does it wait once and queue more, so a high
-    // cap costs nothing while it is not reached.  REFERENCE_EXACT stops after
-    // the first iteration (its error is never recorded, so it never converges
-    // earlier and never runs longer).
+    // The predict-correct loop: the pressure-force kernel of iteration k
+    // reads that iteration's FP64 error on the device and either integrates
+    // or predicts iteration k + 1, and publishes that decision to the host
+    // as soon as it has taken it (the first workgroup, at kernel start).
+    // The host queues one iteration at a time: it waits for the decision of
+    // the last queued one and, if the loop goes on, queues the next while
+    // that pressure-force kernel is still running, so the device does not
+    // wait for the host and no launch is queued that is not needed.  Once
+    // the loop has integrated, the rest of the substep and the next
+    // substep's grid work are queued behind the integrating kernel in the
+    // same way.  REFERENCE_EXACT stops after the first iteration (its error
+    // is never recorded, so it never converges earlier and never runs
+    // longer).
     const int exact = mode == BBX_MODE_REFERENCE_EXACT ? 1 : 0;
     const int cap = exact ? 1 : max_it;
     int launches = 0;
-    int goal = s->last_iterations > 0 ? s->last_iterations + 1 : 2;
+    int goal = s->last_iterations > 1 ? s->last_iterations - 1 : 1;
     for (;;) {
         goal = goal < cap ? goal : cap;
-        for (int k = launches; k < goal; k++) {
-            st.first = (k == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
-            BBX_CHECK(run_predict_pressure(s, st, k));
-            BBX_CHECK(run_pressure_force(s, st, k, max_it, exact, tol));
+        for (; launches < goal; launches++) {
+            st.first = (launches == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
+            BBX_CHECK(run_predict_pressure(s, st, launches));
+            BBX_CHECK(run_pressure_force(s, st, launches, max_it, exact, tol));
         }
-        launches = goal;
         if (launches >= cap) break;
-        BBX_HIP_CHECK(hipMemcpyAsync(s->h_flag, &s->loop->done[launches - 1], sizeof(int),
-                                     hipMemcpyDeviceToHost, s->stream));
-        BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
-        if (*s->h_flag) break;
-        goal = launches + (launches > 4 ? launches / 2 : 2);
+        int done = 0;
+        BBX_CHECK(wait_loop_decision(s, launches - 1, &done));
+        if (done) break;
+        goal = launches + 1;
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));

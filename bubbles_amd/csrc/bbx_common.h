@@ -258,10 +258,17 @@ struct bbx_solver {
     unsigned *h_max_force = nullptr;
     int *flag = nullptr;                     // requiresHigherLevelUpdate
     int *h_flag = nullptr;                   // pinned: one int read back alone
+    // Loop decisions published to the host as they are taken: the pressure-
+    // force kernel of iteration k stores (seq << 9) | (k << 1) | integrate
+    // into this pinned word (d_signal is its device address) when it reads
+    // the iteration's error; seq numbers the substep.
+    unsigned *h_signal = nullptr, *d_signal = nullptr;
+    unsigned loop_seq = 0;
     // substep timing: start and end of the queued work, and the end of the
     // statistics readback; two sets, so that a run of PCISPH substeps can
     // queue the next substep's grid work before it reads this one's
     hipEvent_t ev0[2] = {}, ev1[2] = {}, ev_read[2] = {};
+
 
     bbx_mlp *mlp = nullptr;
     bbx::Profiler prof;

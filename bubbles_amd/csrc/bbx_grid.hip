@@ -596,7 +596,11 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         // the table this lane has just written (its own quads: no other lane
         // or wave writes them, and no earlier load in this kernel cached them)
         __builtin_amdgcn_s_waitcnt(0);
-        // the particle's own index in the bucket (its LDS slot in the centre row)
+        // The particle's own index (its LDS slot in the centre row).  Dead
+        // entries of the last quad read the particle itself: its staged copy
+        // is its position bit for bit, so they add exact zeros to the
+        // feature sums below, as the particle's own entry does, and only
+        // the density masks them.
         const int self = kStaged ? i + tt.rdelta[4] : i;
         for_each_neighbour<kNbrDepth>(nbr16, nbr32, kStaged, i, cnt, c.stride, [&](int j, bool live) {
             const float4 pj = kStaged ? sx[j] : pos[j];
@@ -605,17 +609,17 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             const float w = poly6_w_member(c, d2);
             kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero
             if constexpr (kFeat > 0) {
-                // distance moments and centroid over the members j != i
-                const bool fl = live & (j != self);
-                const float u = keep_if(fl, fast_sqrt(d2) * c.inv_h);
-                m1 += u;
-                m2 += u * u;
-                m3 += u * u * u;
-                cx -= keep_if(fl, dx * c.inv_h);
-                cy -= keep_if(fl, dy * c.inv_h);
-                cz -= keep_if(fl, dz * c.inv_h);
+                // distance moments r, r^2, r^3 and the centroid offset over
+                // the members (powers of 1/h applied once afterwards)
+                const float r = fast_sqrt(d2);
+                m1 += r;
+                m2 += d2;
+                m3 = fmaf(r, d2, m3);
+                cx -= dx;
+                cy -= dy;
+                cz -= dz;
             }
-        });
+        }, self);
         nbr_cnt[i] = cnt;
         di = sum * c.mass;
         rho[i] = di;
@@ -631,12 +635,16 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                 wall = fminf(wall, fminf(lo, hi));
             }
             const float vs = wi.st.dt * c.inv_h;
+            const float s1 = c.inv_h * inv, s2 = s1 * c.inv_h, s3 = s2 * c.inv_h;
+            cx *= c.inv_h;
+            cy *= c.inv_h;
+            cz *= c.inv_h;
             fv[BBX_F_DENSITY_ERR] = di / c.rho0 - 1.f;
             fv[BBX_F_RESERVED1] = 0.f;
             fv[BBX_F_COUNT] = (float)cnt;
-            fv[BBX_F_M1] = m1 * inv;
-            fv[BBX_F_M2] = m2 * inv;
-            fv[BBX_F_M3] = m3 * inv;
+            fv[BBX_F_M1] = m1 * s1;
+            fv[BBX_F_M2] = m2 * s2;
+            fv[BBX_F_M3] = m3 * s3;
             fv[BBX_F_CX] = cx * inv;
             fv[BBX_F_CY] = cy * inv;
             fv[BBX_F_CZ] = cz * inv;

@@ -251,17 +251,18 @@ __device__ __forceinline__ NbrQuads<D> nbr_prefetch(const unsigned short *__rest
 }
 
 // The four entries of one (unpacked) quad go through body(j, live) without
-// branches: an entry at or past the bucket count gets j = 0 (a valid index
-// in either index space) and live = false.  Straight-line code lets the four
-// entries' neighbour loads issue together instead of one load-wait per
-// entry.  The body must make a dead entry contribute nothing (or an exact
-// zero).
+// branches: an entry at or past the bucket count gets j = dead (0 by
+// default; any valid index in the entry's index space) and live = false.
+// Straight-line code lets the four entries' neighbour loads issue together
+// instead of one load-wait per entry.  The body must make a dead entry
+// contribute nothing (or an exact zero); with dead = the particle's own
+// index, a sum over x_j - x_i gets exact zeros from dead entries unmasked.
 template <typename Body>
-__device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &body) {
-    body(k < cnt ? v.x : 0, k < cnt);
-    body(k + 1 < cnt ? v.y : 0, k + 1 < cnt);
-    body(k + 2 < cnt ? v.z : 0, k + 2 < cnt);
-    body(k + 3 < cnt ? v.w : 0, k + 3 < cnt);
+__device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &body, int dead) {
+    body(k < cnt ? v.x : dead, k < cnt);
+    body(k + 1 < cnt ? v.y : dead, k + 1 < cnt);
+    body(k + 2 < cnt ? v.z : dead, k + 2 < cnt);
+    body(k + 3 < cnt ? v.w : dead, k + 3 < cnt);
 }
 
 // Walk bucket slots 0..cnt-1 of particle i in bucket order, starting from
@@ -276,7 +277,7 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
                                                         const unsigned short *__restrict__ n16,
                                                         const int *__restrict__ n32, bool staged,
                                                         int i, int cnt, int stride, Body body,
-                                                        int first0 = 0) {
+                                                        int first0 = 0, int dead = 0) {
     const int nq = (cnt + 3) >> 2;
     const int last = nq > 0 ? nq - 1 : 0;
     int4 cur[D];
@@ -289,7 +290,7 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const int qc = q + d * kStep;
-            if (qc < nq) quad_body(quad_unpack(cur[d], staged), qc * 4, cnt, body);
+            if (qc < nq) quad_body(quad_unpack(cur[d], staged), qc * 4, cnt, body, dead);
         }
 #pragma unroll
         for (int d = 0; d < D; d++) cur[d] = nxt[d];
@@ -299,9 +300,9 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
 template <int D, typename Body>
 __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
                                                    const int *__restrict__ n32, bool staged,
-                                                   int i, int cnt, int stride, Body body) {
+                                                   int i, int cnt, int stride, Body body, int dead = 0) {
     const NbrQuads<D> first = nbr_prefetch<D>(n16, n32, staged, i, stride);
-    for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body);
+    for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body, 0, dead);
 }
 
 // LDS DMA (gfx950 global_load_lds): each active lane of the wave copies
