@@ -124,8 +124,9 @@ struct StepConst {
 // Global maxima are reduced per workgroup and then with atomics into one of
 // kSlots slots (workgroup index mod kSlots), so no single address takes an
 // atomic from every workgroup; readers take the max over the slots.
-constexpr int kMaxIt = 64;
+constexpr int kMaxIt = 256;
 constexpr int kSlots = 32;
+static_assert(kMaxIt <= 256, "the loop signal carries the iteration index in 8 bits");
 struct LoopState {
     unsigned long long err[kMaxIt];              // max over the slots (host readback)
     int done[kMaxIt];

@@ -16,6 +16,7 @@ tolerance.
 CLI:
   python -m bubbles_amd.learned train  [--out PATH] [--quick]
   python -m bubbles_amd.learned eval   [--weights PATH] [--steps N]
+  python -m bubbles_amd.learned caps   [--caps 32,64,128,256] [--seed 2]
 """
 from __future__ import annotations
 
@@ -207,6 +208,38 @@ def run_variant(net, alpha, max_it, seed, steps=480, warmup=100, dims=(128, 64, 
             "psps": round(sc.n * steps / el / 1e6, 1)}
 
 
+def cap_sweep(caps, seed=2, steps=500, dims=(128, 64, 128), tol=0.01, log=print):
+    """The whole dam break from rest at each iteration cap: plain, the AUTO
+    warm start and the warm start on every substep (TEXTBOOK, tolerance
+    `tol`): mean and max iterations, max error, converged fraction,
+    throughput, and the iterations per 100-substep block."""
+    sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=seed)
+    net = load()
+    out = []
+    for cap in caps:
+        for label, warm in (("plain", abi.WARM_NONE), ("learned auto", abi.WARM_AUTO),
+                            ("learned every substep", abi.WARM_MLP)):
+            s = abi.Solver(sc)
+            if warm != abi.WARM_NONE:
+                s.attach_mlp(net)
+            s.synchronize()
+            t0 = time.perf_counter()
+            st = s.pcisph_run(sc.dt, steps, cap, tol, abi.MODE_TEXTBOOK, warm)
+            s.synchronize()
+            el = time.perf_counter() - t0
+            s.close()
+            it = np.array([x.iterations for x in st])
+            er = np.array([x.max_density_error for x in st])
+            rec = {"cap": cap, "run": label, "mean_it": round(float(it.mean()), 3), "max_it": int(it.max()),
+                   "max_err": round(float(er.max()), 5), "conv": round(float((er < tol).mean()), 4),
+                   "first_over_10pct": int(np.argmax(er > 0.1)) if (er > 0.1).any() else None,
+                   "mpsps": round(sc.n * steps / el / 1e6, 1),
+                   "it_per_100": [round(float(it[a:a + 100].mean()), 2) for a in range(0, steps, 100)]}
+            log(json.dumps(rec))
+            out.append(rec)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -234,7 +267,15 @@ def main(argv=None):
     se.add_argument("--alphas", default="0.6,0.7,0.8")
     se.add_argument("--caps", default="8,64")
     se.add_argument("--seeds", default="2000,2")
+    cs = sub.add_parser("caps", help="whole-run plain vs learned at several iteration caps")
+    cs.add_argument("--caps", default="32,64,128,256")
+    cs.add_argument("--seed", type=int, default=2)
+    cs.add_argument("--steps", type=int, default=500)
     args = ap.parse_args(argv)
+    if args.cmd == "caps":
+        cap_sweep([int(c) for c in args.caps.split(",")], seed=args.seed, steps=args.steps,
+                  log=lambda m: print(m, flush=True))
+        return 0
     if args.cmd == "select":
         net = load(args.weights)
         caps = [int(c) for c in args.caps.split(",")]
