@@ -10,8 +10,10 @@ iterations; inputs are resident in HBM before timing starts.
 Headline pair: W warm-up + K timed substeps of plain PCISPH (pressure starts
 at 0) and of PCISPH with the learned warm start in its AUTO mode (the network
 starts a substep only where the plain loop is estimated to need three or more
-corrections), both iterating to the tolerance (cap 64).  `value` is the
-learned run's particle-steps/s; the plain run and the speed-up sit beside it.
+corrections), both iterating to the tolerance (cap 128: a loop that stalls
+is stopped there; a larger cap lets a stalled plain loop inject enough
+pressure to blow the run up).  `value` is the learned run's
+particle-steps/s; the plain run and the speed-up sit beside it.
 Both runs are timed the same way: the last (up to 20) warm-up substeps are
 profiled per kernel with HIP events to pick the dominant kernel, and inside
 the timed region only one launch per substep of that kernel is bracketed by
@@ -19,7 +21,10 @@ events (on the solver's stream).
 
 Whole-run pairs (`config.whole_run`): the same scene over its full 500
 substeps from rest, plain vs learned, to tolerance and at the 8-iteration cap,
-plus the learned run with the network used on every substep; no events.
+plus the learned run with the network used on every substep, and the same
+three to-tolerance runs on a held-out seed (2000); no events.  The flow is
+chaotic, so iteration counts of two runs diverge after a few hundred
+substeps: the second seed shows how much a ratio moves with the trajectory.
 
 The CPU baseline times the FP64 oracle (a restatement of the reference CPU
 path) on a bounded sample of the same workload.
@@ -282,8 +287,10 @@ def main():
     ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--mode", choices=["textbook", "exact"], default="textbook")
-    ap.add_argument("--max-it", type=int, default=64,
-                    help="iteration cap of the headline pair (64: iterate until the tolerance is met)")
+    ap.add_argument("--max-it", type=int, default=128,
+                    help="iteration cap of the headline pair and the to-tolerance runs (at most 256)")
+    ap.add_argument("--held-out-seed", type=int, default=2000,
+                    help="seed of the second whole-run scene (-1: skip)")
     ap.add_argument("--tol", type=float, default=0.01)
     ap.add_argument("--run-steps", type=int, default=500,
                     help="length of the whole-run pairs timed from substep 0 (0: skip)")
@@ -329,6 +336,18 @@ def main():
             whole["cap8"]["max_it"] = args.cap8
         always = run_variant(args, sc, abi.WARM_MLP, args.max_it, args.run_steps, 0, False)
         whole["to_tolerance"]["learned_every_substep"] = summary(always)
+        whole["to_tolerance"]["speedup_every_substep"] = round(
+            always["value"] / whole["to_tolerance"]["plain"]["particle_steps_per_s"], 4)
+        if args.held_out_seed >= 0 and args.scene == "dam_break":
+            sc2 = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0),
+                                   seed=args.held_out_seed)
+            h = pair_summary(*run_pair(args, sc2, args.max_it, args.run_steps, 0, False))
+            ev = run_variant(args, sc2, abi.WARM_MLP, args.max_it, args.run_steps, 0, False)
+            h["learned_every_substep"] = summary(ev)
+            h["speedup_every_substep"] = round(ev["value"] / h["plain"]["particle_steps_per_s"], 4)
+            h["seed"] = args.held_out_seed
+            h["max_it"] = args.max_it
+            whole["held_out_seed"] = h
     gbs, _ = abi.stream_copy(2.0 ** 30, 20)
     roof = roofline_for(head, sc.n, gbs)
     kbar = head["mean_iterations"]
