@@ -42,7 +42,7 @@ extern "C" {
  * src/equations/pcisph_equations3.cpp:218-256).  REFERENCE_EXACT runs one
  * predict-correct iteration per substep, which is what the shipped loop does
  * because its error array is never written; TEXTBOOK iterates until the max
- * compression error / rho0 < tol or max_it is reached. */
+ * compression error / rho0 < tol or max_it (1..256) is reached. */
 #define BBX_MODE_REFERENCE_EXACT 0
 #define BBX_MODE_TEXTBOOK 1
 
