@@ -375,9 +375,10 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                                                             const int *__restrict__ tile_tab,
                                                             int *__restrict__ counters,
                                                             WarmIn wi) {
-    // staged candidate positions (16-byte LDS DMA); four slack slots let the
-    // candidate test read whole groups of four past the end of a row
-    __shared__ float4 sx[kTileCap + 4];
+    // staged candidate positions (16-byte LDS DMA); eight slack slots let
+    // the candidate test read whole groups of four, and the group after
+    // them, past the end of a row
+    __shared__ float4 sx[kTileCap + 8];
     __shared__ TileTab tt;
     const int tile = tile_of_block();
     const int i = tile * kBlock + threadIdx.x;
@@ -503,28 +504,32 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
                     unsigned lo = 0u, hi = 0u;
                     float bdist = 3.4e38f;
                     int t0 = 0;
-                    for (; t0 < len; t0 += 4) {
-                        float4 pc[4];
+                    // Two groups of four per step: the second group's loads
+                    // are in flight while the first group's arithmetic runs
+                    // (one step of a row reads up to seven candidates past
+                    // its end: slack slots when staged, clamped otherwise;
+                    // their bits are cleared below).
+                    for (; t0 < len; t0 += 8) {
+                        float4 pc[8];
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
+                        for (int q = 0; q < 8; q++) {
                             if (kStaged)
-                                pc[q] = sx[A[r] + sh[r] + t0 + q];  // slack slots cover the overrun
+                                pc[q] = sx[A[r] + sh[r] + t0 + q];
                             else
                                 pc[q] = pos[min(A[r] + t0 + q, c.n - 1)];
                         }
-                        float tq[4];
+                        float tq[8];
 #pragma unroll
-                        for (int q = 0; q < 4; q++) tq[q] = cut_offset(pi, pc[q], c.band_c);
-                        bdist = fminf(fminf(bdist, fabsf(tq[0])), fabsf(tq[1]));
-                        bdist = fminf(fminf(bdist, fabsf(tq[2])), fabsf(tq[3]));
-                        unsigned nib = __float_as_uint(tq[3]) >> 31;  // bit q: candidate t0 + q
-                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[2]), 31);
-                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[1]), 31);
-                        nib = __builtin_amdgcn_alignbit(nib, __float_as_uint(tq[0]), 31);
-                        lo = __builtin_amdgcn_alignbit(hi, lo, 4);
-                        hi = __builtin_amdgcn_alignbit(nib, hi, 4);
+                        for (int q = 0; q < 8; q++) tq[q] = cut_offset(pi, pc[q], c.band_c);
+#pragma unroll
+                        for (int q = 0; q < 8; q += 2) bdist = fminf(fminf(bdist, fabsf(tq[q])), fabsf(tq[q + 1]));
+                        unsigned byte = __float_as_uint(tq[7]) >> 31;  // bit q: candidate t0 + q
+#pragma unroll
+                        for (int q = 6; q >= 0; q--) byte = __builtin_amdgcn_alignbit(byte, __float_as_uint(tq[q]), 31);
+                        lo = __builtin_amdgcn_alignbit(hi, lo, 8);
+                        hi = __builtin_amdgcn_alignbit(byte, hi, 8);
                     }
-                    // t0 = 4 * groups; candidate t sits at bit 64 - t0 + t
+                    // t0 = 8 * steps; candidate t sits at bit 64 - t0 + t
                     unsigned long long m = ((unsigned long long)hi << 32) | lo;
                     m = t0 > 0 ? m >> (64 - t0) : 0ull;
                     if (len < 64) m &= (1ull << len) - 1ull;
