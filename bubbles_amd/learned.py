@@ -240,6 +240,41 @@ def cap_sweep(caps, seed=2, steps=500, dims=(128, 64, 128), tol=0.01, log=print)
     return out
 
 
+def select_whole(alphas, seeds, cap=128, steps=500, dims=(128, 64, 128), tol=0.01, log=print):
+    """Whole runs from rest for each (alpha, seed, policy): the shipped
+    network with its warm-start scale alpha replaced, the AUTO policy and
+    the network on every substep, and plain for reference."""
+    net = load()
+    out = []
+    for seed in seeds:
+        sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=seed)
+        variants = [("plain", None, abi.WARM_NONE)]
+        for a in alphas:
+            variants += [("auto", a, abi.WARM_AUTO), ("every", a, abi.WARM_MLP)]
+        for label, a, warm in variants:
+            s = abi.Solver(sc)
+            if a is not None:
+                p = net.get_params()
+                p[OFF_ALPHA] = a
+                v = abi.Mlp(0)
+                v.set_params(p)
+                s.attach_mlp(v)
+            s.synchronize()
+            t0 = time.perf_counter()
+            st = s.pcisph_run(sc.dt, steps, cap, tol, abi.MODE_TEXTBOOK, warm)
+            s.synchronize()
+            el = time.perf_counter() - t0
+            s.close()
+            it = np.array([x.iterations for x in st])
+            er = np.array([x.max_density_error for x in st])
+            rec = {"seed": seed, "run": label, "alpha": a, "cap": cap, "mean_it": round(float(it.mean()), 3),
+                   "max_it": int(it.max()), "max_err": round(float(er.max()), 5),
+                   "conv": round(float((er < tol).mean()), 4), "mpsps": round(sc.n * steps / el / 1e6, 1)}
+            log(json.dumps(rec))
+            out.append(rec)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -267,11 +302,19 @@ def main(argv=None):
     se.add_argument("--alphas", default="0.6,0.7,0.8")
     se.add_argument("--caps", default="8,64")
     se.add_argument("--seeds", default="2000,2")
+    sw = sub.add_parser("select-whole", help="warm-start scale alpha over whole runs from rest")
+    sw.add_argument("--alphas", default="0.7,0.75,0.8,0.85")
+    sw.add_argument("--seeds", default="2000,2001")
+    sw.add_argument("--cap", type=int, default=128)
     cs = sub.add_parser("caps", help="whole-run plain vs learned at several iteration caps")
     cs.add_argument("--caps", default="32,64,128,256")
     cs.add_argument("--seed", type=int, default=2)
     cs.add_argument("--steps", type=int, default=500)
     args = ap.parse_args(argv)
+    if args.cmd == "select-whole":
+        select_whole([float(a) for a in args.alphas.split(",")], [int(x) for x in args.seeds.split(",")],
+                     cap=args.cap, log=lambda m: print(m, flush=True))
+        return 0
     if args.cmd == "caps":
         cap_sweep([int(c) for c in args.caps.split(",")], seed=args.seed, steps=args.steps,
                   log=lambda m: print(m, flush=True))
