@@ -554,6 +554,10 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
         *s->h_signal = 0u;
         if (hipHostGetDevicePointer((void **)&s->d_signal, s->h_signal, 0) != hipSuccess)
             rc = fail(BBX_E_HIP, "device address of the loop signal");
+        // BBX_LOOP_SIGNAL=0: no published decisions; the host reads every
+        // decision back through a copy (the fallback path, for tests)
+        if (const char *e = std::getenv("BBX_LOOP_SIGNAL"))
+            if (std::atoi(e) == 0) s->d_signal = nullptr;
     }
     if (!rc) {
         s->h_loop = &s->h_step->loop;
@@ -1219,7 +1223,7 @@ struct SubstepPlan {
 static int wait_loop_decision(bbx_solver *s, int k, int *done) {
     const unsigned want = s->loop_seq & 0x7fffffu;
     auto decided = [&](unsigned w) { return (w >> 9) == want && ((w & 1u) || (int)((w >> 1) & 0xffu) >= k); };
-    for (unsigned spin = 1;; spin++) {
+    for (unsigned spin = 1; s->d_signal; spin++) {
         unsigned w = __atomic_load_n(s->h_signal, __ATOMIC_ACQUIRE);
         if (decided(w)) {
             *done = (int)(w & 1u) && (int)((w >> 1) & 0xffu) <= k;

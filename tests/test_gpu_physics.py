@@ -140,6 +140,43 @@ def test_pcisph_run_equals_repeated_substeps(bbx, warm):
         b.pcisph_run(sc.dt, -1, 16, 0.01, bbx.MODE_TEXTBOOK, w)
 
 
+def test_loop_decisions_without_the_published_signal(bbx, monkeypatch):
+    """The host learns each loop decision from the pinned word the
+    pressure-force kernel stores it in; with that word switched off
+    (BBX_LOOP_SIGNAL=0) it reads the done flags back through copies instead.
+    Both give the same substeps bit for bit, plain and warm-started, and the
+    cap's range is 1..256."""
+    from bubbles_amd import learned
+
+    sc = small_dam(seed=5)
+
+    def run():
+        out = []
+        for w in (bbx.WARM_NONE, bbx.WARM_AUTO):
+            s = bbx.Solver(sc)
+            s.attach_mlp(learned.load())
+            st = s.pcisph_run(sc.dt, 30, 256, 0.01, bbx.MODE_TEXTBOOK, w)
+            out.append(([x.iterations for x in st], [x.max_density_error for x in st],
+                        s.get("position"), s.get("velocity")))
+            s.close()
+        return out
+
+    ref = run()
+    monkeypatch.setenv("BBX_LOOP_SIGNAL", "0")
+    got = run()
+    for a, b in zip(ref, got):
+        assert a[0] == b[0] and a[1] == b[1]
+        np.testing.assert_array_equal(a[2], b[2])
+        np.testing.assert_array_equal(a[3], b[3])
+    assert max(ref[0][0]) > 1  # the loop did go on at least once
+    s = bbx.Solver(sc)
+    with pytest.raises(bbx.BbxError):
+        s.pcisph_substep(sc.dt, 257, 0.01, bbx.MODE_TEXTBOOK)
+    with pytest.raises(bbx.BbxError):
+        s.pcisph_substep(sc.dt, 0, 0.01, bbx.MODE_TEXTBOOK)
+    s.close()
+
+
 def test_pcisph_textbook_decisions_from_synchronised_states(bbx, oracle):
     """Every 4th of 48 GPU TEXTBOOK substeps, the oracle is reset to the GPU
     state and both run the next substep: the iteration count (the FP64
