@@ -256,6 +256,11 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
     const int e = valid ? b + count[cc] : 0;
     // thread 0 is valid in every launched workgroup (the grid is ceil(n / kBlock))
     if (threadIdx.x == 0) cell0 = cc;
+    // Thread t holds the key of storage slot t, so the keys of the cell
+    // members inside this workgroup's slots come from LDS; only members in
+    // neighbouring workgroups' slots are read from global memory.
+    const int blk0 = (int)(blockIdx.x * blockDim.x);
+    skey[threadIdx.x] = k;
     __syncthreads();
     const bool uniform = __syncthreads_and(!valid || cc == cell0) != 0;
     const int c0 = cell0;
@@ -276,7 +281,11 @@ __global__ void __launch_bounds__(kBlock) k_rank_move(SimConst c, const int *__r
             __syncthreads();
         }
     } else if (valid) {
-        for (int q = b; q < e; q++) rank += key[tmp_idx[q]] < k ? 1 : 0;
+        for (int q = b; q < e; q++) {
+            const int u = q - blk0;
+            const unsigned long long kq = (u >= 0 && u < kBlock) ? skey[u] : key[tmp_idx[q]];
+            rank += kq < k ? 1 : 0;
+        }
     }
     if (!valid) return;
     const int dest = b + rank;
