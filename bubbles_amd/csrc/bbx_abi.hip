@@ -1114,7 +1114,7 @@ int bbx_neighbors_get(bbx_solver *s, int *counts, int *out_ids) {
                     const int *t = &tab[(i / kBlock) * kTabIntsHost];
                     int j = -1;
                     if (t[1]) {  // staged tile: local index -> interval -> slot
-                        const int jl = quad16[4 * i + (k & 3)];
+                        const int jl = quad16[4 * i + (k & 3)] >> 4;  // LDS byte offset -> slot
                         for (int q = 0; q < t[0]; q++) {
                             const int st = t[3 + q], len = t[3 + 9 + q], off = t[3 + 18 + q];
                             if (jl >= off && jl < off + len) j = st + (jl - off);

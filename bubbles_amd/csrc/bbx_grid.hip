@@ -474,9 +474,9 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         // density sum runs afterwards over the finished bucket, one uniform
         // pass.
         auto append = [&](int j, int sh) {
-            if (kStaged) {
+            if (kStaged) {  // the entry is the candidate's LDS byte offset
                 qlo = __builtin_amdgcn_alignbit(qhi, qlo, 16);
-                qhi = __builtin_amdgcn_alignbit((unsigned)(j + sh), qhi, 16);
+                qhi = __builtin_amdgcn_alignbit((unsigned)(j + sh) << kSlotShift, qhi, 16);
             } else {
                 e0 = e1;
                 e1 = e2;
@@ -615,9 +615,9 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         // is its position bit for bit, so they add exact zeros to the
         // feature sums below, as the particle's own entry does, and only
         // the density masks them.
-        const int self = kStaged ? i + tt.rdelta[4] : i;
-        for_each_neighbour<kNbrDepth>(nbr16, nbr32, kStaged, i, cnt, c.stride, [&](int j, bool live) {
-            const float4 pj = kStaged ? sx[j] : pos[j];
+        const int self = kStaged ? (i + tt.rdelta[4]) << kSlotShift : i;
+        for_each_neighbour_dead<kNbrDepth>(nbr16, nbr32, kStaged, i, cnt, c.stride, [&](int j, bool live) {
+            const float4 pj = kStaged ? lds_at(sx, j) : pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
             const float w = poly6_w_member(c, d2);
@@ -773,7 +773,7 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
     with_staging(staged, [&](auto tag) {
         constexpr bool kS = decltype(tag)::value;
         for_each_neighbour<2>(nbr16, nbr32, kS, i, cnt, c.stride, [&](int j, bool live) {
-            const float4 pj = kS ? sx[j] : pos[j];
+            const float4 pj = kS ? lds_at(sx, j) : pos[j];
             float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float w = poly6_w(c, dx * dx + dy * dy + dz * dz);
             kahan_add(sum, comp, live ? w : 0.f);  // a dead entry adds an exact zero

@@ -163,10 +163,10 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
         auto body = [&](int j, bool live) {
             float4 a, b;
             if (kL) {
-                a = sx[j];
-                b = sv[j];
+                a = lds_at(sx, j);
+                b = lds_at(sv, j);
             } else {
-                const int g = staged ? tab_global(tt, j) : j;
+                const int g = staged ? tab_global(tt, j >> kSlotShift) : j;
                 a = sxsrc[g];
                 const float4 v = vel[g];
                 b = make_float4(v.x, v.y, v.z, fast_rcp(rho[g]));
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
             for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
                 float3 xj;
                 if (kS) {
-                    const float4 a = sx[j];
+                    const float4 a = lds_at(sx, j);
                     xj = make_float3(a.x, a.y, a.z);
                 } else {
                     const float4 g = xstar[j];
@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
             for_each_neighbour_from<kNbrDepth>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
-                const float4 a = kS ? sx[j] : xq[j];  // (x_j, q_j)
+                const float4 a = kS ? lds_at(sx, j) : xq[j];  // (x_j, q_j)
                 const float qj = a.w;
                 const float dx = a.x - xi.x, dy = a.y - xi.y, dz = a.z - xi.z;
                 const float d2 = dx * dx + dy * dy + dz * dz;
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
     __syncthreads();
     const int i = tile * kBlock + threadIdx.x;
     if (i >= c.n) return;
-    const int self = staged ? tab_local(tt, i) : i;
+    const int self = staged ? tab_local(tt, i) << kSlotShift : i;
     const float4 xi = pos[i], vi = vel[i];
     const float di = rho[i];
     const float qi = pres[i] / (di * di);
@@ -510,10 +510,10 @@ __global__ void __launch_bounds__(kBlock) k_sph_force(SimConst c, NbrTab nb,
         // unconditional and a skipped pair adds exact zeros
         float4 xj, vj;
         if (in_lds) {
-            xj = sx[j];
-            vj = sv[j];
+            xj = lds_at(sx, j);
+            vj = lds_at(sv, j);
         } else {
-            const int g = staged ? tab_global(tt, j) : j;
+            const int g = staged ? tab_global(tt, j >> kSlotShift) : j;
             xj = xq(g);
             vj = vr(g);
         }
@@ -589,10 +589,10 @@ __global__ void __launch_bounds__(kBlock) k_xsph(SimConst c, StepConst st, NbrTa
     for_each_neighbour<2>(nb.n16, nb.n32, staged, i, cnt, c.stride, [&](int j, bool live) {
         float4 xj, vj;
         if (in_lds) {
-            xj = sx[j];
-            vj = sv[j];
+            xj = lds_at(sx, j);
+            vj = lds_at(sv, j);
         } else {
-            const int g = staged ? tab_global(tt, j) : j;
+            const int g = staged ? tab_global(tt, j >> kSlotShift) : j;
             xj = pos[g];
             vj = vr(g);
         }

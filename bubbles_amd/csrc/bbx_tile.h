@@ -72,6 +72,13 @@ __device__ __forceinline__ int tab_local(const TileTab &t, int j) {
     return -1;
 }
 
+// A staged tile's bucket entries are LDS byte offsets (slot << kSlotShift,
+// slot < kTileCap, so they fit 16 bits): a gather adds no shift per entry.
+constexpr int kSlotShift = 4;  // 16-byte slots
+__device__ __forceinline__ float4 lds_at(const float4 *base, int byte_off) {
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 // storage index of LDS-local index jl (jl < t.total)
 __device__ __forceinline__ int tab_global(const TileTab &t, int jl) {
     int k = 0;
@@ -251,18 +258,29 @@ __device__ __forceinline__ NbrQuads<D> nbr_prefetch(const unsigned short *__rest
 }
 
 // The four entries of one (unpacked) quad go through body(j, live) without
-// branches: an entry at or past the bucket count gets j = dead (0 by
-// default; any valid index in the entry's index space) and live = false.
-// Straight-line code lets the four entries' neighbour loads issue together
-// instead of one load-wait per entry.  The body must make a dead entry
-// contribute nothing (or an exact zero); with dead = the particle's own
-// index, a sum over x_j - x_i gets exact zeros from dead entries unmasked.
-template <typename Body>
+// branches; live = false for an entry at or past the bucket count.  Every
+// slot of a quad the walk reaches holds a valid index: the bucket build
+// zero-fills the unused slots of a bucket's last quad, and index 0 is valid
+// in either index space.  So by default a dead entry keeps its slot's index
+// (no select per entry); with kSelect it gets `dead` instead (the particle's
+// own index: a sum over x_j - x_i then gets exact zeros from dead entries
+// unmasked).  Straight-line code lets the four entries' neighbour loads
+// issue together instead of one load-wait per entry.  The body must make a
+// dead entry contribute nothing (or an exact zero).
+template <bool kSelect, typename Body>
 __device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &body, int dead) {
-    body(k < cnt ? v.x : dead, k < cnt);
-    body(k + 1 < cnt ? v.y : dead, k + 1 < cnt);
-    body(k + 2 < cnt ? v.z : dead, k + 2 < cnt);
-    body(k + 3 < cnt ? v.w : dead, k + 3 < cnt);
+    const int rem = cnt - k;  // entries of the bucket from this quad on
+    if constexpr (kSelect) {
+        body(rem > 0 ? v.x : dead, rem > 0);
+        body(rem > 1 ? v.y : dead, rem > 1);
+        body(rem > 2 ? v.z : dead, rem > 2);
+        body(rem > 3 ? v.w : dead, rem > 3);
+    } else {
+        body(v.x, rem > 0);
+        body(v.y, rem > 1);
+        body(v.z, rem > 2);
+        body(v.w, rem > 3);
+    }
 }
 
 // Walk bucket slots 0..cnt-1 of particle i in bucket order, starting from
@@ -272,7 +290,7 @@ __device__ __forceinline__ void quad_body(const int4 &v, int k, int cnt, Body &b
 // index (staged tile) or a storage index.  The loads of the next quads are
 // unconditional: a quad past the bucket's last one reloads the last (a cache
 // hit), so no load sits in a branch that would have to wait for it.
-template <int D, int kStep = 1, typename Body>
+template <int D, int kStep = 1, bool kSelect = false, typename Body>
 __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first,
                                                         const unsigned short *__restrict__ n16,
                                                         const int *__restrict__ n32, bool staged,
@@ -290,7 +308,7 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const int qc = q + d * kStep;
-            if (qc < nq) quad_body(quad_unpack(cur[d], staged), qc * 4, cnt, body, dead);
+            if (qc < nq) quad_body<kSelect>(quad_unpack(cur[d], staged), qc * 4, cnt, body, dead);
         }
 #pragma unroll
         for (int d = 0; d < D; d++) cur[d] = nxt[d];
@@ -300,9 +318,18 @@ __device__ __forceinline__ void for_each_neighbour_from(const NbrQuads<D> &first
 template <int D, typename Body>
 __device__ __forceinline__ void for_each_neighbour(const unsigned short *__restrict__ n16,
                                                    const int *__restrict__ n32, bool staged,
-                                                   int i, int cnt, int stride, Body body, int dead = 0) {
+                                                   int i, int cnt, int stride, Body body) {
     const NbrQuads<D> first = nbr_prefetch<D>(n16, n32, staged, i, stride);
-    for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body, 0, dead);
+    for_each_neighbour_from<D>(first, n16, n32, staged, i, cnt, stride, body);
+}
+
+// The same with dead entries reading index `dead`.
+template <int D, typename Body>
+__device__ __forceinline__ void for_each_neighbour_dead(const unsigned short *__restrict__ n16,
+                                                        const int *__restrict__ n32, bool staged,
+                                                        int i, int cnt, int stride, Body body, int dead) {
+    const NbrQuads<D> first = nbr_prefetch<D>(n16, n32, staged, i, stride);
+    for_each_neighbour_from<D, 1, true>(first, n16, n32, staged, i, cnt, stride, body, 0, dead);
 }
 
 // LDS DMA (gfx950 global_load_lds): each active lane of the wave copies
