@@ -1303,7 +1303,8 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
     const int cap = exact ? 1 : max_it;
     int launches = 0;
     int goal = s->last_iterations > 1 ? s->last_iterations - 1 : 1;
-    for (;;) {
+    // an empty set has no loop (no kernel would publish a decision)
+    for (; s->n > 0;) {
         goal = goal < cap ? goal : cap;
         for (; launches < goal; launches++) {
             st.first = (launches == 0 && warm != BBX_WARM_MLP) ? 1 : 0;
