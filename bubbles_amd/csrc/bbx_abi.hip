@@ -1301,9 +1301,11 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
     // longer).
     const int exact = mode == BBX_MODE_REFERENCE_EXACT ? 1 : 0;
     const int cap = exact ? 1 : max_it;
-    int launches = 0;
+    // An empty set converges in its first iteration with zero error, as the
+    // reference's loop does over no particles; no kernel runs, so none would
+    // publish a decision to wait for.
+    int launches = s->n > 0 ? 0 : 1;
     int goal = s->last_iterations > 1 ? s->last_iterations - 1 : 1;
-    // an empty set has no loop (no kernel would publish a decision)
     for (; s->n > 0;) {
         goal = goal < cap ? goal : cap;
         for (; launches < goal; launches++) {
