@@ -55,9 +55,11 @@ def training_scenes(seeds=TRAIN_SEEDS, scale=1.0):
 
 
 def harvest(scene_list, steps=400, every=25, start=25, per_snapshot=8192, max_it=8, tol=0.01,
-            seed=0, log=print):
+            seed=0, log=print, drop_unconverged=False):
     """Run plain TEXTBOOK PCISPH on each scene; every `every` substeps record a
-    random subset of (features, converged p / P)."""
+    random subset of (features, converged p / P).  drop_unconverged: leave
+    out snapshots whose loop stopped at the cap above the tolerance (their
+    pressures are what a stalled loop built, not a converged field)."""
     rng = np.random.default_rng(seed)
     xs, ts = [], []
     for si, sc in enumerate(scene_list):
@@ -68,8 +70,9 @@ def harvest(scene_list, steps=400, every=25, start=25, per_snapshot=8192, max_it
                 f, t, st = s.harvest(sc.dt, max_it, tol)
                 k = min(per_snapshot, sc.n)
                 sel = rng.choice(sc.n, size=k, replace=False)
-                xs.append(f[sel])
-                ts.append(t[sel])
+                if not (drop_unconverged and st.max_density_error >= tol):
+                    xs.append(f[sel])
+                    ts.append(t[sel])
             else:
                 st = s.pcisph_substep(sc.dt, max_it, tol, abi.MODE_TEXTBOOK)
             iters.append(st.iterations)
@@ -240,15 +243,16 @@ def cap_sweep(caps, seed=2, steps=500, dims=(128, 64, 128), tol=0.01, log=print)
     return out
 
 
-def select_whole(alphas, seeds, cap=128, steps=500, dims=(128, 64, 128), tol=0.01, log=print):
+def select_whole(alphas, seeds, cap=128, steps=500, dims=(128, 64, 128), tol=0.01, log=print,
+                 weights=DEFAULT_WEIGHTS, plain=True):
     """Whole runs from rest for each (alpha, seed, policy): the shipped
     network with its warm-start scale alpha replaced, the AUTO policy and
     the network on every substep, and plain for reference."""
-    net = load()
+    net = load(weights)
     out = []
     for seed in seeds:
         sc = scenes.dam_break(*dims, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=seed)
-        variants = [("plain", None, abi.WARM_NONE)]
+        variants = [("plain", None, abi.WARM_NONE)] if plain else []
         for a in alphas:
             variants += [("auto", a, abi.WARM_AUTO), ("every", a, abi.WARM_MLP)]
         for label, a, warm in variants:
@@ -289,6 +293,8 @@ def main(argv=None):
     tr.add_argument("--alpha", type=float, default=0.7)
     tr.add_argument("--expectile", type=float, default=0.5,
                     help="training loss expectile (0.5 = mean squared error)")
+    tr.add_argument("--drop-unconverged", action="store_true",
+                    help="leave out snapshots whose loop stopped at the cap above the tolerance")
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--steps", type=int, default=480)
@@ -306,6 +312,8 @@ def main(argv=None):
     sw.add_argument("--alphas", default="0.7,0.75,0.8,0.85")
     sw.add_argument("--seeds", default="2000,2001")
     sw.add_argument("--cap", type=int, default=128)
+    sw.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    sw.add_argument("--no-plain", action="store_true")
     cs = sub.add_parser("caps", help="whole-run plain vs learned at several iteration caps")
     cs.add_argument("--caps", default="32,64,128,256")
     cs.add_argument("--seed", type=int, default=2)
@@ -313,7 +321,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "select-whole":
         select_whole([float(a) for a in args.alphas.split(",")], [int(x) for x in args.seeds.split(",")],
-                     cap=args.cap, log=lambda m: print(m, flush=True))
+                     cap=args.cap, log=lambda m: print(m, flush=True), weights=args.weights,
+                     plain=not args.no_plain)
         return 0
     if args.cmd == "caps":
         cap_sweep([int(c) for c in args.caps.split(",")], seed=args.seed, steps=args.steps,
@@ -337,7 +346,8 @@ def main(argv=None):
         seeds = TRAIN_SEEDS[:4] if args.quick else TRAIN_SEEDS
         scale = 0.5 if args.quick else 1.0
         x, t = harvest(training_scenes(seeds, scale), steps=200 if args.quick else 400,
-                       per_snapshot=2048 if args.quick else 8192, max_it=args.harvest_max_it)
+                       per_snapshot=2048 if args.quick else 8192, max_it=args.harvest_max_it,
+                       drop_unconverged=args.drop_unconverged)
         t1 = time.perf_counter()
         print(f"harvested {x.shape[0]} samples in {t1 - t0:.1f} s; target mean {t.mean():.4f} std {t.std():.4f}")
         net, loss = fit(x, t, steps=200 if args.quick else args.steps, batch=args.batch, lr=args.lr,
@@ -347,7 +357,7 @@ def main(argv=None):
         save(net, args.out, {"samples": int(x.shape[0]), "steps": args.steps, "batch": args.batch,
                              "lr": args.lr, "seeds": seeds, "final_loss": loss,
                              "harvest_max_it": args.harvest_max_it, "alpha": args.alpha,
-                             "expectile": args.expectile})
+                             "expectile": args.expectile, "drop_unconverged": args.drop_unconverged})
         print("saved", args.out)
     else:
         net = load(args.weights)
