@@ -1290,13 +1290,14 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
     // reads that iteration's FP64 error on the device and either integrates
     // or predicts iteration k + 1, and publishes that decision to the host
     // as soon as it has taken it (the first workgroup, at kernel start).
-    // The host queues one iteration at a time: it waits for the decision of
-    // the last queued one and, if the loop goes on, queues the next while
-    // that pressure-force kernel is still running, so the device does not
-    // wait for the host and no launch is queued that is not needed.  Once
-    // the loop has integrated, the rest of the substep and the next
-    // substep's grid work are queued behind the integrating kernel in the
-    // same way.  REFERENCE_EXACT stops after the first iteration (its error
+    // The host queues one iteration fewer than recent substeps needed, then
+    // one at a time: it waits for the decision of the last queued one and,
+    // if the loop goes on, queues the next while that pressure-force kernel
+    // is still running, so the device does not wait for the host and a
+    // launch that exits at entry is queued only when a substep needs two or
+    // more iterations fewer than recent ones.  Once the loop has integrated,
+    // the rest of the substep and the next substep's grid work are queued
+    // behind the integrating kernel in the same way.  REFERENCE_EXACT stops after the first iteration (its error
     // is never recorded, so it never converges earlier and never runs
     // longer).
     const int exact = mode == BBX_MODE_REFERENCE_EXACT ? 1 : 0;
@@ -1305,7 +1306,12 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
     // reference's loop does over no particles; no kernel runs, so none would
     // publish a decision to wait for.
     int launches = s->n > 0 ? 0 : 1;
-    int goal = s->last_iterations > 1 ? s->last_iterations - 1 : 1;
+    // queued at once: one fewer than the smaller of the last two substeps'
+    // counts (a stalled substep at the cap is not followed by a cap's worth
+    // of launches that exit at entry)
+    const int recent = s->prev_iterations > 0 && s->prev_iterations < s->last_iterations ? s->prev_iterations
+                                                                                          : s->last_iterations;
+    int goal = recent > 1 ? recent - 1 : 1;
     for (; s->n > 0;) {
         goal = goal < cap ? goal : cap;
         for (; launches < goal; launches++) {
@@ -1337,6 +1343,7 @@ static int substep_finish(bbx_solver *s, int slot, const SubstepPlan &plan, bbx_
     BBX_HIP_CHECK(hipEventSynchronize(s->ev_read[slot]));
     bbx_stats local{};
     parse_stats(s, &local, plan.launches);
+    s->prev_iterations = s->last_iterations;
     s->last_iterations = local.iterations;
     s->last_warm = plan.warm == BBX_WARM_MLP ? 1 : 0;
     s->warm_ones = s->last_warm && local.iterations == 1 ? s->warm_ones + 1 : 0;

@@ -211,6 +211,7 @@ struct bbx_solver {
     double mass = 1e-3, kr = 0, radius = 1e-3, delta_denom = 0, mass_over_rho2 = 0;
     double delta64 = 0;   // PCISPH delta of the current substep (FP64)
     int last_iterations = 0;  // iterations of the previous PCISPH substep
+    int prev_iterations = 0;  // and of the one before it
     int last_warm = 0;        // whether it started from the warm-start network
     int warm_ones = 0;        // warm substeps in a row that converged in one correction
     int warm_active = 0;      // a substep has needed BBX_WARM_AUTO_ACTIVE or more corrections
