@@ -27,7 +27,8 @@ chaotic, so iteration counts of two runs diverge after a few hundred
 substeps: the second seed shows how much a ratio moves with the trajectory.
 
 The CPU baseline times the FP64 oracle (a restatement of the reference CPU
-path) on a bounded sample of the same workload.
+path) on a bounded sample of the same workload, and on the 16,384-particle
+SPH dam break (C1); `config.c1_sph_dam_break_gpu` times C1 on the GPU.
 """
 from __future__ import annotations
 
@@ -267,6 +268,27 @@ def run_cpu_baseline(sc, args):
                                                 "seconds": round(c1_t_one, 2)}}}
 
 
+def run_c1_gpu():
+    """The 16,384-particle WCSPH dam break (configuration C1) on the GPU:
+    200 Advance(1e-3) frames, each split into the reference's substeps
+    (SphSolver3::Advance, src/solvers/sph_solver3.cpp:69-93), after 5 untimed
+    frames; particle-steps/s over the substeps taken."""
+    from bubbles_amd import abi, scenes
+
+    sc = scenes.sph_dam_break()
+    s = abi.Solver(sc, kind=abi.SPH)
+    for _ in range(5):
+        s.sph_advance(1e-3)
+    s.synchronize()
+    t0 = time.perf_counter()
+    steps = sum(s.sph_advance(1e-3) for _ in range(200))
+    s.synchronize()
+    t = time.perf_counter() - t0
+    s.close()
+    return {"n": sc.n, "frames": 200, "substeps": steps, "particle_steps_per_s": round(sc.n * steps / t, 1),
+            "ms_per_substep": round(t / steps * 1e3, 4)}
+
+
 def summary(res):
     return {"particle_steps_per_s": round(res["value"], 1),
             "ms_per_step": round(res["elapsed"] / max(1, res.get("steps", 1)) * 1e3, 4),
@@ -348,6 +370,7 @@ def main():
             h["seed"] = args.held_out_seed
             h["max_it"] = args.max_it
             whole["held_out_seed"] = h
+    c1_gpu = run_c1_gpu() if args.scene == "dam_break" else None
     gbs, _ = abi.stream_copy(2.0 ** 30, 20)
     roof = roofline_for(head, sc.n, gbs)
     kbar = head["mean_iterations"]
@@ -363,6 +386,8 @@ def main():
            "speedup_learned_vs_plain": round(head["value"] / plain["value"], 4)}
     if whole:
         cfg["whole_run"] = {"substeps": args.run_steps, **whole}
+    if c1_gpu:
+        cfg["c1_sph_dam_break_gpu"] = c1_gpu
     out = {
         "metric": METRIC,
         "value": round(head["value"], 1),

@@ -1154,9 +1154,13 @@ int bbx_pcisph_delta(bbx_solver *s, double dt, double *delta) {
 }
 
 static int max_force_host(bbx_solver *s, double *mf) {
-    BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
-                                 hipMemcpyDeviceToHost, s->stream));
-    BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    // a substep's statistics readback already brought the slots (Advance
+    // asks right after one); otherwise read them now
+    if (!s->maxf_cached) {
+        BBX_HIP_CHECK(hipMemcpyAsync(s->h_max_force, s->max_force, sizeof(unsigned) * kSlots,
+                                     hipMemcpyDeviceToHost, s->stream));
+        BBX_HIP_CHECK(hipStreamSynchronize(s->stream));
+    }
     unsigned m = 0;  // non-negative floats order like their bit patterns
     for (int k = 0; k < kSlots; k++) m = s->h_max_force[k] > m ? s->h_max_force[k] : m;
     float f;
@@ -1205,6 +1209,7 @@ static int check_substep(bbx_solver *s, double dt, int max_it, int warm) {
 // loop's launch count): bucket build + density, non-pressure forces, the
 // predict-correct loop, XSPH, and the statistics copied to pinned memory.
 static int substep_front(bbx_solver *s, int slot) {
+    s->maxf_cached = false;
     BBX_HIP_CHECK(hipEventRecord(s->ev0[slot], s->stream));
     BBX_CHECK(launch_fill_int(s, (int *)s->step, 0, sizeof(StepBlock) / sizeof(int)));
     return distribute_grid(s, false, -1);
@@ -1343,6 +1348,7 @@ static int substep_finish(bbx_solver *s, int slot, const SubstepPlan &plan, bbx_
     BBX_HIP_CHECK(hipEventSynchronize(s->ev_read[slot]));
     bbx_stats local{};
     parse_stats(s, &local, plan.launches);
+    s->maxf_cached = true;
     s->prev_iterations = s->last_iterations;
     s->last_iterations = local.iterations;
     s->last_warm = plan.warm == BBX_WARM_MLP ? 1 : 0;
@@ -1407,6 +1413,7 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     BBX_CHECK(check_setup(s));
     if (s->kind != BBX_SPH) return fail(BBX_E_INVALID, "not an SPH solver");
     if (!(dt > 0)) return fail(BBX_E_INVALID, "dt must be > 0");
+    s->maxf_cached = false;
     BBX_HIP_CHECK(hipEventRecord(s->ev0[0], s->stream));
     BBX_CHECK(launch_fill_int(s, (int *)s->step, 0, sizeof(StepBlock) / sizeof(int)));
     BBX_CHECK(distribute(s, false, true, true));
@@ -1420,6 +1427,7 @@ int bbx_sph_substep(bbx_solver *s, double dt, bbx_stats *stats) {
     BBX_HIP_CHECK(hipEventRecord(s->ev1[0], s->stream));
     bbx_stats local{};
     BBX_CHECK(read_stats(s, &local));
+    s->maxf_cached = true;
     local.max_density_error = 0;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0[0], s->ev1[0]);

@@ -210,6 +210,9 @@ struct bbx_solver {
     int stride = 0;
     double mass = 1e-3, kr = 0, radius = 1e-3, delta_denom = 0, mass_over_rho2 = 0;
     double delta64 = 0;   // PCISPH delta of the current substep (FP64)
+    // h_max_force holds the max-force slots of the last completed substep,
+    // read back with its statistics, and no substep has been queued since
+    bool maxf_cached = false;
     int last_iterations = 0;  // iterations of the previous PCISPH substep
     int prev_iterations = 0;  // and of the one before it
     int last_warm = 0;        // whether it started from the warm-start network
