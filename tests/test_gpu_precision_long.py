@@ -34,7 +34,7 @@ After 1000 substeps (max / mean):
   262,144     1.7e-1 / 1.1e-4   6.3e-1 / 5.4e-4   5.3e-1 / 8.2e-4 h
   1,048,576   2.2e-1 / 7.5e-5   1.4e0  / 3.4e-4   1.0e0  / 5.9e-4 h
   4,194,304   9.0e-2 / 2.6e-5   5.0e-1 / 1.0e-4   4.5e-1 / 2.1e-4 h
-  1M learned  1.7e0  / 1.3e-4   1.2e0  / 2.6e-4   9.8e-1 / 1.0e-3 h
+  1M learned  3.4e-1 / 1.2e-4   4.9e-1 / 1.1e-4   1.3e0  / 7.5e-4 h
 
 4,194,304 particles, 20 substeps: density 3.4e-4 / 9.7e-6, f_p 7.4e-3 /
 5.9e-5, position 3.0e-4 / 9.7e-6 h (the 1M case of the short study reads
@@ -44,7 +44,7 @@ The means stay at round-off while the maxima grow about 10x per 300
 substeps: the splash is chaotic, and particles whose trajectories have
 separated by a fraction of h differ in everything.  Bounds are about 3x the
 measured values.  The learned warm start, fed the same warm pressures, stays
-as close (200 substeps: density 4.9e-3 max, position 6.0e-3 h max).
+as close (200 substeps: density 9.0e-3 max, position 5.1e-3 h max).
 TEXTBOOK over 200 substeps: 6.46 iterations (FP32) vs 6.47 (FP64), the
 same count in 91% of the substeps, never more than one apart.
 """
@@ -159,7 +159,7 @@ def test_learned_exact_against_fp64(bbx, oracle, dims, steps):
     else:
         # the maximum is a splash particle: alone in one run and among
         # neighbours in the other, its density differs by a multiple of the
-        # oracle's (1.7 to 3.3 measured); the means are what the horizon
+        # oracle's (0.34 to 3.3 measured); the means are what the horizon
         # bounds
         assert last["density_rel_max"] <= 10.0 and last["density_rel_mean"] <= 4e-4, last
         assert last["pos_over_h_max"] <= 4.5 and last["pos_over_h_mean"] <= 3e-3, last
