@@ -46,8 +46,10 @@ static hipEvent_t pool_get(std::vector<hipEvent_t> &pool) {
         pool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence (an L2 write-back and invalidate)
+    // when the event is recorded
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -407,6 +409,10 @@ static void free_all(bbx_solver *s) {
         if (s->ev_read[k]) (void)hipEventDestroy(s->ev_read[k]);
         s->ev0[k] = s->ev1[k] = s->ev_read[k] = nullptr;
     }
+    for (int k = 0; k < bbx_solver::kMarks; k++) {
+        if (s->mark[k]) (void)hipEventDestroy(s->mark[k]);
+        s->mark[k] = nullptr;
+    }
     s->prof.destroy();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
@@ -548,10 +554,11 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc && hipHostMalloc((void **)&s->h_flag, sizeof(int)) != hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
-    if (!rc && hipHostMalloc((void **)&s->h_signal, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->h_signal, sizeof(LoopSignal), hipHostMallocMapped | hipHostMallocCoherent) !=
+                   hipSuccess)
         rc = fail(BBX_E_NOMEM, "pinned alloc");
     if (!rc) {
-        *s->h_signal = 0u;
+        std::memset(s->h_signal, 0, sizeof(LoopSignal));
         if (hipHostGetDevicePointer((void **)&s->d_signal, s->h_signal, 0) != hipSuccess)
             rc = fail(BBX_E_HIP, "device address of the loop signal");
         // BBX_LOOP_SIGNAL=0: no published decisions; the host reads every
@@ -564,9 +571,18 @@ int bbx_solver_create(int kind, const bbx_params *params, const double *pos, con
         s->h_max_force = s->h_step->max_force;
         s->h_counters = s->h_step->counters;
     }
+    // ev0 / ev1 bracket a substep's device time and synchronise nothing: they
+    // are recorded without the system-scope fence (an L2 write-back and
+    // invalidate, about 5 us at 1M particles, twice per substep).  ev_read
+    // follows the statistics copy to pinned memory, which the host reads
+    // once it has completed, so it keeps the fence.
     for (int k = 0; k < 2 && !rc; k++)
-        if (hipEventCreate(&s->ev0[k]) != hipSuccess || hipEventCreate(&s->ev1[k]) != hipSuccess ||
+        if (hipEventCreateWithFlags(&s->ev0[k], hipEventDisableSystemFence) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev1[k], hipEventDisableSystemFence) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_read[k], hipEventDisableTiming) != hipSuccess)
+            rc = fail(BBX_E_HIP, "event create");
+    for (int k = 0; k < bbx_solver::kMarks && !rc; k++)
+        if (hipEventCreateWithFlags(&s->mark[k], hipEventDisableSystemFence) != hipSuccess)
             rc = fail(BBX_E_HIP, "event create");
     if (rc) {
         free_all(s);
@@ -1208,16 +1224,18 @@ static int check_substep(bbx_solver *s, double dt, int max_it, int warm) {
 // build).  The back needs that outcome (the warm-start decision and the
 // loop's launch count): bucket build + density, non-pressure forces, the
 // predict-correct loop, XSPH, and the statistics copied to pinned memory.
-static int substep_front(bbx_solver *s, int slot) {
+static int substep_front(bbx_solver *s, hipEvent_t start) {
     s->maxf_cached = false;
-    BBX_HIP_CHECK(hipEventRecord(s->ev0[slot], s->stream));
-    BBX_CHECK(launch_fill_int(s, (int *)s->step, 0, sizeof(StepBlock) / sizeof(int)));
-    return distribute_grid(s, false, -1);
+    if (start) BBX_HIP_CHECK(hipEventRecord(start, s->stream));
+    // the step block is cleared by the launch that clears the cell counts
+    return distribute_grid(s, false, -1, (int *)s->step, sizeof(StepBlock) / sizeof(int));
 }
 
 struct SubstepPlan {
     int warm = BBX_WARM_NONE;  // the start actually used (AUTO resolved)
     int launches = 0;          // loop launches queued
+    bool by_copy = true;       // statistics through the copy to pinned memory
+    int slot = 0;              // the copy's event
 };
 
 // Wait for the pressure-force kernel of iteration k (queued, of the current
@@ -1229,7 +1247,7 @@ static int wait_loop_decision(bbx_solver *s, int k, int *done) {
     const unsigned want = s->loop_seq & 0x7fffffu;
     auto decided = [&](unsigned w) { return (w >> 9) == want && ((w & 1u) || (int)((w >> 1) & 0xffu) >= k); };
     for (unsigned spin = 1; s->d_signal; spin++) {
-        unsigned w = __atomic_load_n(s->h_signal, __ATOMIC_ACQUIRE);
+        unsigned w = __atomic_load_n(&s->h_signal->word, __ATOMIC_ACQUIRE);
         if (decided(w)) {
             *done = (int)(w & 1u) && (int)((w >> 1) & 0xffu) <= k;
             return BBX_OK;
@@ -1238,7 +1256,7 @@ static int wait_loop_decision(bbx_solver *s, int k, int *done) {
             const hipError_t q = hipStreamQuery(s->stream);
             if (q == hipErrorNotReady) continue;
             BBX_HIP_CHECK(q);
-            w = __atomic_load_n(s->h_signal, __ATOMIC_ACQUIRE);
+            w = __atomic_load_n(&s->h_signal->word, __ATOMIC_ACQUIRE);
             if (decided(w)) {
                 *done = (int)(w & 1u) && (int)((w >> 1) & 0xffu) <= k;
                 return BBX_OK;
@@ -1252,8 +1270,10 @@ static int wait_loop_decision(bbx_solver *s, int k, int *done) {
     return BBX_OK;
 }
 
-static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double tol, int mode, int warm,
-                        SubstepPlan *plan) {
+// end: recorded behind the substep's work (nullptr: none); slot picks the
+// statistics copy's event
+static int substep_back(bbx_solver *s, int slot, hipEvent_t end, double dt, int max_it, double tol, int mode,
+                        int warm, bool copy_stats, SubstepPlan *plan) {
     s->loop_seq++;  // numbers this substep's loop decisions
     if (warm == BBX_WARM_AUTO) {
         // The warm start costs about two thirds of a correction, so it only
@@ -1332,62 +1352,155 @@ static int substep_back(bbx_solver *s, int slot, double dt, int max_it, double t
     }
     BBX_CHECK(swap_after_integrate(s));
     BBX_CHECK(run_xsph(s, st));
-    BBX_HIP_CHECK(hipEventRecord(s->ev1[slot], s->stream));
-    BBX_CHECK(queue_stats(s));
-    BBX_HIP_CHECK(hipEventRecord(s->ev_read[slot], s->stream));
+    if (end) BBX_HIP_CHECK(hipEventRecord(end, s->stream));
+    // The statistics come with the integrating decision (LoopSignal), so no
+    // copy is queued: a device-to-host copy and the system-scope fence of the
+    // event behind it leave the device idle for about 20 us per substep.  The
+    // copy remains where the max |f| slots are wanted too (the time-step rule
+    // of Advance), without published decisions, and for an empty set.
+    plan->by_copy = copy_stats || !s->d_signal || s->n == 0;
+    plan->slot = slot;
+    if (plan->by_copy) {
+        BBX_CHECK(queue_stats(s));
+        BBX_HIP_CHECK(hipEventRecord(s->ev_read[slot], s->stream));
+    }
     plan->warm = warm;
     plan->launches = launches;
     return BBX_OK;
 }
 
-// Wait for the back's statistics and update what the next substep's
-// decisions read.  prof_upto: profiler records queued before the next
-// substep's front (those after it are still running).
-static int substep_finish(bbx_solver *s, int slot, const SubstepPlan &plan, bbx_stats *stats,
-                          size_t prof_upto) {
-    BBX_HIP_CHECK(hipEventSynchronize(s->ev_read[slot]));
+// The statistics the integrating decision of the current substep published
+// (LoopSignal).  Should the stream drain without them all carrying the
+// substep's number, that is an error: a later substep's work may already have
+// cleared the device-side counters a copy would read.
+static int published_stats(bbx_solver *s, bbx_stats *st) {
+    const unsigned want = s->loop_seq & 0x7fffffu;
+    unsigned long long v[kPubWords];
+    unsigned w = 0u;
+    auto current = [&]() {
+        w = __atomic_load_n(&s->h_signal->word, __ATOMIC_ACQUIRE);
+        if ((w >> 9) != want || !(w & 1u)) return false;
+        for (int q = 0; q < kPubWords; q++) {
+            v[q] = __atomic_load_n(&s->h_signal->stats[q], __ATOMIC_ACQUIRE);
+            if ((unsigned)(v[q] >> 32) != want) return false;
+        }
+        return true;
+    };
+    for (unsigned spin = 1;; spin++) {
+        if (current()) break;
+        if ((spin & 4095u) == 0u) {
+            const hipError_t q = hipStreamQuery(s->stream);
+            if (q == hipErrorNotReady) continue;
+            BBX_HIP_CHECK(q);
+            if (current()) break;
+            return fail(BBX_E_HIP, "the loop's statistics were not published");
+        }
+    }
+    const unsigned long long bits = (v[PUB_ERR_LO] & 0xffffffffull) | ((v[PUB_ERR_HI] & 0xffffffffull) << 32);
+    double e;
+    std::memcpy(&e, &bits, sizeof(double));
+    st->iterations = (int)((w >> 1) & 0xffu) + 1;
+    st->max_density_error = e / s->prm.target_density;
+    st->bucket_overflows = (int)(unsigned)v[PUB_OVERFLOW];
+    st->lost_particles = (int)(unsigned)v[PUB_LOST];
+    st->full_redistribution = (int)(unsigned)v[PUB_FULL];
+    return BBX_OK;
+}
+
+// The back's statistics (everything but gpu_ms), and what the next
+// substep's decisions read updated from them.  Published statistics need no
+// wait for the substep's end; copied ones wait for the copy.
+static int substep_stats(bbx_solver *s, const SubstepPlan &plan, bbx_stats *out) {
     bbx_stats local{};
-    parse_stats(s, &local, plan.launches);
-    s->maxf_cached = true;
+    if (!plan.by_copy) {
+        BBX_CHECK(published_stats(s, &local));
+        s->maxf_cached = false;
+    } else {
+        BBX_HIP_CHECK(hipEventSynchronize(s->ev_read[plan.slot]));
+        parse_stats(s, &local, plan.launches);
+        s->maxf_cached = true;
+    }
     s->prev_iterations = s->last_iterations;
     s->last_iterations = local.iterations;
     s->last_warm = plan.warm == BBX_WARM_MLP ? 1 : 0;
     s->warm_ones = s->last_warm && local.iterations == 1 ? s->warm_ones + 1 : 0;
     if (local.iterations >= BBX_WARM_AUTO_ACTIVE) s->warm_active = 1;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, s->ev0[slot], s->ev1[slot]);
-    local.gpu_ms = ms;
     local.warm_started = s->last_warm;
+    *out = local;
+    return BBX_OK;
+}
+
+// device time between two recorded events, once the later one has completed
+static int elapsed_ms(hipEvent_t a, hipEvent_t b, double *ms) {
+    BBX_HIP_CHECK(hipEventSynchronize(b));
+    float t = 0.f;
+    BBX_HIP_CHECK(hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    return BBX_OK;
+}
+
+// copy_stats: also bring the max |f| slots to the host (for the time-step
+// rule of the next Advance substep)
+static int pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
+                          bbx_stats *stats, bool copy_stats) {
+    BBX_CHECK(check_substep(s, dt, max_it, warm));
+    SubstepPlan plan;
+    BBX_CHECK(substep_front(s, s->ev0[0]));
+    BBX_CHECK(substep_back(s, 0, s->ev1[0], dt, max_it, tol, mode, warm, copy_stats, &plan));
+    bbx_stats local{};
+    BBX_CHECK(substep_stats(s, plan, &local));
+    BBX_CHECK(elapsed_ms(s->ev0[0], s->ev1[0], &local.gpu_ms));
     if (stats) *stats = local;
-    if (s->prof.enabled) s->prof.resolve(local.iterations, prof_upto);
+    if (s->prof.enabled) s->prof.resolve(local.iterations, s->prof.pending.size());
     return BBX_OK;
 }
 
 int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
                        bbx_stats *stats) {
-    BBX_CHECK(check_substep(s, dt, max_it, warm));
-    SubstepPlan plan;
-    BBX_CHECK(substep_front(s, 0));
-    BBX_CHECK(substep_back(s, 0, dt, max_it, tol, mode, warm, &plan));
-    return substep_finish(s, 0, plan, stats, s->prof.pending.size());
+    return pcisph_substep(s, dt, max_it, tol, mode, warm, stats, false);
 }
 
 int bbx_pcisph_run(bbx_solver *s, double dt, int count, int max_it, double tol, int mode, int warm,
                    bbx_stats *stats) {
     BBX_CHECK(check_substep(s, dt, max_it, warm));
     if (count < 0) return fail(BBX_E_INVALID, "count must be >= 0");
-    // Substep k + 1's front is queued before the host waits for substep k's
-    // statistics, so the device redistributes the grid while the host reads
-    // them; the results are those of `count` bbx_pcisph_substep calls.
-    if (count > 0) BBX_CHECK(substep_front(s, 0));
-    for (int k = 0; k < count; k++) {
-        const int slot = k & 1;
-        SubstepPlan plan;
-        BBX_CHECK(substep_back(s, slot, dt, max_it, tol, mode, warm, &plan));
-        const size_t upto = s->prof.pending.size();
-        if (k + 1 < count) BBX_CHECK(substep_front(s, slot ^ 1));
-        BBX_CHECK(substep_finish(s, slot, plan, stats ? stats + k : nullptr, upto));
+    // Substep k + 1's front is queued as soon as substep k's loop has
+    // integrated, and its back as soon as the host has substep k's
+    // statistics (published with that decision), so the device goes from one
+    // substep to the next without waiting for the host.  The boundaries
+    // between substeps are marked once each in a ring of events (the end of
+    // substep k is the start of k + 1); substep j's device time is read when
+    // its ring slot comes round again, or at the end.  The results are those
+    // of `count` bbx_pcisph_substep calls.
+    constexpr int M = bbx_solver::kMarks;
+    int timed = 0;  // substeps before this one have their gpu_ms
+    auto time_upto = [&](int limit) {  // gpu_ms of substeps [timed, limit)
+        for (; timed < limit; timed++)
+            BBX_CHECK(elapsed_ms(s->mark[timed % M], s->mark[(timed + 1) % M], &stats[timed].gpu_ms));
+        return BBX_OK;
+    };
+    auto mark = [&](int m) {  // the boundary before substep m (m = count: the end)
+        if (!stats) return BBX_OK;
+        if (m >= M) BBX_CHECK(time_upto(m - M + 1));  // the slot's last user has ended
+        BBX_HIP_CHECK(hipEventRecord(s->mark[m % M], s->stream));
+        return BBX_OK;
+    };
+    if (count > 0) {
+        BBX_CHECK(mark(0));
+        BBX_CHECK(substep_front(s, nullptr));
     }
+    for (int k = 0; k < count; k++) {
+        SubstepPlan plan;
+        BBX_CHECK(substep_back(s, k & 1, nullptr, dt, max_it, tol, mode, warm, false, &plan));
+        BBX_CHECK(mark(k + 1));
+        const size_t upto = s->prof.pending.size();
+        if (k + 1 < count) BBX_CHECK(substep_front(s, nullptr));
+        bbx_stats local{};
+        BBX_CHECK(substep_stats(s, plan, &local));
+        if (stats) stats[k] = local;
+        if (s->prof.enabled) s->prof.resolve(local.iterations, upto);
+    }
+    if (stats && count > 0) BBX_CHECK(time_upto(count));
     return BBX_OK;
 }
 
@@ -1401,7 +1514,7 @@ int bbx_pcisph_advance(bbx_solver *s, double interval, int max_it, double tol, i
         BBX_CHECK(time_steps(s, remaining, 5.0, &n));
         if (n == 0) return fail(BBX_E_INVALID, "time-step rule returned 0");
         double dt = remaining / (double)n;
-        BBX_CHECK(bbx_pcisph_substep(s, dt, max_it, tol, mode, warm, nullptr));
+        BBX_CHECK(pcisph_substep(s, dt, max_it, tol, mode, warm, nullptr, true));
         remaining -= dt;
         steps++;
     }

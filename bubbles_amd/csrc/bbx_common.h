@@ -133,6 +133,20 @@ struct LoopState {
     unsigned long long slot[kMaxIt][kSlots];     // per-slot FP64 bits of max |err|
 };
 
+// The pinned block the predict-correct loop publishes into (mapped: the
+// kernels store into it by its device address).  `word` takes every
+// decision, (seq << 9) | (k << 1) | integrate, seq numbering the substep.
+// The integrating decision first stores the substep's statistics into
+// `stats`, each word (seq << 32) | value: a word of another substep carries
+// another seq, so the host needs no ordering between the stores to tell
+// that all of them are current.
+enum { PUB_ERR_LO = 0, PUB_ERR_HI, PUB_OVERFLOW, PUB_LOST, PUB_FULL, kPubWords };
+struct LoopSignal {
+    unsigned word;
+    unsigned pad;
+    unsigned long long stats[kPubWords];
+};
+
 // device-side counters
 enum {
     CNT_OVERFLOW = 0,   // bucket inserts dropped by the cap
@@ -263,16 +277,20 @@ struct bbx_solver {
     unsigned *h_max_force = nullptr;
     int *flag = nullptr;                     // requiresHigherLevelUpdate
     int *h_flag = nullptr;                   // pinned: one int read back alone
-    // Loop decisions published to the host as they are taken: the pressure-
-    // force kernel of iteration k stores (seq << 9) | (k << 1) | integrate
-    // into this pinned word (d_signal is its device address) when it reads
-    // the iteration's error; seq numbers the substep.
-    unsigned *h_signal = nullptr, *d_signal = nullptr;
+    // Loop decisions (and, with the integrating one, the substep's
+    // statistics) published to the host as they are taken by the pressure-
+    // force kernel (bbx::LoopSignal; d_signal is its device address).
+    bbx::LoopSignal *h_signal = nullptr, *d_signal = nullptr;
     unsigned loop_seq = 0;
     // substep timing: start and end of the queued work, and the end of the
     // statistics readback; two sets, so that a run of PCISPH substeps can
     // queue the next substep's grid work before it reads this one's
     hipEvent_t ev0[2] = {}, ev1[2] = {}, ev_read[2] = {};
+    // a run of PCISPH substeps marks each boundary between substeps once (the
+    // end of one is the start of the next) in a ring of kMarks events, read
+    // kMarks - 1 substeps later, so the host never waits for a substep's end
+    static constexpr int kMarks = 8;
+    hipEvent_t mark[kMarks] = {};
 
 
     bbx_mlp *mlp = nullptr;
@@ -302,7 +320,10 @@ int distribute(bbx_solver *s, bool force_full, bool buckets, bool with_pressure,
                int feat_mode = 0, const StepConst *st = nullptr);
 // the two halves of distribute: the grid (hash, scan, scatter, move) and the
 // tile tables + bucket build (the latter is where the warm start is chosen)
-int distribute_grid(bbx_solver *s, bool force_full, int append_from);
+// clear2 / clear2_n: a second int range set to zero by the same launch that
+// clears the cell counts (the substep's step block)
+int distribute_grid(bbx_solver *s, bool force_full, int append_from, int *clear2 = nullptr,
+                    size_t clear2_n = 0);
 int build_buckets(bbx_solver *s, bool with_pressure, int feat_mode, const StepConst *st);
 int run_density_only(bbx_solver *s);
 // mode: 0 force only, 1 force + predict(k = 0), 2 force + the warm start's

@@ -39,6 +39,27 @@ int launch_fill_int(bbx_solver *s, int *p, int value, size_t n) {
     return BBX_OK;
 }
 
+// two int ranges set to zero by one launch: blocks [0, nb0) cover a, the
+// rest cover b
+__global__ void k_zero_int2(int *__restrict__ a, size_t na, size_t nb0, int *__restrict__ b, size_t nb) {
+    const size_t blk = blockIdx.x;
+    if (blk < nb0) {
+        const size_t i = blk * blockDim.x + threadIdx.x;
+        if (i < na) a[i] = 0;
+    } else {
+        const size_t i = (blk - nb0) * blockDim.x + threadIdx.x;
+        if (i < nb) b[i] = 0;
+    }
+}
+
+static int launch_zero_int2(bbx_solver *s, int *a, size_t na, int *b, size_t nb) {
+    if (!b || nb == 0) return launch_fill_int(s, a, 0, na);
+    const size_t ba = (na + kBlock - 1) / kBlock, bb = (nb + kBlock - 1) / kBlock;
+    k_zero_int2<<<dim3((unsigned)(ba + bb)), dim3(kBlock), 0, s->stream>>>(a, na, ba, b, nb);
+    BBX_HIP_CHECK(hipGetLastError());
+    return BBX_OK;
+}
+
 // Block-wide exclusive scan of kScanItems items per thread.  Each workgroup
 // scans kBlock*kScanItems elements and writes its total to block_sums.
 __global__ void __launch_bounds__(kBlock) k_scan_block(const int *__restrict__ in,
@@ -85,6 +106,37 @@ __global__ void k_scan_add(int *__restrict__ out, const int *__restrict__ block_
     if (i < n) out[i] += block_off[i / per];
 }
 
+// Second pass of a scan over at most kScanFuseBlocks first-pass blocks:
+// workgroup w adds to its kBlock * kScanItems elements (those of first-pass
+// block w) the sum of the first-pass totals before it, which it reduces
+// itself, so the totals need no scan of their own.
+constexpr int kScanFuseBlocks = 4 * kBlock;
+__global__ void __launch_bounds__(kBlock) k_scan_add_reduce(int *__restrict__ out,
+                                                            const int *__restrict__ block_sums, size_t n) {
+    __shared__ int wave_tot[kBlock / 64];
+    __shared__ int offset;
+    const int w = (int)blockIdx.x;
+    int part = 0;
+    for (int t = (int)threadIdx.x; t < w; t += kBlock) part += block_sums[t];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (lane == 0) wave_tot[wave] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < kBlock / 64; k++) tot += wave_tot[k];
+        offset = tot;
+    }
+    __syncthreads();
+    const int add = offset;
+    const size_t base = (size_t)w * kBlock * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const size_t i = base + (size_t)k * kBlock + threadIdx.x;
+        if (i < n) out[i] += add;
+    }
+}
+
 // Exclusive scan, recursive over block totals.  Needs scan_tmp sized by
 // scan_scratch_elems(n).
 static size_t blocks_for(size_t n) { return (n + (size_t)kBlock * kScanItems - 1) / ((size_t)kBlock * kScanItems); }
@@ -105,7 +157,10 @@ static int scan_rec(bbx_solver *s, const int *in, int *out, size_t n, int *tmp) 
     int *sums_scanned = tmp + b;
     k_scan_block<<<dim3((unsigned)b), dim3(kBlock), 0, s->stream>>>(in, out, sums, n);
     BBX_HIP_CHECK(hipGetLastError());
-    if (b > 1) {
+    if (b > 1 && b <= (size_t)kScanFuseBlocks) {
+        k_scan_add_reduce<<<dim3((unsigned)b), dim3(kBlock), 0, s->stream>>>(out, sums, n);
+        BBX_HIP_CHECK(hipGetLastError());
+    } else if (b > 1) {
         BBX_CHECK(scan_rec(s, sums, sums_scanned, b, tmp + 2 * b));
         size_t blocks = (n + kBlock - 1) / kBlock;
         k_scan_add<<<dim3((unsigned)blocks), dim3(kBlock), 0, s->stream>>>(out, sums_scanned, n);
@@ -785,11 +840,11 @@ __global__ void __launch_bounds__(kBlock) k_density(SimConst c, const float4 *__
 static unsigned nblocks(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // UpdateGridDistribution: full or incremental chain update, then buckets.
-int distribute_grid(bbx_solver *s, bool force_full, int append_from) {
+int distribute_grid(bbx_solver *s, bool force_full, int append_from, int *clear2, size_t clear2_n) {
     SimConst &c = s->sc;
     const int n = s->n;
     const size_t cells = (size_t)c.g.total + 1;
-    BBX_CHECK(launch_fill_int(s, s->cell_count, 0, cells));
+    BBX_CHECK(launch_zero_int2(s, s->cell_count, cells, clear2, clear2_n));
     if (n > 0) {
         {
             BBX_PROF_BEGIN("grid_hash");

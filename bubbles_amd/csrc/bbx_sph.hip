@@ -358,7 +358,8 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            unsigned *__restrict__ max_force,
                                                            LoopState *__restrict__ loop, int k,
                                                            int max_it, int exact, double tol,
-                                                           unsigned *sig, unsigned seq) {
+                                                           LoopSignal *sig, unsigned seq,
+                                                           const int *__restrict__ counters) {
     // (position, q = p / rho*^2) of the neighbours, copied by 16-byte LDS DMA
     __shared__ float4 sx[kTileCap];
     __shared__ TileTab tt;
@@ -383,11 +384,24 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             loop->err[k] = emax;
             if (integrate) loop->done[k] = 1;
-            // tell the waiting host what this iteration decided (a vector
-            // store, system scope, into pinned host memory)
-            if (sig)
-                __hip_atomic_store(sig, (seq << 9) | ((unsigned)k << 1) | (integrate ? 1u : 0u),
+            // tell the waiting host what this iteration decided, and with
+            // the integrating decision the substep's statistics (vector
+            // stores, system scope, into pinned host memory; the counters
+            // were final before the loop began)
+            if (sig) {
+                if (integrate) {
+                    const unsigned long long tag = (unsigned long long)seq << 32;
+                    const unsigned long long v[kPubWords] = {
+                        tag | (emax & 0xffffffffull), tag | (emax >> 32),
+                        tag | (unsigned)counters[CNT_OVERFLOW], tag | (unsigned)counters[CNT_LOST],
+                        tag | (unsigned)counters[CNT_FULL]};
+#pragma unroll
+                    for (int q = 0; q < kPubWords; q++)
+                        __hip_atomic_store(&sig->stats[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                __hip_atomic_store(&sig->word, (seq << 9) | ((unsigned)k << 1) | (integrate ? 1u : 0u),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
     }
     const int tile = tile_of_block();
@@ -671,7 +685,7 @@ int run_pressure_force(bbx_solver *s, const StepConst &st, int k, int max_it, in
     k_pressure_force<<<nblocks(s->n), kBlock, 0, s->stream>>>(
         s->sc, st, nbr_tab(s), s->posA, s->velA, s->force, s->xq, s->pres, s->xstar, s->fp, s->posB,
         s->velB, s->flag, s->max_force, s->loop, k < 0 ? -1 : k, max_it, exact, tol, s->d_signal,
-        s->loop_seq & 0x7fffffu);
+        s->loop_seq & 0x7fffffu, s->counters);
     BBX_HIP_CHECK(hipGetLastError());
     BBX_PROF_END();
     return BBX_OK;

@@ -4,6 +4,7 @@ Tolerances are stated per quantity; the oracle runs the reference method in
 IEEE double on the same (FP32-rounded) inputs.
 """
 import dataclasses
+import time
 
 import numpy as np
 import pytest
@@ -128,8 +129,16 @@ def test_pcisph_run_equals_repeated_substeps(bbx, warm):
         a.attach_mlp(learned.load())
         b.attach_mlp(learned.load())
     one = [a.pcisph_substep(sc.dt, 16, 0.01, bbx.MODE_TEXTBOOK, w) for _ in range(40)]
+    b.synchronize()
+    t0 = time.perf_counter()
     run = b.pcisph_run(sc.dt, 40, 16, 0.01, bbx.MODE_TEXTBOOK, w)
+    wall_ms = (time.perf_counter() - t0) * 1e3
     assert len(run) == 40
+    # each substep's device time, between the marks at its boundaries: the
+    # substeps follow one another, so the times add up to at most the wall
+    # time of the run
+    assert all(x.gpu_ms > 0 for x in run)
+    assert sum(x.gpu_ms for x in run) <= wall_ms * 1.01 + 0.1
     assert [s.iterations for s in one] == [s.iterations for s in run]
     assert [s.warm_started for s in one] == [s.warm_started for s in run]
     assert [s.max_density_error for s in one] == [s.max_density_error for s in run]
@@ -141,11 +150,12 @@ def test_pcisph_run_equals_repeated_substeps(bbx, warm):
 
 
 def test_loop_decisions_without_the_published_signal(bbx, monkeypatch):
-    """The host learns each loop decision from the pinned word the
-    pressure-force kernel stores it in; with that word switched off
-    (BBX_LOOP_SIGNAL=0) it reads the done flags back through copies instead.
-    Both give the same substeps bit for bit, plain and warm-started, and the
-    cap's range is 1..256."""
+    """The host learns each loop decision, and with the integrating one the
+    substep's statistics, from the pinned block the pressure-force kernel
+    stores them in; with that block switched off (BBX_LOOP_SIGNAL=0) it reads
+    the done flags and the statistics back through copies instead.  Both
+    give the same substeps bit for bit and the same statistics, plain and
+    warm-started, and the cap's range is 1..256."""
     from bubbles_amd import learned
 
     sc = small_dam(seed=5)
@@ -156,7 +166,10 @@ def test_loop_decisions_without_the_published_signal(bbx, monkeypatch):
             s = bbx.Solver(sc)
             s.attach_mlp(learned.load())
             st = s.pcisph_run(sc.dt, 30, 256, 0.01, bbx.MODE_TEXTBOOK, w)
-            out.append(([x.iterations for x in st], [x.max_density_error for x in st],
+            st.append(s.pcisph_substep(sc.dt, 256, 0.01, bbx.MODE_REFERENCE_EXACT, w))
+            keys = ("iterations", "max_density_error", "full_redistribution", "bucket_overflows",
+                    "lost_particles", "warm_started")
+            out.append(([x.iterations for x in st], [tuple(getattr(x, k) for k in keys) for x in st],
                         s.get("position"), s.get("velocity")))
             s.close()
         return out
