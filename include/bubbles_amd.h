@@ -193,14 +193,16 @@ int bbx_stage_nonpressure(bbx_solver *s);      /* ComputeNonPressureForceGPU (:6
  * One PCISPH substep of fixed dt: AdvanceTimeStep (src/solvers/pcisph_solver3.cpp:42-65). */
 int bbx_pcisph_substep(bbx_solver *s, double dt, int max_it, double tol, int mode, int warm,
                        bbx_stats *stats);
-/* PciSphSolver3::Advance (src/solvers/pcisph_solver3.cpp:67-89): reference
- * substep rule with scale 5. */
 /* `count` consecutive substeps of fixed dt, as `count` calls of
  * bbx_pcisph_substep (the reference's frame loop calls AdvanceTimeStep the
  * same way, src/solvers/pcisph_solver3.cpp:67-89) without returning to the
- * caller in between; stats (may be NULL) receives one entry per substep. */
+ * caller in between; stats (may be NULL) receives one entry per substep,
+ * gpu_ms being the device time between the substep's boundaries with its
+ * neighbours (the substeps follow one another on the device). */
 int bbx_pcisph_run(bbx_solver *s, double dt, int count, int max_it, double tol, int mode, int warm,
                    bbx_stats *stats);
+/* PciSphSolver3::Advance (src/solvers/pcisph_solver3.cpp:67-89): reference
+ * substep rule with scale 5. */
 int bbx_pcisph_advance(bbx_solver *s, double interval, int max_it, double tol, int mode,
                        int warm, int *substeps);
 /* SphSolver3 AdvanceTimeStep / Advance (src/solvers/sph_solver3.cpp:48-93),
