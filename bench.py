@@ -328,6 +328,8 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=2,
                     help="timed oracle substeps on one thread (after 1 warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the C1 WCSPH run on the GPU (profiling runs of the PCISPH path)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the PCISPH path "
@@ -370,7 +372,7 @@ def main():
             h["seed"] = args.held_out_seed
             h["max_it"] = args.max_it
             whole["held_out_seed"] = h
-    c1_gpu = run_c1_gpu() if args.scene == "dam_break" else None
+    c1_gpu = run_c1_gpu() if args.scene == "dam_break" and not args.no_c1 else None
     gbs, _ = abi.stream_copy(2.0 ** 30, 20)
     roof = roofline_for(head, sc.n, gbs)
     kbar = head["mean_iterations"]

@@ -31,7 +31,8 @@ COPY_BYTES = float(2 ** 30)
 
 SHORT = {
     "k_hash": "grid_hash", "k_tile_tables": "tile_tables", "k_scatter": "grid_scatter", "k_rank_move": "grid_rank_move",
-    "k_scan_block": "grid_scan", "k_scan_add": "grid_scan_add",
+    "k_scan_block": "grid_scan", "k_scan_add": "grid_scan_add", "k_scan_add_reduce": "grid_scan_add",
+    "k_zero_int2": "grid_clear",
     "k_buckets_density<false, 0>": "buckets_density", "k_buckets_density<true, 0>": "buckets_density_tait",
     "k_buckets_density<false, 1>": "buckets_density_features",
     "k_buckets_density<false, 2>": "buckets_density_warm",
