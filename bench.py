@@ -24,7 +24,9 @@ substeps from rest, plain vs learned, to tolerance and at the 8-iteration cap,
 plus the learned run with the network used on every substep, and the same
 three to-tolerance runs on a held-out seed (2000); no events.  The flow is
 chaotic, so iteration counts of two runs diverge after a few hundred
-substeps: the second seed shows how much a ratio moves with the trajectory.
+substeps: `whole_run.per_seed` lists the plain/AUTO ratio of the whole run
+on the benchmark seed and on all eight held-out seeds (2000-2007), and
+their mean.
 
 The CPU baseline times the FP64 oracle (a restatement of the reference CPU
 path) on a bounded sample of the same workload, and on the 16,384-particle
@@ -313,6 +315,9 @@ def main():
                     help="iteration cap of the headline pair and the to-tolerance runs (at most 256)")
     ap.add_argument("--held-out-seed", type=int, default=2000,
                     help="seed of the second whole-run scene (-1: skip)")
+    ap.add_argument("--more-seeds", default="2001,2002,2003,2004,2005,2006,2007",
+                    help="further held-out seeds whose whole runs (plain and AUTO) enter the "
+                         "per-seed ratios ('' to skip)")
     ap.add_argument("--tol", type=float, default=0.01)
     ap.add_argument("--run-steps", type=int, default=500,
                     help="length of the whole-run pairs timed from substep 0 (0: skip)")
@@ -372,6 +377,33 @@ def main():
             h["seed"] = args.held_out_seed
             h["max_it"] = args.max_it
             whole["held_out_seed"] = h
+        # the same plain/AUTO pair over further held-out seeds: the whole
+        # run's ratio follows a chaotic flow, so one seed is one sample
+        seeds, ratios, conv, worst = [], [], [], []
+        if args.scene == "dam_break":
+            for key, seed in (("to_tolerance", 2), ("held_out_seed", args.held_out_seed)):
+                if key in whole and seed >= 0:
+                    w = whole[key]
+                    seeds.append(seed)
+                    ratios.append(w["speedup"])
+                    conv.append((w["plain"]["converged_fraction"], w["learned"]["converged_fraction"]))
+                    worst.append((w["plain"]["max_density_error"], w["learned"]["max_density_error"]))
+            for seed in [int(x) for x in args.more_seeds.split(",") if x.strip()]:
+                sc3 = scenes.dam_break(args.nx, args.ny, args.nz, spacing=0.01, tank=(2.0, 1.0, 2.0), seed=seed)
+                w = pair_summary(*run_pair(args, sc3, args.max_it, args.run_steps, 0, False))
+                seeds.append(seed)
+                ratios.append(w["speedup"])
+                conv.append((w["plain"]["converged_fraction"], w["learned"]["converged_fraction"]))
+                worst.append((w["plain"]["max_density_error"], w["learned"]["max_density_error"]))
+        if seeds:
+            whole["per_seed"] = {
+                "seeds": seeds, "speedup": ratios,
+                "mean_speedup": round(sum(ratios) / len(ratios), 4),
+                "plain_converged_fraction": [c[0] for c in conv],
+                "learned_converged_fraction": [c[1] for c in conv],
+                "plain_max_density_error": [e[0] for e in worst],
+                "learned_max_density_error": [e[1] for e in worst],
+                "max_it": args.max_it}
     c1_gpu = run_c1_gpu() if args.scene == "dam_break" and not args.no_c1 else None
     gbs, _ = abi.stream_copy(2.0 ** 30, 20)
     roof = roofline_for(head, sc.n, gbs)
