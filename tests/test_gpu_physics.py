@@ -325,3 +325,23 @@ def test_staging_paths_bit_identical(bbx, monkeypatch, env):
     got = _staging_run(bbx, sc_p, sc_s)
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)
+
+
+def test_published_counters_match_the_copied_ones(bbx, monkeypatch):
+    """The counters a PCISPH substep reports with its integrating decision
+    (LoopSignal) equal those the copy path reads back (BBX_LOOP_SIGNAL=0):
+    150 particles within the radius of each other drop 150 * 50 bucket
+    entries in the substep's bucket build (src/core/particle.h:45-51)."""
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-0.004, 0.004, size=(150, 3)).astype(np.float32).astype(np.float64)
+    sc = dataclasses.replace(scenes.dam_break(2, 2, 2), positions=pts, velocities=np.zeros_like(pts))
+    res = []
+    for sig in ("1", "0"):
+        monkeypatch.setenv("BBX_LOOP_SIGNAL", sig)
+        s = bbx.Solver(sc)
+        st = s.pcisph_substep(sc.dt, 8, 0.01, bbx.MODE_TEXTBOOK)
+        res.append((st.bucket_overflows, st.lost_particles, st.full_redistribution, st.iterations,
+                    st.max_density_error))
+        s.close()
+    assert res[0] == res[1]
+    assert res[0][0] == 150 * 50
