@@ -250,32 +250,34 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
     // predicted positions, copied by 16-byte LDS DMA (the w lane is the
     // unused fourth word of x*)
     __shared__ float4 sx[kTileCap];
-    __shared__ TileTab tt;
     __shared__ unsigned long long wmax[kPpThreads / 64];
     if (k > 0 && loop->done[k - 1]) return;  // uniform early exit
     const int tile = tile_of_block();
-    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
-    __syncthreads();
-    const bool staged = tt.staged != 0;
+    // Each wave reads the tile table into its own lanes, so no workgroup
+    // barrier stands between the table and the staging: the particle's own
+    // data and its first quads (LDS-index table; reloaded from the storage-
+    // index table for an unstaged tile) are requested with it and stay in
+    // flight while the staging is issued.
+    // The loads are unconditional (a lane past the last particle reads the
+    // last one's data and ignores it): with no branch between them and the
+    // table word's first use, waiting for the table waits for it alone.
+    const int tw = tab_load_wave(nb.tab + (size_t)tile * kTabInts);
     const int pl = threadIdx.x >> 1, half = threadIdx.x & 1;
     const int i = tile * kBlock + pl;
     const bool valid = i < c.n;
-    // the particle's own data and this lane's first quads do not depend on
-    // the staging: request them first so their latency overlaps it
-    NbrQuads<2> first{};
-    int cnt = 0;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), pi = xi;
-    float pold = 0.f;
-    if (valid) {
-        first = nbr_prefetch<2, 2>(nb.n16, nb.n32, staged, i, c.stride, half);
-        cnt = nb.cnt[i];
-        xi = xstar[i];
-        pi = pos[i];
-        if (!st.first && half == 0) pold = pres[i];
-    }
+    const int ic = valid ? i : c.n - 1;  // the kernel runs only for n > 0
+    NbrQuads<2> first = nbr_prefetch<2, 2>(nb.n16, nb.n32, true, ic, c.stride, half);
+    const int cnt_ic = nb.cnt[ic];
+    const float4 xi = xstar[ic], pi = pos[ic];
+    const int cnt = valid ? cnt_ic : 0;
+    const float pown = pres[ic];
+    const float pold = (!st.first && half == 0) ? pown : 0.f;
+    const bool staged = tab_word(tw, 1) != 0;
     if (staged) {
-        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
+        tab_stage_dma_wave(tw, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
         lds_dma_wait();
+    } else if (valid) {
+        first = nbr_prefetch<2, 2>(nb.n16, nb.n32, false, i, c.stride, half);
     }
     __syncthreads();
     unsigned long long ebits = 0ull;
@@ -362,22 +364,36 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
                                                            const int *__restrict__ counters) {
     // (position, q = p / rho*^2) of the neighbours, copied by 16-byte LDS DMA
     __shared__ float4 sx[kTileCap];
-    __shared__ TileTab tt;
     if (k > 0 && loop->done[k - 1]) {
         // an earlier iteration integrated: pass the flag on and exit (uniform)
         if (blockIdx.x == 0 && threadIdx.x == 0) loop->done[k] = 1;
         return;
     }
-    __shared__ unsigned long long emax;
+    const int tile = tile_of_block();
+    const int lane = (int)threadIdx.x & 63;
+    // Each wave reads the tile table and the iteration's error slots into its
+    // own lanes and reduces the slots itself, so no workgroup barrier stands
+    // before the staging.  The particle's own data and its first quads (LDS-
+    // index table; reloaded from the storage-index table for an unstaged
+    // tile) are requested with them, unconditionally (a lane past the last
+    // particle reads the last one's data and ignores it), so that waiting for
+    // the table or the slots waits for them alone.
+    const int tw = tab_load_wave(nb.tab + (size_t)tile * kTabInts);
+    const unsigned long long slot_err = (k >= 0 && lane < kSlots) ? loop->slot[k][lane] : 0ull;
+    const int i = tile * kBlock + threadIdx.x;
+    const bool valid = i < c.n;
+    const int ic = valid ? i : c.n - 1;  // the kernel runs only for n > 0
+    NbrQuads<kNbrDepth> first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, true, ic, c.stride);
+    const int cnt_ic = nb.cnt[ic];
+    const float4 xi = pos[ic];
+    const float qi = xq[ic].w;
+    const float4 fi = force[ic], vi = vel[ic];
+    const int cnt = valid ? cnt_ic : 0;
+    const bool staged = tab_word(tw, 1) != 0;
     bool integrate = false;
     if (k >= 0) {
         // the iteration's FP64 max error: max over the slots
-        if (threadIdx.x < 64) {
-            unsigned long long v = threadIdx.x < kSlots ? loop->slot[k][threadIdx.x] : 0ull;
-            v = wave_max_u64(v);
-            if (threadIdx.x == 0) emax = v;
-        }
-        __syncthreads();
+        const unsigned long long emax = wave_max_u64(slot_err);
         const double e = __longlong_as_double((long long)emax);
         const double ratio = e / c.rho0d;
         integrate = exact || k == max_it - 1 || fabs(ratio) < tol;
@@ -404,27 +420,11 @@ __global__ void __launch_bounds__(kBlock) k_pressure_force(SimConst c, StepConst
             }
         }
     }
-    const int tile = tile_of_block();
-    tab_load_shared(nb.tab + (size_t)tile * kTabInts, tt);
-    __syncthreads();
-    const bool staged = tt.staged != 0;
-    const int i = tile * kBlock + threadIdx.x;
-    const bool valid = i < c.n;
-    NbrQuads<kNbrDepth> first{};
-    int cnt = 0;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), fi = xi, vi = xi;
-    float qi = 0.f;
-    if (valid) {  // independent of the staging: requested before it
-        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, staged, i, c.stride);
-        cnt = nb.cnt[i];
-        xi = pos[i];
-        qi = xq[i].w;
-        fi = force[i];
-        vi = vel[i];
-    }
     if (staged) {
-        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(xq + j, sx + e); });
+        tab_stage_dma_wave(tw, [&](int j, int e) { lds_dma16(xq + j, sx + e); });
         lds_dma_wait();
+    } else {
+        first = nbr_prefetch<kNbrDepth>(nb.n16, nb.n32, false, ic, c.stride);
     }
     __syncthreads();
     unsigned fbits = 0u;

@@ -13,6 +13,7 @@
 // 32-bit storage indices and reads neighbours from global memory.
 #pragma once
 
+#include <cstddef>
 #include <type_traits>
 
 #include "bbx_device.h"
@@ -52,6 +53,10 @@ struct TileTab {
 constexpr int kTabWords = 3 + 3 * kMaxIvl + 9;
 static_assert(kTabWords <= kTabInts, "tile table stride");
 static_assert(sizeof(TileTab) == kTabWords * sizeof(int), "TileTab layout");
+static_assert(offsetof(TileTab, staged) == 1 * sizeof(int) && offsetof(TileTab, start) == 3 * sizeof(int) &&
+                  offsetof(TileTab, len) == (3 + kMaxIvl) * sizeof(int) &&
+                  offsetof(TileTab, off) == (3 + 2 * kMaxIvl) * sizeof(int),
+              "TileTab word indices (tab_word)");
 
 // the block's threads copy a __shared__ TileTab to global memory
 __device__ __forceinline__ void tab_store(int *__restrict__ g, const TileTab &t) {
@@ -353,6 +358,30 @@ __device__ __forceinline__ void tab_stage_dma(const TileTab &t, Copy copy) {
     const int nivl = t.n_ivl;
     for (int k = 0; k < nivl; k++) {
         const int len = t.len[k], s0 = t.start[k], o0 = t.off[k];
+        for (int b = w0; b < len; b += (int)blockDim.x)
+            if (b + lane < len) copy(s0 + b + lane, o0 + b);
+    }
+}
+
+// The tile table held in registers, one word per lane (lane w < kTabWords
+// holds word w of the TileTab layout), loaded by every wave for itself so
+// that no workgroup barrier is needed before the staging.  tab_word reads a
+// word by a wave-uniform index.
+__device__ __forceinline__ int tab_load_wave(const int *__restrict__ g) {
+    const int lane = (int)threadIdx.x & 63;
+    return lane < kTabWords ? g[lane] : 0;
+}
+__device__ __forceinline__ int tab_word(int tw, int w) { return __builtin_amdgcn_readlane(tw, w); }
+
+// tab_stage_dma from a table held in registers (tab_load_wave)
+template <typename Copy>
+__device__ __forceinline__ void tab_stage_dma_wave(int tw, Copy copy) {
+    const int lane = (int)threadIdx.x & 63;
+    const int w0 = (int)threadIdx.x - lane;
+    const int nivl = tab_word(tw, 0);
+    for (int k = 0; k < nivl; k++) {
+        const int s0 = tab_word(tw, 3 + k), len = tab_word(tw, 3 + kMaxIvl + k),
+                  o0 = tab_word(tw, 3 + 2 * kMaxIvl + k);
         for (int b = w0; b < len; b += (int)blockDim.x)
             if (b + lane < len) copy(s0 + b + lane, o0 + b);
     }
