@@ -445,6 +445,11 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
     __shared__ float4 sx[kTileCap + 8];
     __shared__ TileTab tt;
     const int tile = tile_of_block();
+    // The tile table is read first, into each wave's lanes: the staging
+    // below then waits for nothing but it, not for the row bounds requested
+    // in between, and needs no workgroup barrier (the LDS copy that the rest
+    // of the kernel reads is visible after the barrier behind the staging).
+    const int tw = tab_load_wave(tile_tab + (size_t)tile * kTabInts);
     const int i = tile * kBlock + threadIdx.x;
     const bool valid = i < c.n;
     const int hc = valid ? home[i] : -1;
@@ -468,16 +473,35 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
         O[r] = 0u;
     }
     if (valid) pi = pos[i];
+    // All 36 row-bound loads are issued before any is used (indices clamped
+    // into start[0 .. total]; rows outside the grid and cells past the x
+    // bounds read a valid word and ignore it), and the staging is issued
+    // behind them, so the latencies overlap instead of following one another
+    // row by row.
+    int b[9][4];
     if (hc >= 0) {
         unlinear_fast(c.g, c.inv_nx, c.inv_nxy, hc, u);
+        const int last = (int)c.g.total;
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
+            const bool in = yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+            const int cx = in ? u[0] + yy * nx + zz * nxy : 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) b[r][q] = start[min(max(cx - 1 + q, 0), last)];
+        }
+    }
+    if (threadIdx.x < kTabWords) reinterpret_cast<int *>(&tt)[threadIdx.x] = tw;
+    const bool staged = tab_word(tw, 1) != 0;
+    if (staged) tab_stage_dma_wave(tw, [&](int j, int e) { lds_dma16(pos + j, sx + e); });
+    if (hc >= 0) {
 #pragma unroll
         for (int r = 0; r < 9; r++) {
             const int yy = u[1] + (r % 3) - 1, zz = u[2] + (r / 3) - 1;
             if (yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-                const int cx = u[0] + yy * nx + zz * nxy;
-                const int b1 = start[cx], b2 = start[cx + 1];
-                const int b0 = u[0] > 0 ? start[cx - 1] : b1;
-                const int b3 = u[0] + 1 < nx ? start[cx + 2] : b2;
+                const int b1 = b[r][1], b2 = b[r][2];
+                const int b0 = u[0] > 0 ? b[r][0] : b1;
+                const int b3 = u[0] + 1 < nx ? b[r][3] : b2;
                 A[r] = b0;
                 O[r] = ((unsigned)(b1 - b0) << 8) | ((unsigned)(b2 - b0) << 16) | ((unsigned)(b3 - b0) << 24);
                 narrow = narrow && (b3 - b0 <= c.row_max) && (b1 - b0 <= 32) && (b2 - b1 <= 32) &&
@@ -485,13 +509,7 @@ __global__ void __launch_bounds__(kBlock) k_buckets_density(SimConst c,
             }
         }
     }
-    tab_load_shared(tile_tab + (size_t)tile * kTabInts, tt);
-    __syncthreads();
-    const bool staged = tt.staged != 0;
-    if (staged) {
-        tab_stage_dma(tt, [&](int j, int e) { lds_dma16(pos + j, sx + e); });
-        lds_dma_wait();
-    }
+    lds_dma_wait();
     __syncthreads();
     if (kFeat != 2 && !valid) return;  // the inference below needs the whole block at its barriers
     float fv[kFeat ? BBX_NFEAT : 1];
