@@ -239,6 +239,9 @@ __global__ void __launch_bounds__(kNpThreads) __attribute__((amdgpu_waves_per_eu
 // twice the waves of one lane per particle at the same LDS, to hide the
 // gather's latency.
 constexpr int kPpThreads = 2 * kBlock;
+// Quads in flight per lane in the predicted-density gather: 1 measured 2%
+// faster than 2 over the whole 1M run, and 3 slower still.
+constexpr int kPpDepth = 1;
 __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, StepConst st, double delta,
                                                                  NbrTab nb,
                                                                  const float4 *__restrict__ xstar,
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
     const int i = tile * kBlock + pl;
     const bool valid = i < c.n;
     const int ic = valid ? i : c.n - 1;  // the kernel runs only for n > 0
-    NbrQuads<2> first = nbr_prefetch<2, 2>(nb.n16, nb.n32, true, ic, c.stride, half);
+    NbrQuads<kPpDepth> first = nbr_prefetch<kPpDepth, 2>(nb.n16, nb.n32, true, ic, c.stride, half);
     const int cnt_ic = nb.cnt[ic];
     const float4 xi = xstar[ic], pi = pos[ic];
     const int cnt = valid ? cnt_ic : 0;
@@ -277,7 +280,7 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
         tab_stage_dma_wave(tw, [&](int j, int e) { lds_dma16(xstar + j, sx + e); });
         lds_dma_wait();
     } else if (valid) {
-        first = nbr_prefetch<2, 2>(nb.n16, nb.n32, false, i, c.stride, half);
+        first = nbr_prefetch<kPpDepth, 2>(nb.n16, nb.n32, false, i, c.stride, half);
     }
     __syncthreads();
     unsigned long long ebits = 0ull;
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__(kPpThreads) k_predict_pressure(SimConst c, Ste
         float sum = 0.f, comp = 0.f;
         with_staging(staged, [&](auto tag) {
             constexpr bool kS = decltype(tag)::value;
-            for_each_neighbour_from<2, 2>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
+            for_each_neighbour_from<kPpDepth, 2>(first, nb.n16, nb.n32, kS, i, cnt, c.stride, [&](int j, bool live) {
                 float3 xj;
                 if (kS) {
                     const float4 a = lds_at(sx, j);
